@@ -1,0 +1,181 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's cpu_baseline leg.  The product path
+(nemo_amd/) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_u64 = C.POINTER(C.c_uint64)
+_i32 = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_force.argtypes = [C.c_int, _d, _d, C.c_int, _d, C.c_double, C.c_double, C.c_int,
+                                _d, _d, C.c_double, _d, _d, _i64, _i64]
+        L.orc_force.restype = C.c_int
+        L.orc_direct.argtypes = [C.c_int, _d, _d, C.c_int, _i64, C.c_double, _d, _d]
+        L.orc_direct.restype = None
+        L.orc_step_force.argtypes = [C.c_int, _d, _d, _d, _d, _d, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, _d, _d, C.c_double, _i64, _i64]
+        L.orc_step_force.restype = C.c_int
+        L.orc_step_drift.argtypes = [C.c_int, _d, _d, _d, C.c_double]
+        L.orc_step_drift.restype = None
+        L.orc_diagnostics.argtypes = [C.c_int, _d, _d, _d, _d, _d, _d]
+        L.orc_morton.argtypes = [C.c_int, _d, _d, C.c_double, _u64]
+        L.orc_tree_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _d, _d, _d, _d, C.c_int, C.c_int]
+        L.orc_tree_build.restype = C.c_int
+        L.orc_tree_free.argtypes = [C.c_void_p]
+        L.orc_tree_ncell.argtypes = [C.c_void_p]
+        L.orc_tree_dump.argtypes = [C.c_void_p, _d, _d, _d, _i32, _i32]
+        L.orc_tree_dump.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def force(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fcells=2.0, tpos=None):
+    """hackforce force_calc: returns (phi, acc, n2btot, nbctot, rmin, rsize)."""
+    mass = np.ascontiguousarray(mass, np.float64)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = len(mass)
+    if rmin is None:
+        rmin = np.full(3, -rsize / 2.0)
+    rmin = np.array(rmin, np.float64)
+    rs = np.array([rsize], np.float64)
+    nt = n if tpos is None else len(tpos)
+    phi = np.zeros(nt)
+    acc = np.zeros((nt, 3))
+    a = np.zeros(1, np.int64)
+    b = np.zeros(1, np.int64)
+    tp = None
+    if tpos is not None:
+        tpos = np.ascontiguousarray(tpos, np.float64).reshape(-1, 3)
+        tp = _p(tpos)
+    err = lib().orc_force(n, _p(mass), _p(pos), nt, tp, tol, eps, int(usequad), _p(rmin), _p(rs),
+                          fcells, _p(phi), _p(acc), _p(a, _i64), _p(b, _i64))
+    if err:
+        raise RuntimeError("oracle tree error %d" % err)
+    return phi, acc, int(a[0]), int(b[0]), rmin, float(rs[0])
+
+
+def direct(mass, pos, eps, idx=None):
+    mass = np.ascontiguousarray(mass, np.float64)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = len(mass)
+    if idx is None:
+        nt, ip = n, None
+    else:
+        idx = np.ascontiguousarray(idx, np.int64)
+        nt, ip = len(idx), _p(idx, _i64)
+    phi = np.zeros(nt)
+    acc = np.zeros((nt, 3))
+    lib().orc_direct(n, _p(mass), _p(pos), nt, ip, eps, _p(phi), _p(acc))
+    return phi, acc
+
+
+def morton(pos, rmin, rsize):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    keys = np.zeros(len(pos), np.uint64)
+    rmin = np.ascontiguousarray(rmin, np.float64)
+    lib().orc_morton(len(pos), _p(pos), _p(rmin), rsize, _p(keys, _u64))
+    return keys
+
+
+def tree_cells(mass, pos, rmin, rsize, usequad=True, maxcell=None):
+    """Cells of the reference tree in DFS pre-order: dict of arrays."""
+    mass = np.ascontiguousarray(mass, np.float64)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = len(mass)
+    rmin = np.array(rmin, np.float64)
+    rs = np.array([rsize], np.float64)
+    maxcell = maxcell or 2 * n + 64
+    h = C.c_void_p()
+    err = lib().orc_tree_build(C.byref(h), n, _p(mass), _p(pos), _p(rmin), _p(rs), maxcell, int(usequad))
+    try:
+        if err:
+            raise RuntimeError("oracle tree error %d" % err)
+        nc = lib().orc_tree_ncell(h)
+        m = np.zeros(nc)
+        p = np.zeros((nc, 3))
+        q = np.zeros((nc, 9))
+        lv = np.zeros(nc, np.int32)
+        nb = np.zeros(nc, np.int32)
+        k = lib().orc_tree_dump(h, _p(m), _p(p), _p(q), _p(lv, _i32), _p(nb, _i32))
+        return dict(mass=m[:k], pos=p[:k], quad=q[:k], level=lv[:k], nbody=nb[:k], rmin=rmin, rsize=float(rs[0]))
+    finally:
+        lib().orc_tree_free(h)
+
+
+class Leapfrog:
+    """hackcode1 stepsystem() state machine (code.c:57-64,159-191)."""
+
+    def __init__(self, mass, pos, vel, tol=1.0, eps=0.05, usequad=False, freq=32.0, fcells=2.0):
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        self.pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3).copy()
+        self.vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3).copy()
+        n = len(self.mass)
+        self.acc = np.zeros((n, 3))
+        self.phi = np.zeros(n)
+        self.nstep = np.zeros(1, np.int32)
+        self.tol, self.eps, self.usequad, self.freq, self.fcells = tol, eps, usequad, freq, fcells
+        self.rmin = np.full(3, -2.0)
+        self.rsize = np.array([4.0])
+        self.tnow = 0.0
+
+    def force(self):
+        """Force pass of stepsystem (with the second-order velocity correction
+        when nstep > 0).  Returns (n2b, nbc); the state is now at output time."""
+        n = len(self.mass)
+        a = np.zeros(1, np.int64)
+        b = np.zeros(1, np.int64)
+        err = lib().orc_step_force(n, _p(self.mass), _p(self.pos), _p(self.vel), _p(self.acc),
+                                   _p(self.phi), int(self.nstep[0]), 1.0 / self.freq, self.tol,
+                                   self.eps, int(self.usequad), _p(self.rmin), _p(self.rsize),
+                                   self.fcells, _p(a, _i64), _p(b, _i64))
+        if err:
+            raise RuntimeError("oracle tree error %d" % err)
+        return int(a[0]), int(b[0])
+
+    def diagnostics(self):
+        d = np.zeros(36)
+        lib().orc_diagnostics(len(self.mass), _p(self.mass), _p(self.pos), _p(self.vel), _p(self.acc),
+                              _p(self.phi), _p(d))
+        return d
+
+    def drift(self):
+        dt = 1.0 / self.freq
+        lib().orc_step_drift(len(self.mass), _p(self.pos), _p(self.vel), _p(self.acc), dt)
+        self.nstep[0] += 1
+        self.tnow = self.tnow + dt
+
+    def step(self):
+        """One full step; returns (diag[36], n2b, nbc) with diag at output time."""
+        n2b, nbc = self.force()
+        d = self.diagnostics()
+        self.drift()
+        return d, n2b, nbc
