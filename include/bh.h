@@ -1,0 +1,118 @@
+/*
+ * bh.h -- C ABI of the MI355X Barnes-Hut gravity engine (libnemo_bh.so).
+ *
+ * Drop-in for the force path of NEMO's hackforce/hackcode1.  Each entry
+ * point names the reference interface it replaces (paths relative to the
+ * NEMO source tree):
+ *
+ *   bh_force        hackforce force_calc()            src/nbody/evolve/hackcode/hackcode1/hackforce.c:177-215
+ *                   = maketree() + hackgrav() loop     load.c:32-62, grav.c:22-32
+ *   bh_upload/bh_eval/bh_download
+ *                   the same force_calc() split so the snapshot stays resident on the device
+ *   bh_direct       directcode hackgrav()              src/nbody/evolve/directcode/grav.c:23-65
+ *   bh_lf_*         hackcode1 stepsystem()/diagnostics()  code.c:159-191, code_io.c:351-388
+ *   bh_batch_force  many independent hackforce calls in one launch (no reference counterpart:
+ *                   the reference runs independent processes, src/scripts/nemo.bench:69-90)
+ *   bh_mlp_*        learned far-field correction (no reference counterpart)
+ *   bh_mkplummer    mkplummer's sampling recipe     src/nbody/init/mkplummer.c:257-422 (own RNG)
+ *
+ * Conventions: caller-owned host buffers, FP64 at the boundary (the snapshot
+ * RealType), positions as [n][3].  G = 1.  Functions return a bh_status and
+ * never exit(); bh_last_error() gives the message.  One context per host
+ * thread; a context is not thread safe; contexts are independent.
+ */
+#ifndef NEMO_BH_H
+#define NEMO_BH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BH_OK = 0,
+    BH_ERR_ARG = 1,          /* bad argument (null pointer, n < 1, tol < 0, ...)        */
+    BH_ERR_DEVICE = 2,       /* HIP runtime error or no device                          */
+    BH_ERR_NOMEM = 3,        /* device allocation failed                                 */
+    BH_ERR_NONFINITE = 4,    /* NaN/Inf position: the reference loops forever in expandbox */
+    BH_ERR_CAPACITY = 5,     /* an internal capacity (walk frontier) was exceeded        */
+    BH_ERR_STATE = 6,        /* call out of order (eval before upload, ...)              */
+    BH_ERR_INJECTED = 7      /* failure forced by bh_debug_inject (tests of error paths) */
+} bh_status;
+
+typedef enum {
+    BH_MODE_FAST = 0,    /* group-wise walk, FP32 split-precision interactions (the product path) */
+    BH_MODE_STRICT = 1,  /* per-particle walk in FP64 with the reference's exact decisions/order  */
+    BH_MODE_DIRECT = 2   /* FP64 direct sum over all massive bodies (tol -> 0 limit)              */
+} bh_mode;
+
+typedef struct {
+    double tol;          /* opening parameter: open iff tol^2 |com-x|^2 < size^2 (grav.c:122-131) */
+    double eps;          /* Plummer softening */
+    int usequad;         /* 1: quadrupole cells (hackforce_qp) */
+    int mode;            /* bh_mode */
+    double rsize;        /* initial box side (hackforce rsize=4.0) */
+    double rmin[3];      /* initial lower corner, used when rmin_given */
+    int rmin_given;      /* 0: rmin = -rsize/2 in every dimension (hackforce.c:189-193) */
+    int group_size;      /* fast mode: particles per group (0 = tuned default, else 64 or 128) */
+    int learned;         /* fast mode: apply the loaded MLP correction to accepted cells */
+    int keep_box;        /* bh_eval: start from the box of the previous call (hackcode1 semantics) */
+} bh_params;
+
+typedef struct {
+    int64_t n2btot;      /* body-body interactions (64-bit; the reference's int overflows) */
+    int64_t nbctot;      /* body-cell interactions */
+    int64_t ncell;       /* cells in the tree */
+    int64_t deep_buckets;/* depth-21 cells holding > 1 body (deeper splits of the reference are merged) */
+    double rmin[3];      /* final box */
+    double rsize;
+    double t_tree_ms;    /* device time: box, keys, sort, topology, moments */
+    double t_walk_ms;    /* device time: walk + interactions (+ MLP) */
+    int64_t n_mlp;       /* learned corrections applied */
+} bh_stats;
+
+typedef struct bh_ctx bh_ctx;
+
+void bh_default_params(bh_params *p);
+
+bh_status bh_create(bh_ctx **ctx, int device);
+void bh_destroy(bh_ctx *ctx);
+const char *bh_last_error(const bh_ctx *ctx);
+const char *bh_version(void);
+
+/* hackforce force_calc on host buffers.  ntest == 0 / tpos == NULL: the
+ * mass set is the test set and each body skips itself (pointer identity in
+ * grav.c:356); otherwise every test point sees every massive body.
+ * Zero-mass bodies are not loaded into the tree (load.c:56-60). */
+bh_status bh_force(bh_ctx *ctx, const bh_params *p, int64_t nmass, const double *mass, const double *pos,
+                   int64_t ntest, const double *tpos, double *phi, double *acc, bh_stats *st);
+
+/* Resident variant: upload once, evaluate many times (timing, leapfrog). */
+bh_status bh_upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos,
+                    int64_t ntest, const double *tpos);
+bh_status bh_eval(bh_ctx *ctx, const bh_params *p, bh_stats *st);
+bh_status bh_download(bh_ctx *ctx, double *phi, double *acc);
+/* Evaluate FP64 direct sum on the device for a subset of the test set. */
+bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, double *phi, double *acc);
+/* Synchronise and return the device time of the last bh_eval kernels. */
+bh_status bh_sync(bh_ctx *ctx);
+
+/* Device tree inspection (tests): Morton keys of the uploaded mass set in
+ * input order, and the cells in DFS pre-order. */
+bh_status bh_debug_keys(bh_ctx *ctx, uint64_t *keys);
+bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, double *mass, double *com,
+                         double *quad6, int64_t *ncell);
+/* Force the next call to fail with BH_ERR_INJECTED (error-path tests). */
+void bh_debug_inject(bh_ctx *ctx, int code);
+
+/* Synthetic inputs: equal-mass Plummer sphere in virial units (G=M=1,
+ * E=-1/4) following mkplummer's recipe with mfrac=0.999, rfrac=22.8042468,
+ * zero centre of mass; counter-based RNG keyed by seed, so the result does
+ * not depend on thread count.  vel may be NULL. */
+void bh_mkplummer(int64_t n, uint64_t seed, double *mass, double *pos, double *vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

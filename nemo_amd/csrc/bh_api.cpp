@@ -1,0 +1,299 @@
+// C ABI of the engine (include/bh.h): argument checking, device buffers,
+// the force_calc sequence (tree build then walk) and error mapping.  No entry
+// point exits the process; failures come back as bh_status + bh_last_error.
+#include "bh_ctx.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace bh {
+void debug_keys(bh_ctx *ctx, uint64_t *d_out);
+}
+
+namespace {
+
+using bh::ApiFail;
+using bh::HipFail;
+
+template <class F>
+bh_status guarded(bh_ctx *ctx, F &&f)
+{
+    if (!ctx) return BH_ERR_ARG;
+    try {
+        if (ctx->inject) {
+            int code = ctx->inject;
+            ctx->inject = 0;
+            throw ApiFail((bh_status)code, "injected failure");
+        }
+        BH_HIP(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return BH_OK;
+    } catch (const HipFail &e) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e.e, hipGetErrorString(e.e), e.what, e.file,
+                 e.line);
+        ctx->err = buf;
+        (void)hipGetLastError();
+        return BH_ERR_DEVICE;
+    } catch (const ApiFail &e) {
+        ctx->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return BH_ERR_NOMEM;
+    }
+}
+
+void check_params(const bh_params *p)
+{
+    if (!p) throw ApiFail(BH_ERR_ARG, "params is NULL");
+    if (!(p->tol >= 0.0) || !std::isfinite(p->tol)) throw ApiFail(BH_ERR_ARG, "tol must be finite and >= 0");
+    if (!(p->eps >= 0.0) || !std::isfinite(p->eps)) throw ApiFail(BH_ERR_ARG, "eps must be finite and >= 0");
+    if (!(p->rsize > 0.0) || !std::isfinite(p->rsize)) throw ApiFail(BH_ERR_ARG, "rsize must be finite and > 0");
+    if (p->mode < BH_MODE_FAST || p->mode > BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "unknown mode");
+    if (p->group_size != 0 && p->group_size != 64 && p->group_size != 128)
+        throw ApiFail(BH_ERR_ARG, "group_size must be 0, 64 or 128");
+    if (p->rmin_given)
+        for (int k = 0; k < 3; k++)
+            if (!std::isfinite(p->rmin[k])) throw ApiFail(BH_ERR_ARG, "rmin must be finite");
+}
+
+void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, int64_t ntest, const double *tpos)
+{
+    if (nmass < 1) throw ApiFail(BH_ERR_ARG, "nmass must be >= 1");
+    if (nmass > (int64_t)INT32_MAX / 2) throw ApiFail(BH_ERR_ARG, "nmass too large (max 2^30)");
+    if (!mass || !pos) throw ApiFail(BH_ERR_ARG, "mass/pos is NULL");
+    const bool self = (tpos == nullptr || ntest == 0);
+    if (!self && (ntest < 0 || ntest > (int64_t)INT32_MAX / 2)) throw ApiFail(BH_ERR_ARG, "bad ntest");
+    ctx->nmass = nmass;
+    ctx->self_test = self;
+    ctx->ntest = self ? nmass : ntest;
+    double *dm = bh::ensure<double>(ctx->mass, nmass);
+    double *dp = bh::ensure<double>(ctx->pos, 3 * nmass);
+    BH_HIP(hipMemcpyAsync(dm, mass, nmass * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipMemcpyAsync(dp, pos, 3 * nmass * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (!self) {
+        double *dt = bh::ensure<double>(ctx->tpos, 3 * ntest);
+        BH_HIP(hipMemcpyAsync(dt, tpos, 3 * ntest * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    bh::ensure<double>(ctx->phi, ctx->ntest);
+    bh::ensure<double>(ctx->acc, 3 * ctx->ntest);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->uploaded = true;
+}
+
+void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
+{
+    check_params(p);
+    if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_eval before bh_upload");
+    bh_params q = *p;
+    if (q.group_size == 0) q.group_size = 64;
+    BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    if (q.mode == BH_MODE_DIRECT) {
+        throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
+    }
+    bh::build_tree(ctx, q);
+    BH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
+    else bh::walk_fast(ctx, q);
+    BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    BH_HIP(hipEventSynchronize(ctx->ev[2]));
+    bh_stats s{};
+    unsigned long long c[4];
+    BH_HIP(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    s.n2btot = (int64_t)c[0];
+    s.nbctot = (int64_t)c[1];
+    s.deep_buckets = (int64_t)c[2];
+    s.n_mlp = (int64_t)c[3];
+    s.ncell = ctx->tree.ncell;
+    for (int k = 0; k < 3; k++) s.rmin[k] = ctx->box[k];
+    s.rsize = ctx->box[3];
+    float t1 = 0, t2 = 0;
+    BH_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
+    BH_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
+    s.t_tree_ms = t1;
+    s.t_walk_ms = t2;
+    ctx->last = s;
+    ctx->last_mode = q.mode;
+    if (st) *st = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+void bh_default_params(bh_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->tol = 1.0;
+    p->eps = 0.05;
+    p->usequad = 0;
+    p->mode = BH_MODE_FAST;
+    p->rsize = 4.0;
+    p->rmin_given = 0;
+    p->group_size = 0;
+    p->learned = 0;
+    p->keep_box = 0;
+}
+
+const char *bh_version(void) { return "nemo_amd-bh 0.1 (gfx950)"; }
+
+bh_status bh_create(bh_ctx **out, int device)
+{
+    if (!out) return BH_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return BH_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) return BH_ERR_ARG;
+    bh_ctx *ctx = new bh_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return BH_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->ncu = prop.multiProcessorCount;
+    for (auto &e : ctx->ev) (void)hipEventCreate(&e);
+    *out = ctx;
+    return BH_OK;
+}
+
+void bh_destroy(bh_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    bh::DevBuf *bufs[] = {&ctx->mass,   &ctx->pos,    &ctx->tpos,   &ctx->dbox,    &ctx->dboxi,  &ctx->keys0,
+                          &ctx->keys1,  &ctx->vals0,  &ctx->vals1,  &ctx->hist,    &ctx->scan_tmp, &ctx->tkeys0,
+                          &ctx->tkeys1, &ctx->tvals0, &ctx->tvals1, &ctx->nmassive, &ctx->bd,    &ctx->bhi,
+                          &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself,
+                          &ctx->cpl,    &ctx->ccnt,   &ctx->cbase,  &ctx->cmin1,   &ctx->cmin2,  &ctx->cfirst,
+                          &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->cnkc,   &ctx->cparent,
+                          &ctx->cctr,   &ctx->cskip,  &ctx->pre,    &ctx->cm,      &ctx->ccom,   &ctx->cq,
+                          &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
+                          &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc};
+    for (auto *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *bh_last_error(const bh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void bh_debug_inject(bh_ctx *ctx, int code)
+{
+    if (ctx) ctx->inject = code;
+}
+
+bh_status bh_upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, int64_t ntest,
+                    const double *tpos)
+{
+    return guarded(ctx, [&] {
+        upload(ctx, nmass, mass, pos, ntest, tpos);
+        ctx->box_valid = false;
+    });
+}
+
+bh_status bh_eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
+{
+    return guarded(ctx, [&] { eval(ctx, p, st); });
+}
+
+bh_status bh_download(bh_ctx *ctx, double *phi, double *acc)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_download before bh_upload");
+        if (phi) BH_HIP(hipMemcpy(phi, ctx->phi.p, ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+        if (acc) BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_sync(bh_ctx *ctx)
+{
+    return guarded(ctx, [&] { BH_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+bh_status bh_force(bh_ctx *ctx, const bh_params *p, int64_t nmass, const double *mass, const double *pos,
+                   int64_t ntest, const double *tpos, double *phi, double *acc, bh_stats *st)
+{
+    return guarded(ctx, [&] {
+        check_params(p);
+        if (!phi || !acc) throw ApiFail(BH_ERR_ARG, "phi/acc is NULL");
+        upload(ctx, nmass, mass, pos, ntest, tpos);
+        ctx->box_valid = false;
+        bh_params q = *p;
+        q.keep_box = 0;
+        eval(ctx, &q, st);
+        BH_HIP(hipMemcpy(phi, ctx->phi.p, ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+        BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, double *phi, double *acc)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_direct before bh_upload");
+        if (nidx < 1 || !idx || !phi || !acc) throw ApiFail(BH_ERR_ARG, "bad direct-sum arguments");
+        if (!(eps >= 0.0)) throw ApiFail(BH_ERR_ARG, "eps must be >= 0");
+        for (int64_t i = 0; i < nidx; i++)
+            if (idx[i] < 0 || idx[i] >= ctx->nmass) throw ApiFail(BH_ERR_ARG, "direct-sum index out of range");
+        int64_t *d_idx;
+        double *d_out;
+        BH_HIP(hipMalloc(&d_idx, nidx * sizeof(int64_t)));
+        BH_HIP(hipMalloc(&d_out, 4 * nidx * sizeof(double)));
+        BH_HIP(hipMemcpyAsync(d_idx, idx, nidx * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        bh::direct_sum(ctx, eps, nidx, d_idx, d_out, d_out + nidx);
+        BH_HIP(hipMemcpyAsync(phi, d_out, nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(acc, d_out + nidx, 3 * nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        BH_HIP(hipFree(d_idx));
+        BH_HIP(hipFree(d_out));
+    });
+}
+
+bh_status bh_debug_keys(bh_ctx *ctx, uint64_t *keys)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->uploaded || !ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
+        if (!keys) throw ApiFail(BH_ERR_ARG, "keys is NULL");
+        uint64_t *d;
+        BH_HIP(hipMalloc(&d, ctx->nmass * sizeof(uint64_t)));
+        bh::debug_keys(ctx, d);
+        BH_HIP(hipMemcpy(keys, d, ctx->nmass * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        BH_HIP(hipFree(d));
+    });
+}
+
+bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, double *mass, double *com, double *quad6,
+                         int64_t *ncell)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
+        const int64_t nc = ctx->tree.ncell;
+        if (ncell) *ncell = nc;
+        int64_t k = std::min<int64_t>(cap, nc);
+        if (k <= 0) return;
+        std::vector<int> f(k), l(k);
+        std::vector<uint8_t> lv(k);
+        BH_HIP(hipMemcpy(f.data(), ctx->tree.first, k * sizeof(int), hipMemcpyDeviceToHost));
+        BH_HIP(hipMemcpy(l.data(), ctx->tree.last, k * sizeof(int), hipMemcpyDeviceToHost));
+        BH_HIP(hipMemcpy(lv.data(), ctx->tree.level, k, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < k; i++) {
+            if (level) level[i] = lv[i];
+            if (nbody) nbody[i] = l[i] - f[i] + 1;
+        }
+        if (mass) BH_HIP(hipMemcpy(mass, ctx->tree.cm, k * sizeof(double), hipMemcpyDeviceToHost));
+        if (com) BH_HIP(hipMemcpy(com, ctx->tree.ccom, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
+        if (quad6) BH_HIP(hipMemcpy(quad6, ctx->tree.cq, 6 * k * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// Engine context: device buffers, stream, timing events, last error.
+#pragma once
+
+#include "bh_internal.h"
+
+#include <vector>
+
+struct bh_ctx {
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int inject = 0;
+
+    // uploaded problem
+    int64_t nmass = 0;          // bodies in the mass set (input order)
+    int64_t ntest = 0;          // test points; == nmass when the mass set is the test set
+    bool self_test = true;      // test set is the mass set
+    bool uploaded = false;
+    bool box_valid = false;     // a previous box exists (keep_box)
+    double box[4] = {0, 0, 0, 0};
+
+    // inputs (device, input order)
+    bh::DevBuf mass, pos, tpos;
+    // box state: double4 box, ints {first, start, changed, nonfinite}
+    bh::DevBuf dbox, dboxi;
+    // sorting
+    bh::DevBuf keys0, keys1, vals0, vals1, hist, scan_tmp;
+    bh::DevBuf tkeys0, tkeys1, tvals0, tvals1;
+    bh::DevBuf nmassive;        // int
+    // sorted bodies of the tree and of the test set
+    bh::DevBuf bd, bhi, blo;
+    bh::DevBuf thi, tlo, td, tperm, tself;
+    // topology
+    bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
+    bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre;
+    bh::DevBuf cm, ccom, cq, cA, cB, cQ0, cQ1;
+    bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
+    // walk scratch
+    bh::DevBuf frontier, gctr, werr;
+    int64_t fcap = 0;
+    // outputs (input order of the test set)
+    bh::DevBuf phi, acc;
+
+    bh::Tree tree;
+    int last_mode = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bh_stats last{};
+};
+
+namespace bh {
+
+template <class T>
+T *ensure(DevBuf &b, size_t count)
+{
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    if (b.cap < bytes) {
+        if (b.p) BH_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+        size_t want = bytes + bytes / 8;   // slack so slowly growing N does not reallocate each call
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw ApiFail(BH_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+        }
+        b.cap = want;
+    }
+    return static_cast<T *>(b.p);
+}
+
+template <class T>
+T *ptr(DevBuf &b) { return static_cast<T *>(b.p); }
+
+// bh_tree.hip
+void build_tree(bh_ctx *ctx, const bh_params &p);
+void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n);
+void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, int64_t n,
+                      int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout);
+// bh_exact.hip
+void walk_strict(bh_ctx *ctx, const bh_params &p);
+void direct_sum(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *d_idx, double *d_phi, double *d_acc);
+// bh_walk.hip
+void walk_fast(bh_ctx *ctx, const bh_params &p);
+
+}  // namespace bh

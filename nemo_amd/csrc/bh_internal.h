@@ -1,0 +1,79 @@
+// Internal declarations shared by the engine's HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/bh.h"
+
+namespace bh {
+
+constexpr int kMaxLevel = 21;           // 63-bit Morton keys: 21 levels below the root
+constexpr int kWaveSize = 64;
+
+// Device buffer that only grows.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// Status carried through the host code; set by BH_TRY / fail().
+struct Err {
+    bh_status code = BH_OK;
+    std::string msg;
+};
+
+// Arrays describing one built tree (all device pointers).
+struct Tree {
+    int nb = 0;                 // massive bodies loaded in the tree (sorted Morton order)
+    int ncell = 0;              // cells; cell 0 is the root when nb >= 2
+    double box[4] = {0, 0, 0, 0};   // rmin.xyz, rsize
+    // bodies, sorted
+    const double4 *bd = nullptr;    // x, y, z, m (FP64)
+    const float4 *bhi = nullptr;    // FP32 hi part of x, y, z; w = mass
+    const float4 *blo = nullptr;    // FP32 lo part (x - hi)
+    // cells
+    const int *first = nullptr, *last = nullptr;   // body range [first, last]
+    const int *sub = nullptr;       // [ncell][8] child refs: >= 0 body, < 0 ~cell, -1... see kNoChild
+    const int *nkids = nullptr;     // entries pushed when a cell is opened
+    const uint8_t *level = nullptr;
+    const int *skip = nullptr;      // pre-order position after the subtree
+    const int *pre = nullptr;       // pre-order node refs (nb + ncell entries)
+    const double *cm = nullptr;     // FP64 mass
+    const double *ccom = nullptr;   // FP64 c.o.m. [ncell][3]
+    const double *cq = nullptr;     // FP64 quadrupole [ncell][6] xx xy xz yy yz zz
+    const float4 *cA = nullptr;     // FP32 com hi xyz, mass
+    const float4 *cB = nullptr;     // FP32 com lo xyz, size^2
+    const float4 *cQ0 = nullptr;    // qxx qxy qxz qyy
+    const float4 *cQ1 = nullptr;    // qyz qzz 0 0
+};
+
+constexpr int kNoChild = INT32_MIN;  // empty octant in Tree::sub
+
+}  // namespace bh
+
+#define BH_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            throw bh::HipFail(e_, #call, __FILE__, __LINE__);                              \
+        }                                                                                   \
+    } while (0)
+
+namespace bh {
+struct HipFail {
+    hipError_t e;
+    const char *what;
+    const char *file;
+    int line;
+    HipFail(hipError_t e_, const char *w, const char *f, int l) : e(e_), what(w), file(f), line(l) {}
+};
+struct ApiFail {
+    bh_status code;
+    std::string msg;
+    ApiFail(bh_status c, std::string m) : code(c), msg(std::move(m)) {}
+};
+}  // namespace bh

@@ -1,0 +1,759 @@
+// Tree construction on the device: root box, 63-bit Morton keys, LSD radix
+// sort, octree topology from the sorted keys, and the bottom-up FP64
+// monopole/quadrupole pass.
+//
+// Semantics follow hackcode1's maketree (src/nbody/evolve/hackcode/hackcode1/
+// load.c:32-62): only massive bodies are loaded (56-60); the box starts from
+// the caller's rmin/rsize and doubles towards the first out-of-box body in
+// input order (expandbox, 68-95); a body's integer coordinate is
+// floor(2^30 (x - rmin)/rsize) on the half-open unit interval (intcoord,
+// 135-156); the child index is 4x+2y+z (subindex, 162-177); a cell exists for
+// every key prefix shared by two or more bodies (loadtree splits a leaf on
+// collision, 101-128); c.o.m. and the traceless quadrupole are accumulated
+// over the children in octant order (hackcofm, 183-226).
+//
+// Layout: the tree is described by the sorted key array.  cpl[s] is the
+// number of leading octal digits shared by keys s and s+1; the cells that
+// start at body s are the levels cpl[s-1]+1 .. cpl[s]; numbering them with an
+// exclusive scan gives cells in DFS pre-order.  Depth is capped at 21 levels
+// (63-bit keys); a level-21 cell holding several bodies is a bucket whose
+// bodies are its children.
+#include "bh_ctx.h"
+
+#include <algorithm>
+
+namespace bh {
+namespace {
+
+constexpr int kTpb = 256;
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan across a 256-thread block; *total receives the block sum.
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t ws[kTpb / 64];
+    const int w = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(v);
+    if (lane_id() == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kTpb / 64; k++) {
+        uint32_t x = ws[k];
+        if (k < w) off += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + incl - v;
+}
+
+// ---- box -----------------------------------------------------------------
+
+__device__ __forceinline__ bool fits(const double *x, const double4 b)
+{
+    double r0 = (x[0] - b.x) / b.w;
+    double r1 = (x[1] - b.y) / b.w;
+    double r2 = (x[2] - b.z) / b.w;
+    return (0.0 <= r0 && r0 < 1.0) && (0.0 <= r1 && r1 < 1.0) && (0.0 <= r2 && r2 < 1.0);
+}
+
+// First massive body at index >= start that lies outside the current box.
+// boxi: [0] first (min), [1] start, [2] changed, [3] non-finite seen
+__global__ void k_box_first(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
+                            const double4 *__restrict__ box, int *boxi)
+{
+    const double4 b = *box;
+    const int64_t start = boxi[1];
+    int best = INT32_MAX;
+    int bad = 0;
+    for (int64_t i = start + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (mass[i] == 0.0) continue;
+        const double *x = pos + 3 * i;
+        if (!fits(x, b)) {
+            if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) bad = 1;
+            best = min(best, (int)i);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        best = min(best, __shfl_xor(best, d, 64));
+        bad |= __shfl_xor(bad, d, 64);
+    }
+    if (lane_id() == 0) {
+        if (best != INT32_MAX) atomicMin(&boxi[0], best);
+        if (bad) atomicOr(&boxi[3], 1);
+    }
+}
+
+// Expand the box for the body found by k_box_first (single thread).
+__global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *boxi)
+{
+    int first = boxi[0];
+    boxi[2] = 0;
+    if (first == INT32_MAX || boxi[3]) return;
+    const double *x = pos + 3 * (int64_t)first;
+    double4 b = *box;
+    for (int guard = 0; guard < 2000 && !fits(x, b); guard++) {
+        double rmid0 = b.x + 0.5 * b.w, rmid1 = b.y + 0.5 * b.w, rmid2 = b.z + 0.5 * b.w;
+        if (x[0] < rmid0) b.x -= b.w;
+        if (x[1] < rmid1) b.y -= b.w;
+        if (x[2] < rmid2) b.z -= b.w;
+        b.w = 2.0 * b.w;
+    }
+    *box = b;
+    boxi[0] = INT32_MAX;
+    boxi[1] = first + 1;
+    boxi[2] = 1;
+}
+
+// ---- keys ------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t morton_of(const double *x, const double4 b)
+{
+    uint64_t c[3];
+    const double rm[3] = {b.x, b.y, b.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double xsc = (x[k] - rm[k]) / b.w;
+        int64_t xi;
+        if (!(xsc >= 0.0)) xi = 0;
+        else if (!(xsc < 1.0)) xi = (1 << 30) - 1;
+        else xi = (int64_t)floor(1073741824.0 * xsc);
+        c[k] = (uint64_t)(xi >> 9);
+    }
+    return (spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]);
+}
+
+// Mass set: massless bodies get bit 63 so they sort after every massive body.
+__global__ void k_keys(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
+                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals, int *nmassive)
+{
+    const double4 b = *box;
+    int cnt = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = morton_of(pos + 3 * i, b);
+        if (mass == nullptr || mass[i] != 0.0) cnt++;
+        else k |= (1ULL << 63);
+        keys[i] = k;
+        vals[i] = (uint32_t)i;
+    }
+    if (nmassive) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+        if (lane_id() == 0 && cnt) atomicAdd(nmassive, cnt);
+    }
+}
+
+// ---- exclusive scan (uint32) -------------------------------------------------
+
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kTpb * kScanItems;
+
+__global__ void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, uint32_t *partial)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    uint32_t s = 0;
+    for (int r = 0; r < kScanItems; r++) {
+        int64_t i = base + (int64_t)r * kTpb + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_top(uint32_t *partial, int64_t nparts)
+{
+    uint32_t carry = 0;
+    for (int64_t base = 0; base < nparts; base += kTpb) {
+        int64_t i = base + threadIdx.x;
+        uint32_t v = i < nparts ? partial[i] : 0;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(v, &tot);
+        if (i < nparts) partial[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) partial[nparts] = carry;
+}
+
+__global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n,
+                            const uint32_t *__restrict__ partial)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    const int64_t mine = base + (int64_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        int64_t i = mine + r;
+        v[r] = i < n ? in[i] : 0;
+        s += v[r];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(s, &tot) + partial[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) {
+        int64_t i = mine + r;
+        if (i < n) out[i] = ex;
+        ex += v[r];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kTpb - 1) out[n] = partial[gridDim.x];
+}
+
+// ---- radix sort (LSD, 8-bit digits, stable) -----------------------------------
+
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kTpb * kSortItems;
+
+__global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, uint32_t *hist, int nblk)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        int64_t i = base + (int64_t)r * kTpb + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter one tile.  Within a round the 256 threads hold 256 consecutive
+// keys; ranks among equal digits come from a wave-level match (8 ballots)
+// and a per-digit prefix over the block's 4 waves, so the order of equal
+// digits is preserved (round, wave, lane) = input order.
+__global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
+                                                        int64_t n, int shift, const uint32_t *__restrict__ hist_scan,
+                                                        int nblk)
+{
+    __shared__ uint32_t run[256];          // next free output slot per digit
+    __shared__ uint32_t wcnt[kTpb / 64][256];
+    const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+    run[t] = hist_scan[(int64_t)t * nblk + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    for (int r = 0; r < kSortItems; r++) {
+#pragma unroll
+        for (int k = 0; k < kTpb / 64; k++) wcnt[k][t] = 0;
+        __syncthreads();
+        const int64_t i = base + (int64_t)r * kTpb + t;
+        const bool ok = i < n;
+        uint64_t key = ok ? kin[i] : 0;
+        uint32_t val = ok ? vin[i] : 0;
+        uint32_t d = ok ? (uint32_t)((key >> shift) & 255u) : 0u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (ok && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        {
+            uint32_t acc = run[t];
+#pragma unroll
+            for (int k = 0; k < kTpb / 64; k++) {
+                uint32_t c = wcnt[k][t];
+                wcnt[k][t] = acc;
+                acc += c;
+            }
+            run[t] = acc;
+        }
+        __syncthreads();
+        if (ok) {
+            uint32_t dst = wcnt[w][d] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- gather sorted bodies ----------------------------------------------------
+
+__device__ __forceinline__ void split2(double x, float &hi, float &lo)
+{
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+}
+
+__global__ void k_gather_tree(const uint32_t *__restrict__ perm, int nb, const double *__restrict__ pos,
+                              const double *__restrict__ mass, double4 *bd, float4 *bhi, float4 *blo)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nb) return;
+    int64_t o = perm[s];
+    double x = pos[3 * o], y = pos[3 * o + 1], z = pos[3 * o + 2], m = mass[o];
+    bd[s] = make_double4(x, y, z, m);
+    float4 h, l;
+    split2(x, h.x, l.x);
+    split2(y, h.y, l.y);
+    split2(z, h.z, l.z);
+    h.w = (float)m;
+    l.w = 0.f;
+    bhi[s] = h;
+    blo[s] = l;
+}
+
+// Test set in its sorted order.  selfidx: tree body of the same point when
+// the mass set is the test set (sorted index s < nb), else -1.
+__global__ void k_gather_test(const uint32_t *__restrict__ perm, int64_t nt, const double *__restrict__ pos,
+                              float4 *thi, float4 *tlo, double4 *td, int *tself, int nb, int self_test)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nt) return;
+    int64_t o = perm[s];
+    double x = pos[3 * o], y = pos[3 * o + 1], z = pos[3 * o + 2];
+    float4 h, l;
+    split2(x, h.x, l.x);
+    split2(y, h.y, l.y);
+    split2(z, h.z, l.z);
+    h.w = 0.f;
+    l.w = 0.f;
+    thi[s] = h;
+    tlo[s] = l;
+    td[s] = make_double4(x, y, z, 0.0);
+    tself[s] = (self_test && s < nb) ? (int)s : -1;
+}
+
+// ---- topology ----------------------------------------------------------------
+
+__global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, uint32_t *ccnt)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nb) return;
+    int c = -1, cp = -1;
+    if (s + 1 < nb) c = min(kMaxLevel, (__clzll(keys[s] ^ keys[s + 1]) - 1) / 3);
+    if (s > 0) cp = min(kMaxLevel, (__clzll(keys[s - 1] ^ keys[s]) - 1) / 3);
+    cpl[s] = (int8_t)c;
+    ccnt[s] = (uint32_t)max(0, c - cp);
+}
+
+// Block minima of cpl over 64 and 4096 bodies, for the range-end search.
+__global__ void k_cmin(const int8_t *__restrict__ cpl, int nb, int8_t *m1, int n1)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n1) return;
+    int lo = b * 64, hi = min(nb, lo + 64);
+    int m = 127;
+    for (int j = lo; j < hi; j++) m = min(m, (int)cpl[j]);
+    m1[b] = (int8_t)m;
+}
+
+// first j >= s with cpl[j] < L (exists: cpl[nb-1] = -1)
+__device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restrict__ m1,
+                         const int8_t *__restrict__ m2, int s, int L)
+{
+    int j = s;
+    for (;;) {
+        if ((j & 4095) == 0 && m2[j >> 12] >= L) { j += 4096; continue; }
+        if ((j & 63) == 0 && m1[j >> 6] >= L) { j += 64; continue; }
+        if (cpl[j] < L) return j;
+        j++;
+    }
+}
+
+// Cells starting at body s: levels cpl[s-1]+1 .. cpl[s].
+__global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
+                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, int nb,
+                        int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
+                        unsigned long long *deep)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nb) return;
+    int c = cpl[s];
+    int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
+    uint32_t cb = cbase[s];
+    for (int L = l0; L <= c; L++) {
+        int id = cb + (L - l0);
+        int e = range_end(cpl, m1, m2, s, L);
+        cfirst[id] = s;
+        clast[id] = e;
+        clevel[id] = (uint8_t)L;
+        cskip[id] = (e + 1) + (int)cbase[e + 1];
+        pre[s + id] = ~id;
+        if (L == kMaxLevel) atomicAdd(deep, 1ULL);
+    }
+    pre[s + cb + (c >= l0 ? c - l0 + 1 : 0)] = s;
+}
+
+// Children of each cell in octant order; parent links; child-cell counts.
+__global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                           const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
+                           const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell,
+                           int *csub, int *cnkids, int *cnkc, int *cparent, int *cctr)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int s = cfirst[c], e = clast[c], L = clevel[c];
+    int *sub = csub + 8 * (int64_t)c;
+#pragma unroll
+    for (int o = 0; o < 8; o++) sub[o] = kNoChild;
+    cctr[c] = 0;
+    if (c == 0) cparent[0] = -1;
+    if (L == kMaxLevel) {            // bucket: bodies s..e are the children
+        cnkids[c] = e - s + 1;
+        cnkc[c] = 0;
+        return;
+    }
+    const int shift = 60 - 3 * L;    // octant digit of level L+1
+    int nk = 0, nkc = 0;
+    int t = s;
+    while (t <= e) {
+        int o = (int)((keys[t] >> shift) & 7);
+        int child_is_cell = cpl[t] >= L + 1;
+        if (child_is_cell) {
+            int cc = (t == s) ? c + 1 : (int)cbase[t];
+            sub[o] = ~cc;
+            cparent[cc] = c;
+            nkc++;
+            t = clast[cc] + 1;
+        } else {
+            sub[o] = t;
+            t++;
+        }
+        nk++;
+    }
+    cnkids[c] = nk;
+    cnkc[c] = nkc;
+}
+
+// Bottom-up moments.  Threads start at cells without child cells; the last
+// child to finish carries on with the parent.  Accumulation follows
+// hackcofm's order: children in octant order, FP64, no FMA (this file is
+// compiled with -ffp-contract=off).
+__device__ void cell_moments(int c, const int *__restrict__ csub, const int *__restrict__ cfirst,
+                             const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
+                             const double4 *__restrict__ bd, double *cm, double *ccom, double *cq,
+                             float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad)
+{
+    const int L = clevel[c];
+    const int *sub = csub + 8 * (int64_t)c;
+    double M = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+    const bool bucket = (L == kMaxLevel);
+    const int s = cfirst[c], e = clast[c];
+    const int nchild = bucket ? (e - s + 1) : 8;
+    for (int k = 0; k < nchild; k++) {
+        double m, x, y, z;
+        if (bucket) {
+            double4 b = bd[s + k];
+            m = b.w; x = b.x; y = b.y; z = b.z;
+        } else {
+            int r = sub[k];
+            if (r == kNoChild) continue;
+            if (r >= 0) {
+                double4 b = bd[r];
+                m = b.w; x = b.x; y = b.y; z = b.z;
+            } else {
+                int cc = ~r;
+                m = __builtin_nontemporal_load(cm + cc);
+                x = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc);
+                y = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 1);
+                z = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 2);
+            }
+        }
+        M += m;
+        double t0 = x * m, t1 = y * m, t2 = z * m;
+        px = px + t0;
+        py = py + t1;
+        pz = pz + t2;
+    }
+    px = px / M;
+    py = py / M;
+    pz = pz / M;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (usequad) {
+        for (int k = 0; k < nchild; k++) {
+            double m, x, y, z;
+            const double *qc = nullptr;
+            if (bucket) {
+                double4 b = bd[s + k];
+                m = b.w; x = b.x; y = b.y; z = b.z;
+            } else {
+                int r = sub[k];
+                if (r == kNoChild) continue;
+                if (r >= 0) {
+                    double4 b = bd[r];
+                    m = b.w; x = b.x; y = b.y; z = b.z;
+                } else {
+                    int cc = ~r;
+                    m = __builtin_nontemporal_load(cm + cc);
+                    x = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc);
+                    y = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 1);
+                    z = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 2);
+                    qc = cq + 6 * (int64_t)cc;
+                }
+            }
+            double dr[3] = {x - px, y - py, z - pz};
+            double drsq = dr[0] * dr[0];
+            drsq += dr[1] * dr[1];
+            drsq += dr[2] * dr[2];
+            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+            for (int u = 0; u < 6; u++) {
+                double idr = (ia[u] == ib[u] ? 1.0 : 0.0) * drsq;
+                double tm = (dr[ia[u]] * dr[ib[u]]) * 3.0;
+                tm = tm - idr;
+                tm = tm * m;
+                if (qc) tm = tm + __builtin_nontemporal_load(qc + u);
+                q[u] = q[u] + tm;
+            }
+        }
+    }
+    cm[c] = M;
+    ccom[3 * (int64_t)c] = px;
+    ccom[3 * (int64_t)c + 1] = py;
+    ccom[3 * (int64_t)c + 2] = pz;
+#pragma unroll
+    for (int u = 0; u < 6; u++) cq[6 * (int64_t)c + u] = q[u];
+    float4 A, B;
+    split2(px, A.x, B.x);
+    split2(py, A.y, B.y);
+    split2(pz, A.z, B.z);
+    A.w = (float)M;
+    B.w = (float)(rsize * rsize * ldexp(1.0, -2 * L));
+    cA[c] = A;
+    cB[c] = B;
+    cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+    cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
+}
+
+__global__ void k_moments(const int *__restrict__ csub, const int *__restrict__ cfirst, const int *__restrict__ clast,
+                          const uint8_t *__restrict__ clevel, const int *__restrict__ cnkc,
+                          const int *__restrict__ cparent, int *cctr, int ncell, const double4 *__restrict__ bd,
+                          double *cm, double *ccom, double *cq, float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
+                          double rsize, int usequad)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell || cnkc[c] != 0) return;
+    for (;;) {
+        cell_moments(c, csub, cfirst, clast, clevel, bd, cm, ccom, cq, cA, cB, cQ0, cQ1, rsize, usequad);
+        int p = cparent[c];
+        if (p < 0) return;
+        __threadfence();
+        int done = atomicAdd(&cctr[p], 1);
+        if (done != cnkc[p] - 1) return;
+        __threadfence();
+        c = p;
+    }
+}
+
+inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
+
+}  // namespace
+
+// ---- host launchers ------------------------------------------------------------
+
+void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n)
+{
+    // out must hold n + 1 entries; out[n] = total
+    int64_t nparts = std::max<int64_t>(1, (n + kScanChunk - 1) / kScanChunk);
+    uint32_t *partial = ensure<uint32_t>(ctx->scan_tmp, nparts + 1);
+    k_scan_reduce<<<nparts, kTpb, 0, ctx->stream>>>(in, n, partial);
+    k_scan_top<<<1, kTpb, 0, ctx->stream>>>(partial, nparts);
+    k_scan_down<<<nparts, kTpb, 0, ctx->stream>>>(in, out, n, partial);
+    BH_HIP(hipGetLastError());
+}
+
+void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, int64_t n,
+                      int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout)
+{
+    int nblk = (int)std::max<int64_t>(1, (n + kSortTile - 1) / kSortTile);
+    uint32_t *hist = ensure<uint32_t>(ctx->hist, 256 * (size_t)nblk + 1);
+    uint64_t *ka = k0, *kb = k1;
+    uint32_t *va = v0, *vb = v1;
+    for (int shift = begin_bit; shift < end_bit; shift += 8) {
+        k_radix_hist<<<nblk, kTpb, 0, ctx->stream>>>(ka, n, shift, hist, nblk);
+        exclusive_scan_u32(ctx, hist, hist, 256 * (int64_t)nblk);
+        k_radix_scatter<<<nblk, kTpb, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, hist, nblk);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    BH_HIP(hipGetLastError());
+    *kout = ka;
+    *vout = va;
+}
+
+// Morton keys of the mass set in input order, in the current box.
+void debug_keys(bh_ctx *ctx, uint64_t *d_out)
+{
+    double4 *d_box = ensure<double4>(ctx->dbox, 1);
+    BH_HIP(hipMemcpyAsync(d_box, ctx->box, sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+    uint32_t *tmp;
+    BH_HIP(hipMalloc(&tmp, ctx->nmass * sizeof(uint32_t)));
+    k_keys<<<nblocks(ctx->nmass), kTpb, 0, ctx->stream>>>(ptr<double>(ctx->pos), ptr<double>(ctx->mass), ctx->nmass,
+                                                          d_box, d_out, tmp, nullptr);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    BH_HIP(hipFree(tmp));
+}
+
+// Builds the tree of the uploaded mass set and the sorted test set.
+void build_tree(bh_ctx *ctx, const bh_params &p)
+{
+    hipStream_t st = ctx->stream;
+    const int64_t n = ctx->nmass;
+    double *d_pos = ptr<double>(ctx->pos), *d_mass = ptr<double>(ctx->mass);
+    double4 *d_box = ensure<double4>(ctx->dbox, 1);
+    int *d_boxi = ensure<int>(ctx->dboxi, 4);
+
+    // -- box: start from the caller's box (or the previous one) and expand in input order
+    double box[4];
+    if (p.keep_box && ctx->box_valid) {
+        for (int k = 0; k < 4; k++) box[k] = ctx->box[k];
+    } else {
+        for (int k = 0; k < 3; k++) box[k] = p.rmin_given ? p.rmin[k] : -p.rsize / 2.0;
+        box[3] = p.rsize;
+    }
+    int boxi[4] = {INT32_MAX, 0, 0, 0};
+    BH_HIP(hipMemcpyAsync(d_box, box, sizeof(double4), hipMemcpyHostToDevice, st));
+    BH_HIP(hipMemcpyAsync(d_boxi, boxi, sizeof(boxi), hipMemcpyHostToDevice, st));
+    const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
+    for (int it = 0; it < 4096; it++) {
+        k_box_first<<<box_grid, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, d_boxi);
+        k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
+        BH_HIP(hipMemcpyAsync(boxi, d_boxi, sizeof(boxi), hipMemcpyDeviceToHost, st));
+        BH_HIP(hipStreamSynchronize(st));
+        if (boxi[3]) throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
+        if (!boxi[2]) break;
+    }
+    BH_HIP(hipMemcpyAsync(box, d_box, sizeof(double4), hipMemcpyDeviceToHost, st));
+
+    // -- keys and sort of the mass set
+    uint64_t *k0 = ensure<uint64_t>(ctx->keys0, n), *k1 = ensure<uint64_t>(ctx->keys1, n);
+    uint32_t *v0 = ensure<uint32_t>(ctx->vals0, n), *v1 = ensure<uint32_t>(ctx->vals1, n);
+    int *d_nm = ensure<int>(ctx->nmassive, 1);
+    BH_HIP(hipMemsetAsync(d_nm, 0, sizeof(int), st));
+    const int grid_n = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 16);
+    k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, d_nm);
+    uint64_t *ks;
+    uint32_t *vs;
+    radix_sort_pairs(ctx, k0, k1, v0, v1, n, 0, 64, &ks, &vs);
+    int nb = 0;
+    BH_HIP(hipMemcpyAsync(&nb, d_nm, sizeof(int), hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 4; k++) ctx->box[k] = box[k];
+    ctx->box_valid = true;
+
+    Tree &T = ctx->tree;
+    T.nb = nb;
+    for (int k = 0; k < 4; k++) T.box[k] = box[k];
+    double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
+    float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
+    float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
+    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, st>>>(vs, nb, d_pos, d_mass, bd, bhi, blo);
+    T.bd = bd;
+    T.bhi = bhi;
+    T.blo = blo;
+
+    // -- test set, sorted by its own keys in the same box
+    const int64_t nt = ctx->ntest;
+    float4 *thi = ensure<float4>(ctx->thi, nt), *tlo = ensure<float4>(ctx->tlo, nt);
+    double4 *td = ensure<double4>(ctx->td, nt);
+    int *tself = ensure<int>(ctx->tself, nt);
+    uint32_t *tperm = ensure<uint32_t>(ctx->tperm, nt);
+    if (ctx->self_test) {
+        BH_HIP(hipMemcpyAsync(tperm, vs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_pos, thi, tlo, td, tself, nb, 1);
+    } else {
+        uint64_t *tk0 = ensure<uint64_t>(ctx->tkeys0, nt), *tk1 = ensure<uint64_t>(ctx->tkeys1, nt);
+        uint32_t *tv0 = ensure<uint32_t>(ctx->tvals0, nt), *tv1 = ensure<uint32_t>(ctx->tvals1, nt);
+        double *d_tpos = ptr<double>(ctx->tpos);
+        k_keys<<<(int)std::min<int64_t>(nblocks(nt), (int64_t)ctx->ncu * 16), kTpb, 0, st>>>(d_tpos, nullptr, nt, d_box,
+                                                                                            tk0, tv0, nullptr);
+        uint64_t *tks;
+        uint32_t *tvs;
+        radix_sort_pairs(ctx, tk0, tk1, tv0, tv1, nt, 0, 64, &tks, &tvs);
+        BH_HIP(hipMemcpyAsync(tperm, tvs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_tpos, thi, tlo, td, tself, nb, 0);
+    }
+
+    // -- topology
+    T.ncell = 0;
+    unsigned long long *cnt = ensure<unsigned long long>(ctx->counters, 16);
+    BH_HIP(hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), st));
+    int *pre = ensure<int>(ctx->pre, std::max(nb, 1) * 2 + 64);
+    T.pre = pre;
+    if (nb <= 1) {
+        if (nb == 1) {
+            int zero = 0;
+            BH_HIP(hipMemcpyAsync(pre, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        }
+        BH_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    int8_t *cpl = ensure<int8_t>(ctx->cpl, nb);
+    uint32_t *ccnt = ensure<uint32_t>(ctx->ccnt, nb + 1);
+    uint32_t *cbase = ensure<uint32_t>(ctx->cbase, nb + 1);
+    k_cpl<<<nblocks(nb), kTpb, 0, st>>>(ks, nb, cpl, ccnt);
+    exclusive_scan_u32(ctx, ccnt, cbase, nb);
+    int n1 = (nb + 63) / 64, n2 = (n1 + 63) / 64;
+    int8_t *m1 = ensure<int8_t>(ctx->cmin1, n1 + 1), *m2 = ensure<int8_t>(ctx->cmin2, n2 + 1);
+    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(cpl, nb, m1, n1);
+    k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
+    uint32_t ncell = 0;
+    BH_HIP(hipMemcpyAsync(&ncell, cbase + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    T.ncell = (int)ncell;
+    int *cfirst = ensure<int>(ctx->cfirst, ncell), *clast = ensure<int>(ctx->clast, ncell);
+    uint8_t *clevel = ensure<uint8_t>(ctx->clevel, ncell);
+    int *cskip = ensure<int>(ctx->cskip, ncell);
+    pre = ensure<int>(ctx->pre, (size_t)nb + ncell);
+    T.pre = pre;
+    k_cells<<<nblocks(nb), kTpb, 0, st>>>(cpl, cbase, m1, m2, nb, cfirst, clast, clevel, cskip, pre, cnt + 2);
+    int *csub = ensure<int>(ctx->csub, 8 * (size_t)ncell);
+    int *cnkids = ensure<int>(ctx->cnkids, ncell), *cnkc = ensure<int>(ctx->cnkc, ncell);
+    int *cparent = ensure<int>(ctx->cparent, ncell), *cctr = ensure<int>(ctx->cctr, ncell);
+    k_children<<<nblocks(ncell), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, (int)ncell, csub, cnkids, cnkc,
+                                                cparent, cctr);
+    double *cm = ensure<double>(ctx->cm, ncell), *ccom = ensure<double>(ctx->ccom, 3 * (size_t)ncell);
+    double *cq = ensure<double>(ctx->cq, 6 * (size_t)ncell);
+    float4 *cA = ensure<float4>(ctx->cA, ncell), *cB = ensure<float4>(ctx->cB, ncell);
+    float4 *cQ0 = ensure<float4>(ctx->cQ0, ncell), *cQ1 = ensure<float4>(ctx->cQ1, ncell);
+    k_moments<<<nblocks(ncell), kTpb, 0, st>>>(csub, cfirst, clast, clevel, cnkc, cparent, cctr, (int)ncell, bd, cm, ccom,
+                                               cq, cA, cB, cQ0, cQ1, box[3], p.usequad);
+    BH_HIP(hipGetLastError());
+    T.first = cfirst;
+    T.last = clast;
+    T.level = clevel;
+    T.skip = cskip;
+    T.sub = csub;
+    T.nkids = cnkids;
+    T.cm = cm;
+    T.ccom = ccom;
+    T.cq = cq;
+    T.cA = cA;
+    T.cB = cB;
+    T.cQ0 = cQ0;
+    T.cQ1 = cQ1;
+}
+
+}  // namespace bh

@@ -1,0 +1,498 @@
+// Group-wise Barnes-Hut walk with FP32 split-precision interactions.
+//
+// One wavefront owns one group of 64*NPL test particles that are contiguous
+// in Morton order.  The wave walks the tree breadth first: it takes 64
+// frontier nodes at a time (one per lane), applies hackforce's opening rule
+// (grav.c:122-131: open iff tol^2 |com - x|^2 < size^2) conservatively for
+// the whole group by using the distance from the cell's c.o.m. to the
+// group's bounding box, and sorts every node into one of three places:
+//   opened cell   -> its children are appended to the next frontier,
+//   accepted cell -> the P2C list (monopole + optional quadrupole),
+//   body          -> the P2P list.
+// Both lists live in LDS; whenever one holds 64 entries the wave consumes it
+// as a 64 x (64*NPL) tile (every lane reads the same LDS entry, a broadcast)
+// and continues.  The frontier lives in a per-wave global scratch area.
+//
+// Interaction arithmetic follows gravsub (grav.c:42-73): r^2 = |d|^2 + eps^2,
+// phi -= m/r, a += m d/r^3, and for quadrupole cells phi -= 1/2 d.Q.d / r^5,
+// a -= 5 (-1/2 d.Q.d / r^5) d / r^2 + Q d / r^5, softened r throughout.
+// Positions are split in FP32 hi+lo parts; every source is converted to
+// coordinates relative to the group centre when it enters a list, so the
+// inner loops subtract small numbers.  Each lane sums a tile in plain FP32
+// and folds the tile sum into a (hi, lo) pair with TwoSum.
+//
+// A body's own contribution is removed exactly: with eps > 0 the self pair
+// adds m*rsq(eps^2) to the potential sum and nothing to the acceleration, so
+// that term is subtracted at the end; with eps == 0 the kernel masks r = 0.
+#include "bh_ctx.h"
+
+#include <algorithm>
+
+namespace bh {
+namespace {
+
+constexpr int kWaves = 4;          // waves per block
+constexpr int kTpb = 64 * kWaves;
+constexpr int kListCap = 128;      // entries per list (a tile is 64)
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_excl_scan(int v, int *total)
+{
+    const int lane = lane_id();
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(x, d, 64);
+        if (lane >= d) x += t;
+    }
+    *total = __shfl(x, 63, 64);
+    return x - v;
+}
+__device__ __forceinline__ int mbcnt(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0));
+}
+
+// TwoSum fold of a tile sum into a compensated accumulator.
+__device__ __forceinline__ void fold(float &hi, float &lo, float v)
+{
+    float s = hi + v;
+    float bb = s - hi;
+    float err = (hi - (s - bb)) + (v - bb);
+    hi = s;
+    lo += err;
+}
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <int NPL>
+struct Lane {
+    float px[NPL], py[NPL], pz[NPL];
+    float axh[NPL], axl[NPL], ayh[NPL], ayl[NPL], azh[NPL], azl[NPL], ph[NPL], pl[NPL];
+};
+
+template <int NPL, bool SOFT0>
+__device__ __forceinline__ void eval_p2p(Lane<NPL> &L, const float4 *src, int cnt, float eps2)
+{
+    float sx[NPL], sy[NPL], sz[NPL], sp[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
+#pragma unroll 4
+    for (int j = 0; j < cnt; j++) {
+        const float4 s = src[j];
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            float dx = s.x - L.px[q], dy = s.y - L.py[q], dz = s.z - L.pz[q];
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            float ri = rsq(r2);
+            if (SOFT0) ri = r2 > 0.f ? ri : 0.f;
+            float mr = s.w * ri;
+            sp[q] += mr;
+            float mr3 = mr * ri * ri;
+            sx[q] = fmaf(dx, mr3, sx[q]);
+            sy[q] = fmaf(dy, mr3, sy[q]);
+            sz[q] = fmaf(dz, mr3, sz[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NPL; q++) {
+        fold(L.axh[q], L.axl[q], sx[q]);
+        fold(L.ayh[q], L.ayl[q], sy[q]);
+        fold(L.azh[q], L.azl[q], sz[q]);
+        fold(L.ph[q], L.pl[q], sp[q]);
+    }
+}
+
+// P2C entry: e[0] = (dx0, dy0, dz0, m) relative to the group centre,
+// e[1] = (qxx, qxy, qxz, qyy), e[2] = (qyz, qzz, -, -).
+template <int NPL, bool QUAD, bool SOFT0>
+__device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cnt, float eps2)
+{
+    constexpr int S = QUAD ? 3 : 1;
+    float sx[NPL], sy[NPL], sz[NPL], sp[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
+#pragma unroll 2
+    for (int j = 0; j < cnt; j++) {
+        const float4 s = src[S * j];
+        float4 q0, q1;
+        if (QUAD) {
+            q0 = src[S * j + 1];
+            q1 = src[S * j + 2];
+        }
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            float dx = s.x - L.px[q], dy = s.y - L.py[q], dz = s.z - L.pz[q];
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+            float ri = rsq(r2);
+            if (SOFT0) ri = r2 > 0.f ? ri : 0.f;
+            float ri2 = ri * ri;
+            float mr = s.w * ri;
+            if (QUAD) {
+                float qx = fmaf(q0.x, dx, fmaf(q0.y, dy, q0.z * dz));
+                float qy = fmaf(q0.y, dx, fmaf(q0.w, dy, q1.x * dz));
+                float qz = fmaf(q0.z, dx, fmaf(q1.x, dy, q1.y * dz));
+                float dqd = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+                float r5 = ri2 * ri2 * ri;
+                float phq = -0.5f * r5 * dqd;          // quadrupole potential term
+                float coef = (mr - 5.f * phq) * ri2;   // radial factor of a
+                sp[q] += mr - phq;
+                sx[q] = fmaf(dx, coef, fmaf(-qx, r5, sx[q]));
+                sy[q] = fmaf(dy, coef, fmaf(-qy, r5, sy[q]));
+                sz[q] = fmaf(dz, coef, fmaf(-qz, r5, sz[q]));
+            } else {
+                float mr3 = mr * ri2;
+                sp[q] += mr;
+                sx[q] = fmaf(dx, mr3, sx[q]);
+                sy[q] = fmaf(dy, mr3, sy[q]);
+                sz[q] = fmaf(dz, mr3, sz[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NPL; q++) {
+        fold(L.axh[q], L.axl[q], sx[q]);
+        fold(L.ayh[q], L.ayl[q], sy[q]);
+        fold(L.azh[q], L.azl[q], sz[q]);
+        fold(L.ph[q], L.pl[q], sp[q]);
+    }
+}
+
+struct WalkArgs {
+    const float4 *thi, *tlo;
+    const int *tself;
+    const uint32_t *tperm;
+    int64_t ntest;
+    int ngroups;
+    const float4 *bhi, *blo;
+    const float4 *cA, *cB, *cQ0, *cQ1;
+    const int *sub, *nkids, *first;
+    const uint8_t *level;
+    int nb, ncell;
+    float tolsq, eps2;
+    int *frontier;
+    int fcap;
+    int *gctr;
+    int *werr;
+    unsigned long long *counters;
+    double *phi, *acc;
+};
+
+template <int NPL, bool QUAD, bool SOFT0>
+__global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
+{
+    constexpr int G = 64 * NPL;
+    constexpr int S = QUAD ? 3 : 1;
+    __shared__ float4 sP[kWaves][kListCap];
+    __shared__ float4 sC[kWaves][kListCap * S];
+    __shared__ unsigned char sSelf[kWaves][G];
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    float4 *P = sP[w];
+    float4 *Cl = sC[w];
+    int *fr0 = a.frontier + (int64_t)(blockIdx.x * kWaves + w) * 2 * a.fcap;
+    int *fr1 = fr0 + a.fcap;
+
+    for (;;) {
+        int g = 0;
+        if (lane == 0) g = atomicAdd(a.gctr, 1);
+        g = __shfl(g, 0, 64);
+        if (g >= a.ngroups) break;
+        const int64_t gbase = (int64_t)g * G;
+        const int gcount = (int)min<int64_t>(G, a.ntest - gbase);
+
+        // particles of this lane; inactive slots copy the first particle
+        float4 h[NPL], l[NPL];
+        bool act[NPL];
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            int k = q * 64 + lane;
+            act[q] = k < gcount;
+            int64_t i = gbase + (act[q] ? k : 0);
+            h[q] = a.thi[i];
+            l[q] = a.tlo[i];
+        }
+        float mnx = h[0].x, mny = h[0].y, mnz = h[0].z, mxx = mnx, mxy = mny, mxz = mnz;
+#pragma unroll
+        for (int q = 1; q < NPL; q++) {
+            mnx = fminf(mnx, h[q].x); mny = fminf(mny, h[q].y); mnz = fminf(mnz, h[q].z);
+            mxx = fmaxf(mxx, h[q].x); mxy = fmaxf(mxy, h[q].y); mxz = fmaxf(mxz, h[q].z);
+        }
+        const float cx = 0.5f * (wave_min(mnx) + wave_max(mxx));
+        const float cy = 0.5f * (wave_min(mny) + wave_max(mxy));
+        const float cz = 0.5f * (wave_min(mnz) + wave_max(mxz));
+        Lane<NPL> L;
+        float bminx = 3.0e38f, bminy = 3.0e38f, bminz = 3.0e38f, bmaxx = -3.0e38f, bmaxy = -3.0e38f, bmaxz = -3.0e38f;
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            L.px[q] = (h[q].x - cx) + l[q].x;
+            L.py[q] = (h[q].y - cy) + l[q].y;
+            L.pz[q] = (h[q].z - cz) + l[q].z;
+            L.axh[q] = L.axl[q] = L.ayh[q] = L.ayl[q] = L.azh[q] = L.azl[q] = L.ph[q] = L.pl[q] = 0.f;
+            bminx = fminf(bminx, L.px[q]); bminy = fminf(bminy, L.py[q]); bminz = fminf(bminz, L.pz[q]);
+            bmaxx = fmaxf(bmaxx, L.px[q]); bmaxy = fmaxf(bmaxy, L.py[q]); bmaxz = fmaxf(bmaxz, L.pz[q]);
+        }
+        bminx = wave_min(bminx); bminy = wave_min(bminy); bminz = wave_min(bminz);
+        bmaxx = wave_max(bmaxx); bmaxy = wave_max(bmaxy); bmaxz = wave_max(bmaxz);
+#pragma unroll
+        for (int q = 0; q < NPL; q++) sSelf[w][q * 64 + lane] = 0;
+
+        int np = 0, nc = 0, nf = 0;
+        long long tot_pp = 0, tot_pc = 0;
+        if (a.ncell == 0) {
+            if (a.nb == 1 && lane == 0) fr0[0] = 0;
+            nf = a.nb == 1 ? 1 : 0;
+        } else {
+            if (lane == 0) fr0[0] = ~0;
+            nf = 1;
+        }
+        bool overflow = false;
+        while (nf > 0 && !overflow) {
+            int nn = 0;
+            for (int base = 0; base < nf; base += 64) {
+                wave_lds_sync();
+                const int k = base + lane;
+                const bool v = k < nf;
+                const int node = v ? fr0[k] : 0;
+                const bool isBody = v && node >= 0;
+                const bool isCell = v && node < 0;
+                bool open = false, accept = false;
+                int c = 0, nk = 0;
+                float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+                float rx = 0.f, ry = 0.f, rz = 0.f;
+                if (isCell) {
+                    c = ~node;
+                    A = a.cA[c];
+                    const float4 B = a.cB[c];
+                    rx = (A.x - cx) + B.x;
+                    ry = (A.y - cy) + B.y;
+                    rz = (A.z - cz) + B.z;
+                    float dx = fmaxf(fmaxf(bminx - rx, rx - bmaxx), 0.f);
+                    float dy = fmaxf(fmaxf(bminy - ry, ry - bmaxy), 0.f);
+                    float dz = fmaxf(fmaxf(bminz - rz, rz - bmaxz), 0.f);
+                    float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    open = a.tolsq * d2 < B.w;
+                    accept = !open;
+                    if (open) nk = a.nkids[c];
+                }
+                // bodies -> P2P list
+                {
+                    const unsigned long long m = __ballot(isBody);
+                    if (isBody) {
+                        const float4 bh = a.bhi[node], bl = a.blo[node];
+                        P[np + mbcnt(m)] = make_float4((bh.x - cx) + bl.x, (bh.y - cy) + bl.y, (bh.z - cz) + bl.z, bh.w);
+                        const int64_t rel = (int64_t)node - gbase;
+                        if (rel >= 0 && rel < gcount && a.tself[gbase + rel] == node) sSelf[w][rel] = 1;
+                    }
+                    np += __popcll(m);
+                }
+                // accepted cells -> P2C list
+                {
+                    const unsigned long long m = __ballot(accept);
+                    if (accept) {
+                        const int slot = nc + mbcnt(m);
+                        Cl[S * slot] = make_float4(rx, ry, rz, A.w);
+                        if (QUAD) {
+                            Cl[S * slot + 1] = a.cQ0[c];
+                            Cl[S * slot + 2] = a.cQ1[c];
+                        }
+                    }
+                    nc += __popcll(m);
+                }
+                // opened cells -> next frontier
+                {
+                    int tot;
+                    const int off = wave_excl_scan(nk, &tot);
+                    if (nn + tot > a.fcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (open) {
+                        int *dst = fr1 + nn + off;
+                        if (a.level[c] == kMaxLevel) {
+                            const int f = a.first[c];
+                            for (int j = 0; j < nk; j++) dst[j] = f + j;
+                        } else {
+                            const int *sub = a.sub + 8 * (int64_t)c;
+                            int j = 0;
+#pragma unroll
+                            for (int o = 0; o < 8; o++) {
+                                int r = sub[o];
+                                if (r != kNoChild) dst[j++] = r;
+                            }
+                        }
+                    }
+                    nn += tot;
+                }
+                wave_lds_sync();
+                while (np >= 64) {
+                    eval_p2p<NPL, SOFT0>(L, P, 64, a.eps2);
+                    tot_pp += 64;
+                    np -= 64;
+                    wave_lds_sync();
+                    float4 t = P[64 + (lane < np ? lane : 0)];
+                    wave_lds_sync();
+                    if (lane < np) P[lane] = t;
+                    wave_lds_sync();
+                }
+                while (nc >= 64) {
+                    eval_p2c<NPL, QUAD, SOFT0>(L, Cl, 64, a.eps2);
+                    tot_pc += 64;
+                    nc -= 64;
+                    wave_lds_sync();
+                    float4 t0 = Cl[S * (64 + (lane < nc ? lane : 0))];
+                    float4 t1, t2;
+                    if (QUAD) {
+                        t1 = Cl[S * (64 + (lane < nc ? lane : 0)) + 1];
+                        t2 = Cl[S * (64 + (lane < nc ? lane : 0)) + 2];
+                    }
+                    wave_lds_sync();
+                    if (lane < nc) {
+                        Cl[S * lane] = t0;
+                        if (QUAD) {
+                            Cl[S * lane + 1] = t1;
+                            Cl[S * lane + 2] = t2;
+                        }
+                    }
+                    wave_lds_sync();
+                }
+            }
+            int *tmp = fr0;
+            fr0 = fr1;
+            fr1 = tmp;
+            nf = nn;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+        if (overflow) {
+            if (lane == 0) atomicOr(a.werr, 1);
+            continue;
+        }
+        wave_lds_sync();
+        if (np > 0) eval_p2p<NPL, SOFT0>(L, P, np, a.eps2);
+        if (nc > 0) eval_p2c<NPL, QUAD, SOFT0>(L, Cl, nc, a.eps2);
+        tot_pp += np;
+        tot_pc += nc;
+        wave_lds_sync();
+
+        int nself = 0;
+#pragma unroll
+        for (int q = 0; q < NPL; q++) {
+            if (!act[q]) continue;
+            const int k = q * 64 + lane;
+            double ph = (double)L.ph[q] + (double)L.pl[q];
+            if (sSelf[w][k]) {
+                nself++;
+                if (!SOFT0) ph -= (double)(a.bhi[gbase + k].w * rsq(a.eps2));
+            }
+            const int64_t o = a.tperm[gbase + k];
+            a.phi[o] = -ph;
+            a.acc[3 * o] = (double)L.axh[q] + (double)L.axl[q];
+            a.acc[3 * o + 1] = (double)L.ayh[q] + (double)L.ayl[q];
+            a.acc[3 * o + 2] = (double)L.azh[q] + (double)L.azl[q];
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) nself += __shfl_xor(nself, d, 64);
+        if (lane == 0) {
+            atomicAdd(&a.counters[0], (unsigned long long)(tot_pp * gcount - nself));
+            atomicAdd(&a.counters[1], (unsigned long long)(tot_pc * gcount));
+        }
+    }
+}
+
+template <int NPL, bool QUAD, bool SOFT0>
+void launch(bh_ctx *ctx, WalkArgs &a, int grid)
+{
+    k_walk<NPL, QUAD, SOFT0><<<grid, kTpb, 0, ctx->stream>>>(a);
+}
+
+}  // namespace
+
+void walk_fast(bh_ctx *ctx, const bh_params &p)
+{
+    const Tree &T = ctx->tree;
+    const int64_t nt = ctx->ntest;
+    double *phi = ptr<double>(ctx->phi), *acc = ptr<double>(ctx->acc);
+    if (T.nb == 0) {
+        BH_HIP(hipMemsetAsync(phi, 0, nt * sizeof(double), ctx->stream));
+        BH_HIP(hipMemsetAsync(acc, 0, 3 * nt * sizeof(double), ctx->stream));
+        return;
+    }
+    const int npl = (p.group_size == 128) ? 2 : 1;
+    const int G = 64 * npl;
+    WalkArgs a{};
+    a.thi = ptr<float4>(ctx->thi);
+    a.tlo = ptr<float4>(ctx->tlo);
+    a.tself = ptr<int>(ctx->tself);
+    a.tperm = ptr<uint32_t>(ctx->tperm);
+    a.ntest = nt;
+    a.ngroups = (int)((nt + G - 1) / G);
+    a.bhi = T.bhi;
+    a.blo = T.blo;
+    a.cA = T.cA;
+    a.cB = T.cB;
+    a.cQ0 = T.cQ0;
+    a.cQ1 = T.cQ1;
+    a.sub = T.sub;
+    a.nkids = T.nkids;
+    a.first = T.first;
+    a.level = T.level;
+    a.nb = T.nb;
+    a.ncell = T.ncell;
+    a.tolsq = (float)(p.tol * p.tol);
+    a.eps2 = (float)(p.eps * p.eps);
+    a.counters = ptr<unsigned long long>(ctx->counters);
+    a.phi = phi;
+    a.acc = acc;
+    // persistent grid: enough blocks to fill the device, each wave loops over groups
+    int grid = std::min<int>((a.ngroups + kWaves - 1) / kWaves, ctx->ncu * 8);
+    grid = std::max(grid, 1);
+    if (ctx->fcap == 0) ctx->fcap = 1 << 15;
+    for (int attempt = 0; attempt < 6; attempt++) {
+        a.fcap = (int)ctx->fcap;
+        a.frontier = ensure<int>(ctx->frontier, (size_t)grid * kWaves * 2 * ctx->fcap);
+        a.gctr = ensure<int>(ctx->gctr, 1);
+        a.werr = ensure<int>(ctx->werr, 1);
+        BH_HIP(hipMemsetAsync(a.gctr, 0, sizeof(int), ctx->stream));
+        BH_HIP(hipMemsetAsync(a.werr, 0, sizeof(int), ctx->stream));
+        const bool soft0 = p.eps == 0.0;
+        if (npl == 2) {
+            if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);
+            else soft0 ? launch<2, false, true>(ctx, a, grid) : launch<2, false, false>(ctx, a, grid);
+        } else {
+            if (p.usequad) soft0 ? launch<1, true, true>(ctx, a, grid) : launch<1, true, false>(ctx, a, grid);
+            else soft0 ? launch<1, false, true>(ctx, a, grid) : launch<1, false, false>(ctx, a, grid);
+        }
+        BH_HIP(hipGetLastError());
+        int werr = 0;
+        BH_HIP(hipMemcpyAsync(&werr, a.werr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        if (!werr) return;
+        // frontier overflow: counters of the failed pass are discarded, retry with 4x capacity
+        BH_HIP(hipMemsetAsync(a.counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
+        ctx->fcap *= 4;
+        if ((size_t)grid * kWaves * 2 * ctx->fcap * sizeof(int) > ((size_t)64 << 30))
+            throw ApiFail(BH_ERR_CAPACITY, "walk frontier would exceed 64 GiB; use a larger tol");
+    }
+    throw ApiFail(BH_ERR_CAPACITY, "walk frontier capacity exceeded");
+}
+
+}  // namespace bh

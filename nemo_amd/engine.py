@@ -1,0 +1,231 @@
+"""Python host interface of the engine: ctypes over libnemo_bh.so (include/bh.h).
+
+``hackforce(...)`` mirrors NEMO hackforce's keywords and semantics
+(src/nbody/evolve/hackcode/hackcode1/hackforce.c:38-50,177-215): ``tol``,
+``eps``, ``rsize``, ``rmin`` (default -rsize/2), quadrupole through
+``usequad`` (hackforce_qp), test points through ``tpos`` (test=).  There is no
+CPU fallback: if the shared library is missing or no device is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libnemo_bh.so")
+
+MODE_FAST, MODE_STRICT, MODE_DIRECT = 0, 1, 2
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_NONFINITE", 5: "ERR_CAPACITY",
+          6: "ERR_STATE", 7: "ERR_INJECTED"}
+
+
+class BHError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("tol", C.c_double), ("eps", C.c_double), ("usequad", C.c_int), ("mode", C.c_int),
+                ("rsize", C.c_double), ("rmin", C.c_double * 3), ("rmin_given", C.c_int),
+                ("group_size", C.c_int), ("learned", C.c_int), ("keep_box", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n2btot", C.c_int64), ("nbctot", C.c_int64), ("ncell", C.c_int64), ("deep_buckets", C.c_int64),
+                ("rmin", C.c_double * 3), ("rsize", C.c_double), ("t_tree_ms", C.c_double),
+                ("t_walk_ms", C.c_double), ("n_mlp", C.c_int64)]
+
+    def as_dict(self):
+        return {k: (list(getattr(self, k)) if k == "rmin" else getattr(self, k)) for k, _ in self._fields_}
+
+
+_LIB = None
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_u64 = C.POINTER(C.c_uint64)
+_i32 = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+# every symbol include/bh.h declares, with its ctypes signature
+SIGNATURES = {
+    "bh_default_params": (None, [C.POINTER(Params)]),
+    "bh_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "bh_destroy": (None, [_vp]),
+    "bh_last_error": (C.c_char_p, [_vp]),
+    "bh_version": (C.c_char_p, []),
+    "bh_force": (C.c_int, [_vp, C.POINTER(Params), C.c_int64, _d, _d, C.c_int64, _d, _d, _d, C.POINTER(Stats)]),
+    "bh_upload": (C.c_int, [_vp, C.c_int64, _d, _d, C.c_int64, _d]),
+    "bh_eval": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
+    "bh_download": (C.c_int, [_vp, _d, _d]),
+    "bh_direct": (C.c_int, [_vp, C.c_double, C.c_int64, _i64, _d, _d]),
+    "bh_sync": (C.c_int, [_vp]),
+    "bh_debug_keys": (C.c_int, [_vp, _u64]),
+    "bh_debug_cells": (C.c_int, [_vp, C.c_int64, _i32, _i32, _d, _d, _d, _i64]),
+    "bh_debug_inject": (None, [_vp, C.c_int]),
+    "bh_mkplummer": (None, [C.c_int64, C.c_uint64, _d, _d, _d]),
+}
+
+
+def lib():
+    """Load libnemo_bh.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("libnemo_bh.so not built: run `make -C nemo_amd` or __graft_entry__.build()")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().bh_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "rmin":
+            if v is not None:
+                p.rmin[:] = [float(x) for x in v]
+                p.rmin_given = 1
+        elif k == "mode" and isinstance(v, str):
+            p.mode = {"fast": MODE_FAST, "strict": MODE_STRICT, "direct": MODE_DIRECT}[v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def mkplummer(n, seed=123, with_vel=True):
+    """Equal-mass Plummer sphere in virial units (mkplummer recipe, own RNG)."""
+    mass = np.zeros(n)
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3)) if with_vel else None
+    lib().bh_mkplummer(n, seed, _p(mass), _p(pos), _p(vel) if with_vel else None)
+    return mass, pos, vel
+
+
+class Engine:
+    """One device context (bh_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = _vp()
+        st = lib().bh_create(C.byref(self._h), device)
+        if st != 0:
+            raise BHError(st, "bh_create(device=%d) failed: no usable HIP device" % device)
+        self.ntest = 0
+
+    def close(self):
+        if self._h:
+            lib().bh_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise BHError(st, lib().bh_last_error(self._h).decode())
+
+    def upload(self, mass, pos, tpos=None):
+        self._mass = np.ascontiguousarray(mass, np.float64)
+        self._pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        n = len(self._mass)
+        if tpos is not None:
+            self._tpos = np.ascontiguousarray(tpos, np.float64).reshape(-1, 3)
+            nt = len(self._tpos)
+            self._check(lib().bh_upload(self._h, n, _p(self._mass), _p(self._pos), nt, _p(self._tpos)))
+        else:
+            nt = n
+            self._check(lib().bh_upload(self._h, n, _p(self._mass), _p(self._pos), 0, None))
+        self.ntest = nt
+
+    def eval(self, params: Params) -> Stats:
+        st = Stats()
+        self._check(lib().bh_eval(self._h, C.byref(params), C.byref(st)))
+        return st
+
+    def download(self):
+        phi = np.zeros(self.ntest)
+        acc = np.zeros((self.ntest, 3))
+        self._check(lib().bh_download(self._h, _p(phi), _p(acc)))
+        return phi, acc
+
+    def direct(self, eps, idx):
+        idx = np.ascontiguousarray(idx, np.int64)
+        phi = np.zeros(len(idx))
+        acc = np.zeros((len(idx), 3))
+        self._check(lib().bh_direct(self._h, eps, len(idx), _p(idx, _i64), _p(phi), _p(acc)))
+        return phi, acc
+
+    def force(self, mass, pos, params: Params, tpos=None):
+        mass = np.ascontiguousarray(mass, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        n = len(mass)
+        if tpos is not None:
+            tpos = np.ascontiguousarray(tpos, np.float64).reshape(-1, 3)
+            nt, tp = len(tpos), _p(tpos)
+        else:
+            nt, tp = n, None
+        phi = np.zeros(nt)
+        acc = np.zeros((nt, 3))
+        st = Stats()
+        self._check(lib().bh_force(self._h, C.byref(params), n, _p(mass), _p(pos), nt if tpos is not None else 0, tp,
+                                   _p(phi), _p(acc), C.byref(st)))
+        self.ntest = nt
+        return phi, acc, st
+
+    def debug_keys(self):
+        k = np.zeros(len(self._mass), np.uint64)
+        self._check(lib().bh_debug_keys(self._h, _p(k, _u64)))
+        return k
+
+    def debug_cells(self):
+        nc = np.zeros(1, np.int64)
+        self._check(lib().bh_debug_cells(self._h, 0, None, None, None, None, None, _p(nc, _i64)))
+        n = int(nc[0])
+        lv = np.zeros(n, np.int32)
+        nb = np.zeros(n, np.int32)
+        m = np.zeros(n)
+        com = np.zeros((n, 3))
+        q = np.zeros((n, 6))
+        self._check(lib().bh_debug_cells(self._h, n, _p(lv, _i32), _p(nb, _i32), _p(m), _p(com), _p(q), _p(nc, _i64)))
+        return dict(level=lv, nbody=nb, mass=m, pos=com, quad6=q)
+
+    def inject(self, code):
+        lib().bh_debug_inject(self._h, code)
+
+
+@dataclass
+class ForceResult:
+    phi: np.ndarray
+    acc: np.ndarray
+    n2btot: int
+    nbctot: int
+    rmin: list
+    rsize: float
+    cputree: float   # minutes, as hackforce's Diagnostics
+    cpufcal: float
+
+
+def hackforce(mass, pos, tpos=None, tol=1.0, eps=0.05, rsize=4.0, rmin=None, usequad=False, mode="fast",
+              engine: Engine | None = None, **kw) -> ForceResult:
+    """force_calc of hackforce on the GPU engine (hackforce.c:177-215)."""
+    eng = engine or Engine()
+    p = default_params(tol=tol, eps=eps, rsize=rsize, rmin=rmin, usequad=int(bool(usequad)), mode=mode, **kw)
+    phi, acc, st = eng.force(mass, pos, p, tpos=tpos)
+    return ForceResult(phi, acc, st.n2btot, st.nbctot, list(st.rmin), st.rsize, st.t_tree_ms / 60000.0,
+                       st.t_walk_ms / 60000.0)
