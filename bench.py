@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-force evaluations per second at N = 2^22 Plummer,
+quadrupole Barnes-Hut on one MI355X, at the opening angle where the median
+|da|/|a| against an FP64 direct sum is 1e-3.
+
+A step is one full force evaluation of the resident snapshot: root box,
+Morton keys, radix sort, topology, FP64 moments, group walk with the
+interactions (and the learned correction when enabled).  Inputs are resident
+in HBM before timing starts; results stay on the device.
+
+Prints one JSON line (see the keys below).  The roofline entry is for the
+walk kernel, the dominant kernel: 20 flop per interaction (P2P and P2C,
+counted by the kernel) over the walk kernel's own event-timed duration.
+The cpu_baseline entry times the CPU restatement of hackforce (FP64,
+single thread) on a bounded sample of the same snapshot: one full tree
+build plus the walk of a subset of bodies, extrapolated to N.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-force evals/s, N=4M, median |da|/|a|=1e-3: learned vs plain tree"
+FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32 (packed); unpacked FMA rate is 78.6
+FLOP_PER_INTERACTION = 20
+
+
+def rel_err(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def measure_err(eng, E, p, idx, dacc):
+    eng.eval(p)
+    _, acc = eng.download()
+    e = rel_err(acc[idx], dacc)
+    return float(np.median(e)), float(np.percentile(e, 99))
+
+
+def calibrate_tol(eng, E, base, idx, dacc, target, lo=0.2, hi=1.3, iters=9):
+    """Largest tol whose median error is <= target (bisection; error grows with tol)."""
+    best = None
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        p = E.default_params(**dict(base, tol=mid))
+        med, p99 = measure_err(eng, E, p, idx, dacc)
+        if med <= target:
+            best = (mid, med, p99)
+            lo = mid
+        else:
+            hi = mid
+    if best is None:
+        p = E.default_params(**dict(base, tol=lo))
+        med, p99 = measure_err(eng, E, p, idx, dacc)
+        best = (lo, med, p99)
+    return best
+
+
+def time_steps(eng, p, steps, warmup):
+    for _ in range(warmup):
+        eng.eval(p)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(eng.eval(p))
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, stats
+
+
+def cpu_baseline(m, x, idx_err, dacc, tol, eps, usequad, budget_s):
+    from oracle import oracle
+
+    # pilot: size the sample for the time budget
+    pilot = idx_err[:64]
+    _, _, _, _, t_tree, t_walk = oracle.force_sample(m, x, pilot, tol, eps, usequad)
+    per = t_walk / len(pilot)
+    ns = int(max(64, min(len(idx_err), (budget_s - t_tree) / max(per, 1e-9))))
+    sample = idx_err[:ns]
+    _, acc, n2b, nbc, t_tree, t_walk = oracle.force_sample(m, x, sample, tol, eps, usequad)
+    per = t_walk / ns
+    n = len(m)
+    t_full = t_tree + per * n
+    e = rel_err(acc, dacc[:ns])
+    return {
+        "value": n / t_full,
+        "unit": "evals/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle hackforce restatement (FP64, 1 thread): full tree build of N=%d (%.2f s) + walk of %d "
+                  "bodies (%.2f s, %.0f interactions/body), extrapolated to N; same tol=%.4f eps=%g quad=%d; "
+                  "sample median |da|/|a|=%.3g" % (n, t_tree, ns, t_walk, (n2b + nbc) / ns, tol, eps, usequad,
+                                                   float(np.median(e))),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 22)
+    ap.add_argument("--eps", type=float, default=0.01)
+    ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--nsub", type=int, default=8192)
+    ap.add_argument("--tol", type=float, default=None, help="skip calibration and use this tol")
+    ap.add_argument("--group", type=int, default=64)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
+              "described in DESIGN.md, not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from nemo_amd import engine as E
+
+    n = args.n
+    m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
+    eng = E.Engine(0)
+    eng.upload(m, x)
+    rng = np.random.default_rng(123)
+    idx = np.sort(rng.permutation(n)[: args.nsub])
+    t0 = time.perf_counter()
+    _, dacc = eng.direct(args.eps, idx)
+    t_direct = time.perf_counter() - t0
+    print("direct-sum reference: %d targets x %d sources in %.2f s" % (len(idx), n, t_direct), file=sys.stderr,
+          flush=True)
+
+    base = dict(eps=args.eps, usequad=1, group_size=args.group)
+    if args.tol is None:
+        tol, med, p99 = calibrate_tol(eng, E, base, idx, dacc, args.target)
+    else:
+        tol = args.tol
+        med, p99 = measure_err(eng, E, E.default_params(**dict(base, tol=tol)), idx, dacc)
+    print("plain tree: tol=%.4f median=%.3g p99=%.3g" % (tol, med, p99), file=sys.stderr, flush=True)
+    p = E.default_params(**dict(base, tol=tol))
+    sec, stats = time_steps(eng, p, args.steps, args.warmup)
+    value = n / sec
+    n_int = stats[-1].n2btot + stats[-1].nbctot
+    t_walk = float(np.mean([s.t_walk_kernel_ms for s in stats])) * 1e-3
+    achieved = FLOP_PER_INTERACTION * n_int / t_walk / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic Plummer (mkplummer recipe, own counter RNG, seed 123)",
+        "config": {
+            "workload": "plummer_n%d_quad_plain_tree_matched_median_err" % n,
+            "n": n, "eps": args.eps, "tol": tol, "usequad": 1, "group_size": args.group,
+            "median_rel_err": med, "p99_rel_err": p99, "target_median_err": args.target,
+            "err_subset": len(idx), "learned": None,
+            "interactions_per_step": int(n_int), "n2b": int(stats[-1].n2btot), "nbc": int(stats[-1].nbctot),
+            "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])),
+            "t_sort_ms": float(np.mean([s.t_sort_ms for s in stats])),
+            "t_walk_kernel_ms": t_walk * 1e3,
+            "interactions_per_s": n_int / sec,
+            "ngroups": int(stats[-1].ngroups), "mean_group_fill": n / max(1, stats[-1].ngroups),
+        },
+        "roofline": {
+            "bound": "valu_fp32",
+            "kernel": "k_walk<1,true,false>",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "flop_convention": "20 flop per interaction (P2P and quadrupole P2C)",
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m, x, idx, dacc, tol, args.eps, 1, args.cpu_budget)
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -70,6 +70,9 @@ typedef struct {
     double t_tree_ms;    /* device time: box, keys, sort, topology, moments */
     double t_walk_ms;    /* device time: walk + interactions (+ MLP) */
     int64_t n_mlp;       /* learned corrections applied */
+    double t_walk_kernel_ms; /* device time of the walk kernel launch alone */
+    double t_sort_ms;    /* device time of the mass-set key sort */
+    int64_t ngroups;     /* fast mode: groups of test points walked */
 } bh_stats;
 
 typedef struct bh_ctx bh_ctx;
@@ -103,6 +106,10 @@ bh_status bh_sync(bh_ctx *ctx);
 bh_status bh_debug_keys(bh_ctx *ctx, uint64_t *keys);
 bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, double *mass, double *com,
                          double *quad6, int64_t *ncell);
+/* Sorted order of the test set (test index -> input index) and the group
+ * partition of the last fast-mode evaluation (first, count per group). */
+bh_status bh_debug_perm(bh_ctx *ctx, uint32_t *perm);
+bh_status bh_debug_groups(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ngroups);
 /* Force the next call to fail with BH_ERR_INJECTED (error-path tests). */
 void bh_debug_inject(bh_ctx *ctx, int code);
 

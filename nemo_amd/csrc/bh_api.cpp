@@ -114,6 +114,9 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     BH_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
     s.t_tree_ms = t1;
     s.t_walk_ms = t2;
+    s.t_walk_kernel_ms = q.mode == BH_MODE_FAST ? ctx->t_walk_kernel_ms : t2;
+    s.t_sort_ms = ctx->t_sort_ms;
+    s.ngroups = q.mode == BH_MODE_FAST ? ctx->ngroups : 0;
     ctx->last = s;
     ctx->last_mode = q.mode;
     if (st) *st = s;
@@ -177,7 +180,9 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->cnkc,   &ctx->cparent,
                           &ctx->cctr,   &ctx->cskip,  &ctx->pre,    &ctx->cm,      &ctx->ccom,   &ctx->cq,
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
-                          &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc};
+                          &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
+                          &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
+                          &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
@@ -293,6 +298,26 @@ bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, doubl
         if (mass) BH_HIP(hipMemcpy(mass, ctx->tree.cm, k * sizeof(double), hipMemcpyDeviceToHost));
         if (com) BH_HIP(hipMemcpy(com, ctx->tree.ccom, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
         if (quad6) BH_HIP(hipMemcpy(quad6, ctx->tree.cq, 6 * k * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_debug_perm(bh_ctx *ctx, uint32_t *perm)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->box_valid || !perm) throw ApiFail(BH_ERR_STATE, "no tree built");
+        BH_HIP(hipMemcpy(perm, ctx->tperm.p, ctx->ntest * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_debug_groups(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ngroups)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
+        if (ngroups) *ngroups = ctx->ngroups;
+        int64_t k = std::min<int64_t>(cap, ctx->ngroups);
+        if (k <= 0) return;
+        if (first) BH_HIP(hipMemcpy(first, ctx->gfirst.p, k * sizeof(int), hipMemcpyDeviceToHost));
+        if (count) BH_HIP(hipMemcpy(count, ctx->gcount.p, k * sizeof(int), hipMemcpyDeviceToHost));
     });
 }
 

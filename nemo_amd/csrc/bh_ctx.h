@@ -36,6 +36,10 @@ struct bh_ctx {
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre;
     bh::DevBuf cm, ccom, cq, cA, cB, cQ0, cQ1;
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
+    // groups of the test set (fast walk)
+    bh::DevBuf gfirst, gcount, gscan, goff;
+    bh::DevBuf gcpl, gccnt, gcbase, gm1, gm2, gcf, gcl, gclv;
+    int ngroups = 0;
     // walk scratch
     bh::DevBuf frontier, gctr, werr;
     int64_t fcap = 0;
@@ -44,7 +48,8 @@ struct bh_ctx {
 
     bh::Tree tree;
     int last_mode = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float t_walk_kernel_ms = 0.f, t_sort_ms = 0.f;
     bh_stats last{};
 };
 

@@ -399,11 +399,121 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restri
         cfirst[id] = s;
         clast[id] = e;
         clevel[id] = (uint8_t)L;
-        cskip[id] = (e + 1) + (int)cbase[e + 1];
-        pre[s + id] = ~id;
-        if (L == kMaxLevel) atomicAdd(deep, 1ULL);
+        if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
+        if (pre) pre[s + id] = ~id;
+        if (deep && L == kMaxLevel) atomicAdd(deep, 1ULL);
     }
-    pre[s + cb + (c >= l0 ? c - l0 + 1 : 0)] = s;
+    if (pre) pre[s + cb + (c >= l0 ? c - l0 + 1 : 0)] = s;
+}
+
+// ---- groups of test points -------------------------------------------------
+// A cell holding more than G points emits groups made of runs of consecutive
+// children that hold at most G points each (siblings are adjacent in space, so
+// a group's extent is at most its parent cell); children with more than G
+// points emit their own groups.  A level-21 bucket with more than G points is
+// cut into chunks of G.  Every point lands in exactly one group.
+// Octants inside the axis-aligned box spanned by the octants of mask m.
+__device__ __forceinline__ unsigned octant_box(unsigned m)
+{
+    unsigned lo[3] = {1, 1, 1}, hi[3] = {0, 0, 0};
+    for (int q = 0; q < 8; q++)
+        if (m & (1u << q))
+            for (int a = 0; a < 3; a++) {
+                unsigned b = (q >> (2 - a)) & 1u;
+                lo[a] = min(lo[a], b);
+                hi[a] = max(hi[a], b);
+            }
+    unsigned box = 0;
+    for (int q = 0; q < 8; q++) {
+        bool in = true;
+        for (int a = 0; a < 3; a++) {
+            unsigned b = (q >> (2 - a)) & 1u;
+            in = in && b >= lo[a] && b <= hi[a];
+        }
+        if (in) box |= 1u << q;
+    }
+    return box;
+}
+
+template <bool WRITE>
+__device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                                const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
+                                const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int G, int *gfirst,
+                                int *gcount, uint32_t off)
+{
+    const int s = cfirst[c], e = clast[c], L = clevel[c];
+    if (e - s + 1 <= G) return 0;
+    uint32_t k = 0;
+    auto emit = [&](int start, int count) {
+        if (WRITE) {
+            gfirst[off + k] = start;
+            gcount[off + k] = count;
+        }
+        k++;
+    };
+    if (L == kMaxLevel) {
+        for (int t = s; t <= e; t += G) emit(t, min(G, e - t + 1));
+        return k;
+    }
+    const int shift = 60 - 3 * L;
+    // octants of the children holding more than G points
+    unsigned big = 0;
+    for (int t = s; t <= e;) {
+        const bool is_cell = cpl[t] >= L + 1;
+        const int cc = (t == s) ? c + 1 : (int)cbase[t];
+        const int tn = is_cell ? clast[cc] + 1 : t + 1;
+        if (tn - t > G) big |= 1u << ((keys[t] >> shift) & 7);
+        t = tn;
+    }
+    // runs of consecutive small children; a run also ends when the box spanned
+    // by its octants would cover a big child's octant
+    int t = s, run = 0, rs = s;
+    unsigned rmask = 0;
+    while (t <= e) {
+        const bool is_cell = cpl[t] >= L + 1;
+        const int cc = (t == s) ? c + 1 : (int)cbase[t];
+        const int tn = is_cell ? clast[cc] + 1 : t + 1;
+        const int size = tn - t;
+        const unsigned o = 1u << ((keys[t] >> shift) & 7);
+        if (size > G) {
+            if (run) emit(rs, run);
+            run = 0;
+            rmask = 0;
+        } else if (run + size > G || (run && (octant_box(rmask | o) & big))) {
+            emit(rs, run);
+            rs = t;
+            run = size;
+            rmask = o;
+        } else {
+            if (run == 0) rs = t;
+            run += size;
+            rmask |= o;
+        }
+        t = tn;
+    }
+    if (run) emit(rs, run);
+    return k;
+}
+
+__global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                              const uint32_t *__restrict__ cbase,
+                              const int *__restrict__ cfirst, const int *__restrict__ clast,
+                              const uint8_t *__restrict__ clevel, int ncell, int G, uint32_t *gcnt)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, nullptr, nullptr, 0);
+}
+
+__global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                              const uint32_t *__restrict__ cbase,
+                              const int *__restrict__ cfirst, const int *__restrict__ clast,
+                              const uint8_t *__restrict__ clevel, int ncell, int G, const uint32_t *__restrict__ goff,
+                              int *gfirst, int *gcount)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, gfirst, gcount, goff[c]);
 }
 
 // Children of each cell in octant order; parent links; child-cell counts.
@@ -603,6 +713,77 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
     *vout = va;
 }
 
+// Cell ranges of the sorted key array keys[0..n): cpl, cbase (n+1 entries),
+// block minima, and first/last/level of every cell.  n >= 2.
+struct Topo {
+    int8_t *cpl;
+    uint32_t *cbase;
+    int *first, *last;
+    uint8_t *level;
+    int ncell;
+};
+
+static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, DevBuf &bcnt, DevBuf &bbase,
+                     DevBuf &bm1, DevBuf &bm2, DevBuf &bfirst, DevBuf &blast, DevBuf &blevel, DevBuf *bskip,
+                     DevBuf *bpre, unsigned long long *deep)
+{
+    hipStream_t st = ctx->stream;
+    Topo t{};
+    t.cpl = ensure<int8_t>(bcpl, n);
+    uint32_t *ccnt = ensure<uint32_t>(bcnt, n + 1);
+    t.cbase = ensure<uint32_t>(bbase, n + 1);
+    k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, ccnt);
+    exclusive_scan_u32(ctx, ccnt, t.cbase, n);
+    int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64;
+    int8_t *m1 = ensure<int8_t>(bm1, n1 + 1), *m2 = ensure<int8_t>(bm2, n2 + 1);
+    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(t.cpl, n, m1, n1);
+    k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
+    uint32_t ncell = 0;
+    BH_HIP(hipMemcpyAsync(&ncell, t.cbase + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    t.ncell = (int)ncell;
+    t.first = ensure<int>(bfirst, ncell);
+    t.last = ensure<int>(blast, ncell);
+    t.level = ensure<uint8_t>(blevel, ncell);
+    int *cskip = bskip ? ensure<int>(*bskip, ncell) : nullptr;
+    int *pre = bpre ? ensure<int>(*bpre, (size_t)n + ncell) : nullptr;
+    k_cells<<<nblocks(n), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, n, t.first, t.last, t.level, cskip, pre, deep);
+    BH_HIP(hipGetLastError());
+    return t;
+}
+
+// Groups of the sorted test set for the fast walk (ctx->gfirst/gcount).
+static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G)
+{
+    hipStream_t st = ctx->stream;
+    if (n <= G || t == nullptr || t->ncell == 0) {
+        int64_t ng = (n + G - 1) / G;
+        std::vector<int> f(ng), c(ng);
+        for (int64_t g = 0; g < ng; g++) {
+            f[g] = (int)(g * G);
+            c[g] = (int)std::min<int64_t>(G, n - g * G);
+        }
+        int *gf = ensure<int>(ctx->gfirst, ng), *gc = ensure<int>(ctx->gcount, ng);
+        BH_HIP(hipMemcpyAsync(gf, f.data(), ng * sizeof(int), hipMemcpyHostToDevice, st));
+        BH_HIP(hipMemcpyAsync(gc, c.data(), ng * sizeof(int), hipMemcpyHostToDevice, st));
+        BH_HIP(hipStreamSynchronize(st));
+        ctx->ngroups = (int)ng;
+        return;
+    }
+    uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->ncell + 1);
+    uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->ncell + 1);
+    k_group_count<<<nblocks(t->ncell), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncell, G, gcnt);
+    exclusive_scan_u32(ctx, gcnt, goff, t->ncell);
+    uint32_t ng = 0;
+    BH_HIP(hipMemcpyAsync(&ng, goff + t->ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    int *gf = ensure<int>(ctx->gfirst, ng), *gc = ensure<int>(ctx->gcount, ng);
+    k_group_write<<<nblocks(t->ncell), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncell, G, goff,
+                                                      gf, gc);
+    BH_HIP(hipGetLastError());
+    ctx->ngroups = (int)ng;
+}
+
 // Morton keys of the mass set in input order, in the current box.
 void debug_keys(bh_ctx *ctx, uint64_t *d_out)
 {
@@ -614,6 +795,28 @@ void debug_keys(bh_ctx *ctx, uint64_t *d_out)
                                                           d_box, d_out, tmp, nullptr);
     BH_HIP(hipStreamSynchronize(ctx->stream));
     BH_HIP(hipFree(tmp));
+}
+
+// Groups of the sorted test set.  When the test set is exactly the tree's
+// body set the tree topology is reused; otherwise the test keys get their own.
+static void build_groups(bh_ctx *ctx, const bh_params &p, const uint64_t *ks, const uint64_t *tks)
+{
+    const int G = p.group_size > 0 ? p.group_size : 64;
+    const int64_t nt = ctx->ntest;
+    const Tree &T = ctx->tree;
+    if (ctx->self_test && nt == T.nb && T.ncell > 0) {
+        Topo t{ptr<int8_t>(ctx->cpl), ptr<uint32_t>(ctx->cbase), ptr<int>(ctx->cfirst), ptr<int>(ctx->clast),
+               ptr<uint8_t>(ctx->clevel), T.ncell};
+        make_groups(ctx, &t, ks, nt, G);
+        return;
+    }
+    if (nt <= G) {
+        make_groups(ctx, nullptr, nullptr, nt, G);
+        return;
+    }
+    Topo t = topology(ctx, tks, (int)nt, ctx->gcpl, ctx->gccnt, ctx->gcbase, ctx->gm1, ctx->gm2, ctx->gcf, ctx->gcl,
+                      ctx->gclv, nullptr, nullptr, nullptr);
+    make_groups(ctx, &t, tks, nt, G);
 }
 
 // Builds the tree of the uploaded mass set and the sorted test set.
@@ -656,10 +859,13 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, d_nm);
     uint64_t *ks;
     uint32_t *vs;
+    BH_HIP(hipEventRecord(ctx->ev[6], st));
     radix_sort_pairs(ctx, k0, k1, v0, v1, n, 0, 64, &ks, &vs);
+    BH_HIP(hipEventRecord(ctx->ev[7], st));
     int nb = 0;
     BH_HIP(hipMemcpyAsync(&nb, d_nm, sizeof(int), hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
+    BH_HIP(hipEventElapsedTime(&ctx->t_sort_ms, ctx->ev[6], ctx->ev[7]));
     for (int k = 0; k < 4; k++) ctx->box[k] = box[k];
     ctx->box_valid = true;
 
@@ -680,6 +886,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double4 *td = ensure<double4>(ctx->td, nt);
     int *tself = ensure<int>(ctx->tself, nt);
     uint32_t *tperm = ensure<uint32_t>(ctx->tperm, nt);
+    const uint64_t *tks_for_groups = ks;
     if (ctx->self_test) {
         BH_HIP(hipMemcpyAsync(tperm, vs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_pos, thi, tlo, td, tself, nb, 1);
@@ -692,6 +899,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         uint64_t *tks;
         uint32_t *tvs;
         radix_sort_pairs(ctx, tk0, tk1, tv0, tv1, nt, 0, 64, &tks, &tvs);
+        tks_for_groups = tks;
         BH_HIP(hipMemcpyAsync(tperm, tvs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_tpos, thi, tlo, td, tself, nb, 0);
     }
@@ -708,27 +916,20 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
             BH_HIP(hipMemcpyAsync(pre, &zero, sizeof(int), hipMemcpyHostToDevice, st));
         }
         BH_HIP(hipStreamSynchronize(st));
+        if (p.mode == BH_MODE_FAST) build_groups(ctx, p, ks, tks_for_groups);
         return;
     }
-    int8_t *cpl = ensure<int8_t>(ctx->cpl, nb);
-    uint32_t *ccnt = ensure<uint32_t>(ctx->ccnt, nb + 1);
-    uint32_t *cbase = ensure<uint32_t>(ctx->cbase, nb + 1);
-    k_cpl<<<nblocks(nb), kTpb, 0, st>>>(ks, nb, cpl, ccnt);
-    exclusive_scan_u32(ctx, ccnt, cbase, nb);
-    int n1 = (nb + 63) / 64, n2 = (n1 + 63) / 64;
-    int8_t *m1 = ensure<int8_t>(ctx->cmin1, n1 + 1), *m2 = ensure<int8_t>(ctx->cmin2, n2 + 1);
-    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(cpl, nb, m1, n1);
-    k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
-    uint32_t ncell = 0;
-    BH_HIP(hipMemcpyAsync(&ncell, cbase + nb, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BH_HIP(hipStreamSynchronize(st));
-    T.ncell = (int)ncell;
-    int *cfirst = ensure<int>(ctx->cfirst, ncell), *clast = ensure<int>(ctx->clast, ncell);
-    uint8_t *clevel = ensure<uint8_t>(ctx->clevel, ncell);
-    int *cskip = ensure<int>(ctx->cskip, ncell);
-    pre = ensure<int>(ctx->pre, (size_t)nb + ncell);
+    Topo tt = topology(ctx, ks, nb, ctx->cpl, ctx->ccnt, ctx->cbase, ctx->cmin1, ctx->cmin2, ctx->cfirst, ctx->clast,
+                       ctx->clevel, &ctx->cskip, &ctx->pre, cnt + 2);
+    const uint32_t ncell = (uint32_t)tt.ncell;
+    T.ncell = tt.ncell;
+    int8_t *cpl = tt.cpl;
+    uint32_t *cbase = tt.cbase;
+    int *cfirst = tt.first, *clast = tt.last;
+    uint8_t *clevel = tt.level;
+    int *cskip = ptr<int>(ctx->cskip);
+    pre = ptr<int>(ctx->pre);
     T.pre = pre;
-    k_cells<<<nblocks(nb), kTpb, 0, st>>>(cpl, cbase, m1, m2, nb, cfirst, clast, clevel, cskip, pre, cnt + 2);
     int *csub = ensure<int>(ctx->csub, 8 * (size_t)ncell);
     int *cnkids = ensure<int>(ctx->cnkids, ncell), *cnkc = ensure<int>(ctx->cnkc, ncell);
     int *cparent = ensure<int>(ctx->cparent, ncell), *cctr = ensure<int>(ctx->cctr, ncell);
@@ -754,6 +955,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cB = cB;
     T.cQ0 = cQ0;
     T.cQ1 = cQ1;
+    if (p.mode == BH_MODE_FAST) build_groups(ctx, p, ks, tks_for_groups);
 }
 
 }  // namespace bh

@@ -184,6 +184,7 @@ struct WalkArgs {
     const uint32_t *tperm;
     int64_t ntest;
     int ngroups;
+    const int *gfirst, *gcount;
     const float4 *bhi, *blo;
     const float4 *cA, *cB, *cQ0, *cQ1;
     const int *sub, *nkids, *first;
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         if (lane == 0) g = atomicAdd(a.gctr, 1);
         g = __shfl(g, 0, 64);
         if (g >= a.ngroups) break;
-        const int64_t gbase = (int64_t)g * G;
-        const int gcount = (int)min<int64_t>(G, a.ntest - gbase);
+        const int64_t gbase = a.gfirst[g];
+        const int gcount = a.gcount[g];
 
         // particles of this lane; inactive slots copy the first particle
         float4 h[NPL], l[NPL];
@@ -444,7 +445,10 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.tself = ptr<int>(ctx->tself);
     a.tperm = ptr<uint32_t>(ctx->tperm);
     a.ntest = nt;
-    a.ngroups = (int)((nt + G - 1) / G);
+    a.ngroups = ctx->ngroups;
+    a.gfirst = ptr<int>(ctx->gfirst);
+    a.gcount = ptr<int>(ctx->gcount);
+    (void)G;
     a.bhi = T.bhi;
     a.blo = T.blo;
     a.cA = T.cA;
@@ -474,6 +478,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
         BH_HIP(hipMemsetAsync(a.gctr, 0, sizeof(int), ctx->stream));
         BH_HIP(hipMemsetAsync(a.werr, 0, sizeof(int), ctx->stream));
         const bool soft0 = p.eps == 0.0;
+        BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
         if (npl == 2) {
             if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);
             else soft0 ? launch<2, false, true>(ctx, a, grid) : launch<2, false, false>(ctx, a, grid);
@@ -482,10 +487,14 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
             else soft0 ? launch<1, false, true>(ctx, a, grid) : launch<1, false, false>(ctx, a, grid);
         }
         BH_HIP(hipGetLastError());
+        BH_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
         int werr = 0;
         BH_HIP(hipMemcpyAsync(&werr, a.werr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
-        if (!werr) return;
+        if (!werr) {
+            BH_HIP(hipEventElapsedTime(&ctx->t_walk_kernel_ms, ctx->ev[4], ctx->ev[5]));
+            return;
+        }
         // frontier overflow: counters of the failed pass are discarded, retry with 4x capacity
         BH_HIP(hipMemsetAsync(a.counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
         ctx->fcap *= 4;

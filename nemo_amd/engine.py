@@ -39,7 +39,8 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("n2btot", C.c_int64), ("nbctot", C.c_int64), ("ncell", C.c_int64), ("deep_buckets", C.c_int64),
                 ("rmin", C.c_double * 3), ("rsize", C.c_double), ("t_tree_ms", C.c_double),
-                ("t_walk_ms", C.c_double), ("n_mlp", C.c_int64)]
+                ("t_walk_ms", C.c_double), ("n_mlp", C.c_int64), ("t_walk_kernel_ms", C.c_double),
+                ("t_sort_ms", C.c_double), ("ngroups", C.c_int64)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k == "rmin" else getattr(self, k)) for k, _ in self._fields_}
@@ -68,6 +69,8 @@ SIGNATURES = {
     "bh_debug_keys": (C.c_int, [_vp, _u64]),
     "bh_debug_cells": (C.c_int, [_vp, C.c_int64, _i32, _i32, _d, _d, _d, _i64]),
     "bh_debug_inject": (None, [_vp, C.c_int]),
+    "bh_debug_perm": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
+    "bh_debug_groups": (C.c_int, [_vp, C.c_int64, _i32, _i32, _i64]),
     "bh_mkplummer": (None, [C.c_int64, C.c_uint64, _d, _d, _d]),
 }
 
@@ -204,6 +207,20 @@ class Engine:
         q = np.zeros((n, 6))
         self._check(lib().bh_debug_cells(self._h, n, _p(lv, _i32), _p(nb, _i32), _p(m), _p(com), _p(q), _p(nc, _i64)))
         return dict(level=lv, nbody=nb, mass=m, pos=com, quad6=q)
+
+    def debug_perm(self):
+        p = np.zeros(self.ntest, np.uint32)
+        self._check(lib().bh_debug_perm(self._h, p.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return p
+
+    def debug_groups(self):
+        ng = np.zeros(1, np.int64)
+        self._check(lib().bh_debug_groups(self._h, 0, None, None, _p(ng, _i64)))
+        n = int(ng[0])
+        f = np.zeros(n, np.int32)
+        c = np.zeros(n, np.int32)
+        self._check(lib().bh_debug_groups(self._h, n, _p(f, _i32), _p(c, _i32), _p(ng, _i64)))
+        return f, c
 
     def inject(self, code):
         lib().bh_debug_inject(self._h, code)

@@ -544,3 +544,40 @@ void orc_morton(int n, const double *pos, const double *rmin, double rsize, uint
         keys[b] = key;
     }
 }
+
+/* ---- timed sample for the CPU baseline ----------------------------------- */
+#include <time.h>
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* Build the full tree (timed), then evaluate the bodies idx[0..ns) of the
+ * mass set, each skipping itself (timed).  The caller extrapolates
+ * per-body walk time to the whole set. */
+int orc_force_sample(int n, const double *mass, const double *pos, int ns, const int64_t *idx,
+                     double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
+                     double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, double *t_tree, double *t_walk)
+{
+    orc_tree *t;
+    double t0 = now_s();
+    int err = orc_tree_build(&t, n, mass, pos, rmin, rsize, (int)(fcells * n), usequad);
+    double t1 = now_s();
+    *n2btot = *nbctot = 0;
+    if (err) { orc_tree_free(t); return err; }
+    for (int k = 0; k < ns; k++) {
+        int a, b;
+        int64_t i = idx[k];
+        orc_tree_grav(t, pos + 3 * i, (int)i, tol, eps, phi + k, acc + 3 * (size_t)k, &a, &b);
+        *n2btot += a;
+        *nbctot += b;
+    }
+    double t2 = now_s();
+    *t_tree = t1 - t0;
+    *t_walk = t2 - t1;
+    orc_tree_free(t);
+    return 0;
+}
