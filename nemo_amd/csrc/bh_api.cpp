@@ -182,7 +182,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
-                          &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv};
+                          &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)

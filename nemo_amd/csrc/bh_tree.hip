@@ -70,33 +70,37 @@ __device__ __forceinline__ bool fits(const double *x, const double4 b)
     return (0.0 <= r0 && r0 < 1.0) && (0.0 <= r1 && r1 < 1.0) && (0.0 <= r2 && r2 < 1.0);
 }
 
-// First massive body at index >= start that lies outside the current box.
-// boxi: [0] first (min), [1] start, [2] changed, [3] non-finite seen
-__global__ void k_box_first(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                            const double4 *__restrict__ box, int *boxi)
+// Massive bodies outside the starting box (the only ones that can ever make
+// expandbox grow it, since the box only grows).  Unordered; indices kept.
+__global__ void k_box_cand(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
+                           const double4 *__restrict__ box, int *cand, int *boxi)
 {
     const double4 b = *box;
-    const int64_t start = boxi[1];
-    int best = INT32_MAX;
-    int bad = 0;
-    for (int64_t i = start + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (mass[i] == 0.0) continue;
         const double *x = pos + 3 * i;
         if (!fits(x, b)) {
-            if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) bad = 1;
-            best = min(best, (int)i);
+            if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) atomicOr(&boxi[3], 1);
+            int slot = atomicAdd(&boxi[4], 1);
+            cand[slot] = (int)i;
         }
     }
+}
+
+// First candidate (lowest input index >= start) outside the current box.
+__global__ void k_box_first_c(const double *__restrict__ pos, const int *__restrict__ cand,
+                              const double4 *__restrict__ box, int *boxi)
+{
+    const double4 b = *box;
+    const int start = boxi[1], nc = boxi[4];
+    int best = INT32_MAX;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        int i = cand[k];
+        if (i >= start && i < best && !fits(pos + 3 * (int64_t)i, b)) best = i;
+    }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        best = min(best, __shfl_xor(best, d, 64));
-        bad |= __shfl_xor(bad, d, 64);
-    }
-    if (lane_id() == 0) {
-        if (best != INT32_MAX) atomicMin(&boxi[0], best);
-        if (bad) atomicOr(&boxi[3], 1);
-    }
+    for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+    if (lane_id() == 0 && best != INT32_MAX) atomicMin(&boxi[0], best);
 }
 
 // Expand the box for the body found by k_box_first (single thread).
@@ -105,6 +109,7 @@ __global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *
     int first = boxi[0];
     boxi[2] = 0;
     if (first == INT32_MAX || boxi[3]) return;
+    boxi[5] += 1;
     const double *x = pos + 3 * (int64_t)first;
     double4 b = *box;
     for (int guard = 0; guard < 2000 && !fits(x, b); guard++) {
@@ -371,10 +376,11 @@ __global__ void k_cmin(const int8_t *__restrict__ cpl, int nb, int8_t *m1, int n
 
 // first j >= s with cpl[j] < L (exists: cpl[nb-1] = -1)
 __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restrict__ m1,
-                         const int8_t *__restrict__ m2, int s, int L)
+                         const int8_t *__restrict__ m2, const int8_t *__restrict__ m3, int s, int L)
 {
     int j = s;
     for (;;) {
+        if ((j & 262143) == 0 && m3[j >> 18] >= L) { j += 262144; continue; }
         if ((j & 4095) == 0 && m2[j >> 12] >= L) { j += 4096; continue; }
         if ((j & 63) == 0 && m1[j >> 6] >= L) { j += 64; continue; }
         if (cpl[j] < L) return j;
@@ -384,7 +390,8 @@ __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restric
 
 // Cells starting at body s: levels cpl[s-1]+1 .. cpl[s].
 __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
-                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, int nb,
+                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
+                        int nb,
                         int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
                         unsigned long long *deep)
 {
@@ -395,7 +402,7 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restri
     uint32_t cb = cbase[s];
     for (int L = l0; L <= c; L++) {
         int id = cb + (L - l0);
-        int e = range_end(cpl, m1, m2, s, L);
+        int e = range_end(cpl, m1, m2, m3, s, L);
         cfirst[id] = s;
         clast[id] = e;
         clevel[id] = (uint8_t)L;
@@ -585,10 +592,10 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
                 m = b.w; x = b.x; y = b.y; z = b.z;
             } else {
                 int cc = ~r;
-                m = __builtin_nontemporal_load(cm + cc);
-                x = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc);
-                y = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 1);
-                z = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 2);
+                m = cm[cc];
+                x = ccom[3 * (int64_t)cc];
+                y = ccom[3 * (int64_t)cc + 1];
+                z = ccom[3 * (int64_t)cc + 2];
             }
         }
         M += m;
@@ -616,10 +623,10 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
                     m = b.w; x = b.x; y = b.y; z = b.z;
                 } else {
                     int cc = ~r;
-                    m = __builtin_nontemporal_load(cm + cc);
-                    x = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc);
-                    y = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 1);
-                    z = __builtin_nontemporal_load(ccom + 3 * (int64_t)cc + 2);
+                    m = cm[cc];
+                    x = ccom[3 * (int64_t)cc];
+                    y = ccom[3 * (int64_t)cc + 1];
+                    z = ccom[3 * (int64_t)cc + 2];
                     qc = cq + 6 * (int64_t)cc;
                 }
             }
@@ -634,7 +641,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
                 double tm = (dr[ia[u]] * dr[ib[u]]) * 3.0;
                 tm = tm - idr;
                 tm = tm * m;
-                if (qc) tm = tm + __builtin_nontemporal_load(qc + u);
+                if (qc) tm = tm + qc[u];
                 q[u] = q[u] + tm;
             }
         }
@@ -657,24 +664,46 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
 }
 
-__global__ void k_moments(const int *__restrict__ csub, const int *__restrict__ cfirst, const int *__restrict__ clast,
-                          const uint8_t *__restrict__ clevel, const int *__restrict__ cnkc,
-                          const int *__restrict__ cparent, int *cctr, int ncell, const double4 *__restrict__ bd,
-                          double *cm, double *ccom, double *cq, float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
-                          double rsize, int usequad)
+// Cells bucketed by level so the moments can be computed level by level,
+// deepest first; a kernel boundary separates the levels.
+__global__ void k_level_hist(const uint8_t *__restrict__ clevel, int ncell, uint32_t *hist)
 {
+    __shared__ uint32_t h[kMaxLevel + 1];
+    if (threadIdx.x <= kMaxLevel) h[threadIdx.x] = 0;
+    __syncthreads();
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell || cnkc[c] != 0) return;
-    for (;;) {
-        cell_moments(c, csub, cfirst, clast, clevel, bd, cm, ccom, cq, cA, cB, cQ0, cQ1, rsize, usequad);
-        int p = cparent[c];
-        if (p < 0) return;
-        __threadfence();
-        int done = atomicAdd(&cctr[p], 1);
-        if (done != cnkc[p] - 1) return;
-        __threadfence();
-        c = p;
+    if (c < ncell) atomicAdd(&h[clevel[c]], 1u);
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevel && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_level_scatter(const uint8_t *__restrict__ clevel, int ncell, uint32_t *cursor, int *list)
+{
+    __shared__ uint32_t h[kMaxLevel + 1], base[kMaxLevel + 1];
+    if (threadIdx.x <= kMaxLevel) h[threadIdx.x] = 0;
+    __syncthreads();
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t r = 0;
+    int L = 0;
+    if (c < ncell) {
+        L = clevel[c];
+        r = atomicAdd(&h[L], 1u);
     }
+    __syncthreads();
+    if (threadIdx.x <= kMaxLevel && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (c < ncell) list[base[L] + r] = c;
+}
+
+__global__ void k_moments_level(const int *__restrict__ list, int cnt, const int *__restrict__ csub,
+                                const int *__restrict__ cfirst, const int *__restrict__ clast,
+                                const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *cm,
+                                double *ccom, double *cq, float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
+                                double rsize, int usequad)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    cell_moments(list[k], csub, cfirst, clast, clevel, bd, cm, ccom, cq, cA, cB, cQ0, cQ1, rsize, usequad);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -734,10 +763,13 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, Dev
     t.cbase = ensure<uint32_t>(bbase, n + 1);
     k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, ccnt);
     exclusive_scan_u32(ctx, ccnt, t.cbase, n);
-    int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64;
-    int8_t *m1 = ensure<int8_t>(bm1, n1 + 1), *m2 = ensure<int8_t>(bm2, n2 + 1);
+    int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64, n3 = (n2 + 63) / 64;
+    int8_t *m1 = ensure<int8_t>(bm1, n1 + n2 + n3 + 3);
+    int8_t *m2 = m1 + n1 + 1, *m3 = m2 + n2 + 1;
+    (void)bm2;
     k_cmin<<<nblocks(n1), kTpb, 0, st>>>(t.cpl, n, m1, n1);
     k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
+    k_cmin<<<nblocks(n3), kTpb, 0, st>>>(m2, n2, m3, n3);
     uint32_t ncell = 0;
     BH_HIP(hipMemcpyAsync(&ncell, t.cbase + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
@@ -747,7 +779,7 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, Dev
     t.level = ensure<uint8_t>(blevel, ncell);
     int *cskip = bskip ? ensure<int>(*bskip, ncell) : nullptr;
     int *pre = bpre ? ensure<int>(*bpre, (size_t)n + ncell) : nullptr;
-    k_cells<<<nblocks(n), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, n, t.first, t.last, t.level, cskip, pre, deep);
+    k_cells<<<nblocks(n), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, t.first, t.last, t.level, cskip, pre, deep);
     BH_HIP(hipGetLastError());
     return t;
 }
@@ -826,7 +858,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int64_t n = ctx->nmass;
     double *d_pos = ptr<double>(ctx->pos), *d_mass = ptr<double>(ctx->mass);
     double4 *d_box = ensure<double4>(ctx->dbox, 1);
-    int *d_boxi = ensure<int>(ctx->dboxi, 4);
+    int *d_boxi = ensure<int>(ctx->dboxi, 8);
 
     // -- box: start from the caller's box (or the previous one) and expand in input order
     double box[4];
@@ -836,13 +868,18 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         for (int k = 0; k < 3; k++) box[k] = p.rmin_given ? p.rmin[k] : -p.rsize / 2.0;
         box[3] = p.rsize;
     }
-    int boxi[4] = {INT32_MAX, 0, 0, 0};
+    // boxi: first, start, changed (last round), non-finite, ncand, expansions
+    int boxi[6] = {INT32_MAX, 0, 0, 0, 0, 0};
     BH_HIP(hipMemcpyAsync(d_box, box, sizeof(double4), hipMemcpyHostToDevice, st));
     BH_HIP(hipMemcpyAsync(d_boxi, boxi, sizeof(boxi), hipMemcpyHostToDevice, st));
     const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
-    for (int it = 0; it < 4096; it++) {
-        k_box_first<<<box_grid, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, d_boxi);
-        k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
+    int *cand = ensure<int>(ctx->bcand, n);
+    k_box_cand<<<box_grid, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, cand, d_boxi);
+    for (int batch = 0; batch < 512; batch++) {
+        for (int r = 0; r < 8; r++) {
+            k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, d_box, d_boxi);
+            k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
+        }
         BH_HIP(hipMemcpyAsync(boxi, d_boxi, sizeof(boxi), hipMemcpyDeviceToHost, st));
         BH_HIP(hipStreamSynchronize(st));
         if (boxi[3]) throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
@@ -939,8 +976,25 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double *cq = ensure<double>(ctx->cq, 6 * (size_t)ncell);
     float4 *cA = ensure<float4>(ctx->cA, ncell), *cB = ensure<float4>(ctx->cB, ncell);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, ncell), *cQ1 = ensure<float4>(ctx->cQ1, ncell);
-    k_moments<<<nblocks(ncell), kTpb, 0, st>>>(csub, cfirst, clast, clevel, cnkc, cparent, cctr, (int)ncell, bd, cm, ccom,
-                                               cq, cA, cB, cQ0, cQ1, box[3], p.usequad);
+    uint32_t *lhist = ensure<uint32_t>(ctx->lhist, 2 * (kMaxLevel + 1));
+    int *llist = ensure<int>(ctx->llist, ncell);
+    BH_HIP(hipMemsetAsync(lhist, 0, (kMaxLevel + 1) * sizeof(uint32_t), st));
+    k_level_hist<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lhist);
+    uint32_t lcnt[kMaxLevel + 1], loff[kMaxLevel + 1];
+    BH_HIP(hipMemcpyAsync(lcnt, lhist, sizeof lcnt, hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    uint32_t acc = 0;
+    for (int L = 0; L <= kMaxLevel; L++) {
+        loff[L] = acc;
+        acc += lcnt[L];
+    }
+    BH_HIP(hipMemcpyAsync(lhist + kMaxLevel + 1, loff, sizeof loff, hipMemcpyHostToDevice, st));
+    k_level_scatter<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lhist + kMaxLevel + 1, llist);
+    for (int L = kMaxLevel; L >= 0; L--) {
+        if (!lcnt[L]) continue;
+        k_moments_level<<<nblocks(lcnt[L]), kTpb, 0, st>>>(llist + loff[L], (int)lcnt[L], csub, cfirst, clast, clevel,
+                                                           bd, cm, ccom, cq, cA, cB, cQ0, cQ1, box[3], p.usequad);
+    }
     BH_HIP(hipGetLastError());
     T.first = cfirst;
     T.last = clast;
