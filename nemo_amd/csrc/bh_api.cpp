@@ -178,11 +178,12 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself,
                           &ctx->cpl,    &ctx->ccnt,   &ctx->cbase,  &ctx->cmin1,   &ctx->cmin2,  &ctx->cfirst,
                           &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->cnkc,   &ctx->cparent,
-                          &ctx->cctr,   &ctx->cskip,  &ctx->pre,    &ctx->cm,      &ctx->ccom,   &ctx->cq,
+                          &ctx->cctr,   &ctx->cskip,  &ctx->pre,    &ctx->cmo,
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
-                          &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand};
+                          &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
+                          &ctx->lvals0, &ctx->lvals1};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
@@ -295,9 +296,15 @@ bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, doubl
             if (level) level[i] = lv[i];
             if (nbody) nbody[i] = l[i] - f[i] + 1;
         }
-        if (mass) BH_HIP(hipMemcpy(mass, ctx->tree.cm, k * sizeof(double), hipMemcpyDeviceToHost));
-        if (com) BH_HIP(hipMemcpy(com, ctx->tree.ccom, 3 * k * sizeof(double), hipMemcpyDeviceToHost));
-        if (quad6) BH_HIP(hipMemcpy(quad6, ctx->tree.cq, 6 * k * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> mo(10 * k);
+        BH_HIP(hipMemcpy(mo.data(), ctx->tree.mo, 10 * k * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < k; i++) {
+            if (mass) mass[i] = mo[10 * i];
+            for (int a = 0; a < 3; a++)
+                if (com) com[3 * i + a] = mo[10 * i + 1 + a];
+            for (int a = 0; a < 6; a++)
+                if (quad6) quad6[6 * i + a] = mo[10 * i + 4 + a];
+        }
     });
 }
 
@@ -305,7 +312,7 @@ bh_status bh_debug_perm(bh_ctx *ctx, uint32_t *perm)
 {
     return guarded(ctx, [&] {
         if (!ctx->box_valid || !perm) throw ApiFail(BH_ERR_STATE, "no tree built");
-        BH_HIP(hipMemcpy(perm, ctx->tperm.p, ctx->ntest * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        BH_HIP(hipMemcpy(perm, ctx->tree.tperm, ctx->ntest * sizeof(uint32_t), hipMemcpyDeviceToHost));
     });
 }
 

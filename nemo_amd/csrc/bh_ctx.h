@@ -23,7 +23,7 @@ struct bh_ctx {
     // inputs (device, input order)
     bh::DevBuf mass, pos, tpos;
     // box state: double4 box, ints {first, start, changed, nonfinite}
-    bh::DevBuf dbox, dboxi, bcand;
+    bh::DevBuf dbox, dboxi, bcand, bcount;
     // sorting
     bh::DevBuf keys0, keys1, vals0, vals1, hist, scan_tmp;
     bh::DevBuf tkeys0, tkeys1, tvals0, tvals1;
@@ -34,7 +34,7 @@ struct bh_ctx {
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre;
-    bh::DevBuf cm, ccom, cq, cA, cB, cQ0, cQ1, lhist, llist;
+    bh::DevBuf cmo, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
     bh::DevBuf gfirst, gcount, gscan, goff;

@@ -57,9 +57,10 @@ __device__ __forceinline__ void gravsub64(Acc64 &a, double dx, double dy, double
 }
 
 __global__ __launch_bounds__(kTpb) void k_walk_strict(
-    const double4 *__restrict__ td, const int *__restrict__ tself, const uint32_t *__restrict__ tperm, int64_t nt,
+    const double4 *__restrict__ td, const int *__restrict__ tself, int selfmode, const uint32_t *__restrict__ tperm,
+    int64_t nt,
     const int *__restrict__ pre, const int *__restrict__ skip, const uint8_t *__restrict__ level,
-    const double *__restrict__ cm, const double *__restrict__ ccom, const double *__restrict__ cq,
+    const double *__restrict__ mo,
     const double4 *__restrict__ bd, int nnode, double rsize, double tol, double eps, int usequad, double *phi_out,
     double *acc_out, unsigned long long *counters)
 {
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kTpb) void k_walk_strict(
     unsigned long long n2b = 0, nbc = 0;
     if (i < nt) {
         const double4 x0 = td[i];
-        const int self = tself[i];
+        const int self = selfmode == 1 ? (int)i : (selfmode == 2 ? tself[i] : -1);
         const double tolsq = tol * tol;
         const double dsq0 = rsize * rsize;
         Acc64 a;
@@ -76,9 +77,10 @@ __global__ __launch_bounds__(kTpb) void k_walk_strict(
             int r = pre[p];
             if (r < 0) {
                 int c = ~r;
-                double dx = ccom[3 * (int64_t)c] - x0.x;
-                double dy = ccom[3 * (int64_t)c + 1] - x0.y;
-                double dz = ccom[3 * (int64_t)c + 2] - x0.z;
+                const double *mc = mo + 10 * (int64_t)c;
+                double dx = mc[1] - x0.x;
+                double dy = mc[2] - x0.y;
+                double dz = mc[3] - x0.z;
                 double drsq = dx * dx;
                 drsq += dy * dy;
                 drsq += dz * dz;
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(kTpb) void k_walk_strict(
                     p++;
                     continue;
                 }
-                gravsub64(a, dx, dy, dz, drsq, cm[c], eps, usequad ? cq + 6 * (int64_t)c : nullptr);
+                gravsub64(a, dx, dy, dz, drsq, mc[0], eps, usequad ? mc + 4 : nullptr);
                 nbc++;
                 p = skip[c];
             } else {
@@ -171,8 +173,7 @@ void walk_strict(bh_ctx *ctx, const bh_params &p)
         return;
     }
     int grid = (int)std::max<int64_t>(1, (nt + kTpb - 1) / kTpb);
-    k_walk_strict<<<grid, kTpb, 0, ctx->stream>>>(ptr<double4>(ctx->td), ptr<int>(ctx->tself), ptr<uint32_t>(ctx->tperm),
-                                                  nt, T.pre, T.skip, T.level, T.cm, T.ccom, T.cq, T.bd, nnode, T.box[3],
+    k_walk_strict<<<grid, kTpb, 0, ctx->stream>>>(T.td, T.tself, T.selfmode, T.tperm, nt, T.pre, T.skip, T.level, T.mo, T.bd, nnode, T.box[3],
                                                   p.tol, p.eps, p.usequad, phi, acc, cnt);
     BH_HIP(hipGetLastError());
 }

@@ -42,13 +42,17 @@ struct Tree {
     const uint8_t *level = nullptr;
     const int *skip = nullptr;      // pre-order position after the subtree
     const int *pre = nullptr;       // pre-order node refs (nb + ncell entries)
-    const double *cm = nullptr;     // FP64 mass
-    const double *ccom = nullptr;   // FP64 c.o.m. [ncell][3]
-    const double *cq = nullptr;     // FP64 quadrupole [ncell][6] xx xy xz yy yz zz
+    const double *mo = nullptr;     // FP64 moments, 10 per cell: m, x, y, z, qxx qxy qxz qyy qyz qzz
     const float4 *cA = nullptr;     // FP32 com hi xyz, mass
     const float4 *cB = nullptr;     // FP32 com lo xyz, size^2
     const float4 *cQ0 = nullptr;    // qxx qxy qxz qyy
     const float4 *cQ1 = nullptr;    // qyz qzz 0 0
+    // test set in its sorted order
+    const float4 *thi = nullptr, *tlo = nullptr;
+    const double4 *td = nullptr;
+    const uint32_t *tperm = nullptr;   // sorted test index -> input index
+    const int *tself = nullptr;        // selfmode 2: tree body of each test point or -1
+    int selfmode = 0;                  // 0 no self pairs, 1 test index == tree body, 2 tself[]
 };
 
 constexpr int kNoChild = INT32_MIN;  // empty octant in Tree::sub

@@ -71,28 +71,56 @@ __device__ __forceinline__ bool fits(const double *x, const double4 b)
 }
 
 // Massive bodies outside the starting box (the only ones that can ever make
-// expandbox grow it, since the box only grows).  Unordered; indices kept.
-__global__ void k_box_cand(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                           const double4 *__restrict__ box, int *cand, int *boxi)
+// expandbox grow it, since the box only grows), compacted in input order:
+// per-block counts, a scan, then per-block writes.  Block b owns the input
+// range [b*chunk, (b+1)*chunk).
+__device__ __forceinline__ bool outside_massive(const double *__restrict__ pos, const double *__restrict__ mass,
+                                                int64_t i, int64_t n, const double4 b, int *bad)
+{
+    if (i >= n || mass[i] == 0.0) return false;
+    const double *x = pos + 3 * i;
+    if (fits(x, b)) return false;
+    if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) *bad = 1;
+    return true;
+}
+
+__global__ void k_box_count(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
+                            const double4 *__restrict__ box, int64_t chunk, uint32_t *bcount, int *boxi)
 {
     const double4 b = *box;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (mass[i] == 0.0) continue;
-        const double *x = pos + 3 * i;
-        if (!fits(x, b)) {
-            if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) atomicOr(&boxi[3], 1);
-            int slot = atomicAdd(&boxi[4], 1);
-            cand[slot] = (int)i;
-        }
+    const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    int cnt = 0, bad = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) cnt += outside_massive(pos, mass, i, hi, b, &bad);
+    uint32_t tot;
+    block_excl_scan((uint32_t)cnt, &tot);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
+    if (bad) atomicOr(&boxi[3], 1);
+}
+
+__global__ void k_box_write(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
+                            const double4 *__restrict__ box, int64_t chunk, const uint32_t *__restrict__ boff,
+                            int *cand)
+{
+    const double4 b = *box;
+    const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+    uint32_t run = boff[blockIdx.x];
+    int bad = 0;
+    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool out = outside_massive(pos, mass, i, hi, b, &bad);
+        uint32_t tot;
+        const uint32_t off = block_excl_scan(out ? 1u : 0u, &tot);
+        if (out) cand[run + off] = (int)i;
+        run += tot;
     }
 }
 
 // First candidate (lowest input index >= start) outside the current box.
 __global__ void k_box_first_c(const double *__restrict__ pos, const int *__restrict__ cand,
-                              const double4 *__restrict__ box, int *boxi)
+                              const uint32_t *__restrict__ ncand, const double4 *__restrict__ box, int *boxi)
 {
     const double4 b = *box;
-    const int start = boxi[1], nc = boxi[4];
+    const int start = boxi[1], nc = (int)*ncand;
     int best = INT32_MAX;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
         int i = cand[k];
@@ -156,22 +184,27 @@ __device__ __forceinline__ uint64_t morton_of(const double *x, const double4 b)
 
 // Mass set: massless bodies get bit 63 so they sort after every massive body.
 __global__ void k_keys(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals, int *nmassive)
+                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals)
 {
     const double4 b = *box;
-    int cnt = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         uint64_t k = morton_of(pos + 3 * i, b);
-        if (mass == nullptr || mass[i] != 0.0) cnt++;
-        else k |= (1ULL << 63);
+        if (mass != nullptr && mass[i] == 0.0) k |= (1ULL << 63);
         keys[i] = k;
         vals[i] = (uint32_t)i;
     }
-    if (nmassive) {
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
-        if (lane_id() == 0 && cnt) atomicAdd(nmassive, cnt);
+}
+
+// Number of massive bodies = first sorted key with bit 63 set.
+__global__ void k_count_massive(const uint64_t *__restrict__ keys, int64_t n, int *nb)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (keys[mid] >> 63) hi = mid;
+        else lo = mid + 1;
     }
+    *nb = (int)lo;
 }
 
 // ---- exclusive scan (uint32) -------------------------------------------------
@@ -388,29 +421,40 @@ __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restric
     }
 }
 
-// Cells starting at body s: levels cpl[s-1]+1 .. cpl[s].
+// One thread per cell.  The cells that start at body s are the levels
+// cpl[s-1]+1 .. cpl[s] with ids cbase[s] ..; the owner of cell id is the last
+// s with cbase[s] <= id.
 __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
                         const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
-                        int nb,
-                        int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
+                        int nb, int ncell, int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
                         unsigned long long *deep)
+{
+    int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= ncell) return;
+    int lo = 0, hi = nb;           // upper_bound over cbase[0..nb)
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cbase[mid] <= (uint32_t)id) lo = mid + 1;
+        else hi = mid;
+    }
+    const int s = lo - 1;
+    const int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
+    const int L = l0 + (id - (int)cbase[s]);
+    const int e = range_end(cpl, m1, m2, m3, s, L);
+    cfirst[id] = s;
+    clast[id] = e;
+    clevel[id] = (uint8_t)L;
+    if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
+    if (pre) pre[s + id] = ~id;
+    if (deep && L == kMaxLevel) atomicAdd(deep, 1ULL);
+}
+
+// Pre-order position of body s: after the cells that start at s.
+__global__ void k_body_pre(const uint32_t *__restrict__ cbase, int nb, int *pre)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nb) return;
-    int c = cpl[s];
-    int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
-    uint32_t cb = cbase[s];
-    for (int L = l0; L <= c; L++) {
-        int id = cb + (L - l0);
-        int e = range_end(cpl, m1, m2, m3, s, L);
-        cfirst[id] = s;
-        clast[id] = e;
-        clevel[id] = (uint8_t)L;
-        if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
-        if (pre) pre[s + id] = ~id;
-        if (deep && L == kMaxLevel) atomicAdd(deep, 1ULL);
-    }
-    if (pre) pre[s + cb + (c >= l0 ? c - l0 + 1 : 0)] = s;
+    pre[s + cbase[s + 1]] = s;
 }
 
 // ---- groups of test points -------------------------------------------------
@@ -570,88 +614,110 @@ __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__re
 // compiled with -ffp-contract=off).
 __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__restrict__ cfirst,
                              const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
-                             const double4 *__restrict__ bd, double *cm, double *ccom, double *cq,
+                             const double4 *__restrict__ bd, double *mo,
                              float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad)
 {
     const int L = clevel[c];
-    const int *sub = csub + 8 * (int64_t)c;
     double M = 0.0, px = 0.0, py = 0.0, pz = 0.0;
-    const bool bucket = (L == kMaxLevel);
-    const int s = cfirst[c], e = clast[c];
-    const int nchild = bucket ? (e - s + 1) : 8;
-    for (int k = 0; k < nchild; k++) {
-        double m, x, y, z;
-        if (bucket) {
-            double4 b = bd[s + k];
-            m = b.w; x = b.x; y = b.y; z = b.z;
-        } else {
-            int r = sub[k];
-            if (r == kNoChild) continue;
-            if (r >= 0) {
-                double4 b = bd[r];
-                m = b.w; x = b.x; y = b.y; z = b.z;
-            } else {
-                int cc = ~r;
-                m = cm[cc];
-                x = ccom[3 * (int64_t)cc];
-                y = ccom[3 * (int64_t)cc + 1];
-                z = ccom[3 * (int64_t)cc + 2];
-            }
-        }
-        M += m;
-        double t0 = x * m, t1 = y * m, t2 = z * m;
-        px = px + t0;
-        py = py + t1;
-        pz = pz + t2;
-    }
-    px = px / M;
-    py = py / M;
-    pz = pz / M;
     double q[6] = {0, 0, 0, 0, 0, 0};
-    if (usequad) {
-        for (int k = 0; k < nchild; k++) {
-            double m, x, y, z;
-            const double *qc = nullptr;
-            if (bucket) {
-                double4 b = bd[s + k];
-                m = b.w; x = b.x; y = b.y; z = b.z;
-            } else {
-                int r = sub[k];
-                if (r == kNoChild) continue;
-                if (r >= 0) {
-                    double4 b = bd[r];
-                    m = b.w; x = b.x; y = b.y; z = b.z;
-                } else {
-                    int cc = ~r;
-                    m = cm[cc];
-                    x = ccom[3 * (int64_t)cc];
-                    y = ccom[3 * (int64_t)cc + 1];
-                    z = ccom[3 * (int64_t)cc + 2];
-                    qc = cq + 6 * (int64_t)cc;
+    if (L == kMaxLevel) {
+        // bucket: the bodies s..e are the children, in sorted order
+        const int s = cfirst[c], e = clast[c];
+        for (int t = s; t <= e; t++) {
+            double4 b = bd[t];
+            M += b.w;
+            double t0 = b.x * b.w, t1 = b.y * b.w, t2 = b.z * b.w;
+            px = px + t0;
+            py = py + t1;
+            pz = pz + t2;
+        }
+        px = px / M;
+        py = py / M;
+        pz = pz / M;
+        if (usequad) {
+            for (int t = s; t <= e; t++) {
+                double4 b = bd[t];
+                double dr[3] = {b.x - px, b.y - py, b.z - pz};
+                double drsq = dr[0] * dr[0];
+                drsq += dr[1] * dr[1];
+                drsq += dr[2] * dr[2];
+                const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+                for (int u = 0; u < 6; u++) {
+                    double idr = (ia[u] == ib[u] ? 1.0 : 0.0) * drsq;
+                    double tm = (dr[ia[u]] * dr[ib[u]]) * 3.0;
+                    tm = tm - idr;
+                    tm = tm * b.w;
+                    q[u] = q[u] + tm;
                 }
             }
-            double dr[3] = {x - px, y - py, z - pz};
-            double drsq = dr[0] * dr[0];
-            drsq += dr[1] * dr[1];
-            drsq += dr[2] * dr[2];
-            const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+        }
+    } else {
+        // up to 8 children in octant order; all loads issued before the sums
+        const int4 s0 = reinterpret_cast<const int4 *>(csub)[2 * (int64_t)c];
+        const int4 s1 = reinterpret_cast<const int4 *>(csub)[2 * (int64_t)c + 1];
+        const int r[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+        double m[8], x[8], y[8], z[8];
 #pragma unroll
-            for (int u = 0; u < 6; u++) {
-                double idr = (ia[u] == ib[u] ? 1.0 : 0.0) * drsq;
-                double tm = (dr[ia[u]] * dr[ib[u]]) * 3.0;
-                tm = tm - idr;
-                tm = tm * m;
-                if (qc) tm = tm + qc[u];
-                q[u] = q[u] + tm;
+        for (int k = 0; k < 8; k++) {
+            m[k] = x[k] = y[k] = z[k] = 0.0;
+            if (r[k] == kNoChild) continue;
+            if (r[k] >= 0) {
+                double4 b = bd[r[k]];
+                m[k] = b.w; x[k] = b.x; y[k] = b.y; z[k] = b.z;
+            } else {
+                const double2 *mc = reinterpret_cast<const double2 *>(mo + 10 * (int64_t)(~r[k]));
+                double2 a = mc[0], b = mc[1];
+                m[k] = a.x; x[k] = a.y; y[k] = b.x; z[k] = b.y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (r[k] == kNoChild) continue;
+            M += m[k];
+            double t0 = x[k] * m[k], t1 = y[k] * m[k], t2 = z[k] * m[k];
+            px = px + t0;
+            py = py + t1;
+            pz = pz + t2;
+        }
+        px = px / M;
+        py = py / M;
+        pz = pz / M;
+        if (usequad) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (r[k] == kNoChild) continue;
+                double qc[6] = {0, 0, 0, 0, 0, 0};
+                const bool cell = r[k] < 0;
+                if (cell) {
+                    const double2 *mq = reinterpret_cast<const double2 *>(mo + 10 * (int64_t)(~r[k]) + 4);
+                    double2 a = mq[0], b = mq[1], d = mq[2];
+                    qc[0] = a.x; qc[1] = a.y; qc[2] = b.x; qc[3] = b.y; qc[4] = d.x; qc[5] = d.y;
+                }
+                double dr[3] = {x[k] - px, y[k] - py, z[k] - pz};
+                double drsq = dr[0] * dr[0];
+                drsq += dr[1] * dr[1];
+                drsq += dr[2] * dr[2];
+                const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+                for (int u = 0; u < 6; u++) {
+                    double idr = (ia[u] == ib[u] ? 1.0 : 0.0) * drsq;
+                    double tm = (dr[ia[u]] * dr[ib[u]]) * 3.0;
+                    tm = tm - idr;
+                    tm = tm * m[k];
+                    if (cell) tm = tm + qc[u];
+                    q[u] = q[u] + tm;
+                }
             }
         }
     }
-    cm[c] = M;
-    ccom[3 * (int64_t)c] = px;
-    ccom[3 * (int64_t)c + 1] = py;
-    ccom[3 * (int64_t)c + 2] = pz;
+    double *me = mo + 10 * (int64_t)c;
+    me[0] = M;
+    me[1] = px;
+    me[2] = py;
+    me[3] = pz;
 #pragma unroll
-    for (int u = 0; u < 6; u++) cq[6 * (int64_t)c + u] = q[u];
+    for (int u = 0; u < 6; u++) me[4 + u] = q[u];
     float4 A, B;
     split2(px, A.x, B.x);
     split2(py, A.y, B.y);
@@ -664,46 +730,34 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
 }
 
-// Cells bucketed by level so the moments can be computed level by level,
-// deepest first; a kernel boundary separates the levels.
-__global__ void k_level_hist(const uint8_t *__restrict__ clevel, int ncell, uint32_t *hist)
+// Cells bucketed by level (a one-digit radix sort of (level, cell id)) so the
+// moments can be computed level by level, deepest first, with a kernel
+// boundary between levels.
+__global__ void k_level_keys(const uint8_t *__restrict__ clevel, int ncell, uint64_t *keys, uint32_t *vals)
 {
-    __shared__ uint32_t h[kMaxLevel + 1];
-    if (threadIdx.x <= kMaxLevel) h[threadIdx.x] = 0;
-    __syncthreads();
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell) atomicAdd(&h[clevel[c]], 1u);
-    __syncthreads();
-    if (threadIdx.x <= kMaxLevel && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+    if (c >= ncell) return;
+    keys[c] = clevel[c];
+    vals[c] = (uint32_t)c;
 }
 
-__global__ void k_level_scatter(const uint8_t *__restrict__ clevel, int ncell, uint32_t *cursor, int *list)
+// Start of each digit 0..kMaxLevel+1 in the sorted order, from the scanned
+// digit-major histogram of the pass (entry d*nblk is digit d in block 0).
+__global__ void k_level_offsets(const uint32_t *__restrict__ hist_scan, int nblk, uint32_t *out)
 {
-    __shared__ uint32_t h[kMaxLevel + 1], base[kMaxLevel + 1];
-    if (threadIdx.x <= kMaxLevel) h[threadIdx.x] = 0;
-    __syncthreads();
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t r = 0;
-    int L = 0;
-    if (c < ncell) {
-        L = clevel[c];
-        r = atomicAdd(&h[L], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x <= kMaxLevel && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
-    __syncthreads();
-    if (c < ncell) list[base[L] + r] = c;
+    int d = threadIdx.x;
+    if (d <= kMaxLevel + 1) out[d] = hist_scan[(int64_t)d * nblk];
 }
 
 __global__ void k_moments_level(const int *__restrict__ list, int cnt, const int *__restrict__ csub,
                                 const int *__restrict__ cfirst, const int *__restrict__ clast,
-                                const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *cm,
-                                double *ccom, double *cq, float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
+                                const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *mo,
+                                float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
                                 double rsize, int usequad)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= cnt) return;
-    cell_moments(list[k], csub, cfirst, clast, clevel, bd, cm, ccom, cq, cA, cB, cQ0, cQ1, rsize, usequad);
+    cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -779,7 +833,10 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, Dev
     t.level = ensure<uint8_t>(blevel, ncell);
     int *cskip = bskip ? ensure<int>(*bskip, ncell) : nullptr;
     int *pre = bpre ? ensure<int>(*bpre, (size_t)n + ncell) : nullptr;
-    k_cells<<<nblocks(n), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, t.first, t.last, t.level, cskip, pre, deep);
+    if (ncell > 0)
+        k_cells<<<nblocks(ncell), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, (int)ncell, t.first, t.last, t.level,
+                                                 cskip, pre, deep);
+    if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, pre);
     BH_HIP(hipGetLastError());
     return t;
 }
@@ -824,7 +881,7 @@ void debug_keys(bh_ctx *ctx, uint64_t *d_out)
     uint32_t *tmp;
     BH_HIP(hipMalloc(&tmp, ctx->nmass * sizeof(uint32_t)));
     k_keys<<<nblocks(ctx->nmass), kTpb, 0, ctx->stream>>>(ptr<double>(ctx->pos), ptr<double>(ctx->mass), ctx->nmass,
-                                                          d_box, d_out, tmp, nullptr);
+                                                          d_box, d_out, tmp);
     BH_HIP(hipStreamSynchronize(ctx->stream));
     BH_HIP(hipFree(tmp));
 }
@@ -874,10 +931,16 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     BH_HIP(hipMemcpyAsync(d_boxi, boxi, sizeof(boxi), hipMemcpyHostToDevice, st));
     const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
     int *cand = ensure<int>(ctx->bcand, n);
-    k_box_cand<<<box_grid, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, cand, d_boxi);
+    const int64_t chunk = std::max<int64_t>(kTpb, (n + box_grid - 1) / box_grid);
+    const int nchunk = (int)((n + chunk - 1) / chunk);
+    uint32_t *bcount = ensure<uint32_t>(ctx->bcount, 2 * (size_t)nchunk + 2);
+    uint32_t *boff = bcount + nchunk + 1;
+    k_box_count<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, bcount, d_boxi);
+    exclusive_scan_u32(ctx, bcount, boff, nchunk);
+    k_box_write<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, boff, cand);
     for (int batch = 0; batch < 512; batch++) {
-        for (int r = 0; r < 8; r++) {
-            k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, d_box, d_boxi);
+        for (int r = 0; r < 6; r++) {
+            k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
             k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
         }
         BH_HIP(hipMemcpyAsync(boxi, d_boxi, sizeof(boxi), hipMemcpyDeviceToHost, st));
@@ -891,14 +954,14 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     uint64_t *k0 = ensure<uint64_t>(ctx->keys0, n), *k1 = ensure<uint64_t>(ctx->keys1, n);
     uint32_t *v0 = ensure<uint32_t>(ctx->vals0, n), *v1 = ensure<uint32_t>(ctx->vals1, n);
     int *d_nm = ensure<int>(ctx->nmassive, 1);
-    BH_HIP(hipMemsetAsync(d_nm, 0, sizeof(int), st));
     const int grid_n = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 16);
-    k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, d_nm);
+    k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0);
     uint64_t *ks;
     uint32_t *vs;
     BH_HIP(hipEventRecord(ctx->ev[6], st));
     radix_sort_pairs(ctx, k0, k1, v0, v1, n, 0, 64, &ks, &vs);
     BH_HIP(hipEventRecord(ctx->ev[7], st));
+    k_count_massive<<<1, 1, 0, st>>>(ks, n, d_nm);
     int nb = 0;
     BH_HIP(hipMemcpyAsync(&nb, d_nm, sizeof(int), hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
@@ -919,12 +982,27 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
 
     // -- test set, sorted by its own keys in the same box
     const int64_t nt = ctx->ntest;
-    float4 *thi = ensure<float4>(ctx->thi, nt), *tlo = ensure<float4>(ctx->tlo, nt);
-    double4 *td = ensure<double4>(ctx->td, nt);
-    int *tself = ensure<int>(ctx->tself, nt);
-    uint32_t *tperm = ensure<uint32_t>(ctx->tperm, nt);
+    const bool alias = ctx->self_test && nt == nb;
+    float4 *thi = alias ? nullptr : ensure<float4>(ctx->thi, nt), *tlo = alias ? nullptr : ensure<float4>(ctx->tlo, nt);
+    double4 *td = alias ? nullptr : ensure<double4>(ctx->td, nt);
+    int *tself = alias ? nullptr : ensure<int>(ctx->tself, nt);
+    uint32_t *tperm = alias ? nullptr : ensure<uint32_t>(ctx->tperm, nt);
     const uint64_t *tks_for_groups = ks;
-    if (ctx->self_test) {
+    T.thi = thi;
+    T.tlo = tlo;
+    T.td = td;
+    T.tperm = tperm;
+    T.tself = tself;
+    T.selfmode = ctx->self_test ? 2 : 0;
+    if (ctx->self_test && nt == nb) {
+        // the test set is the tree's body set in the same order
+        T.thi = bhi;
+        T.tlo = blo;
+        T.td = bd;
+        T.tperm = vs;
+        T.tself = nullptr;
+        T.selfmode = 1;
+    } else if (ctx->self_test) {
         BH_HIP(hipMemcpyAsync(tperm, vs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_pos, thi, tlo, td, tself, nb, 1);
     } else {
@@ -932,7 +1010,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         uint32_t *tv0 = ensure<uint32_t>(ctx->tvals0, nt), *tv1 = ensure<uint32_t>(ctx->tvals1, nt);
         double *d_tpos = ptr<double>(ctx->tpos);
         k_keys<<<(int)std::min<int64_t>(nblocks(nt), (int64_t)ctx->ncu * 16), kTpb, 0, st>>>(d_tpos, nullptr, nt, d_box,
-                                                                                            tk0, tv0, nullptr);
+                                                                                            tk0, tv0);
         uint64_t *tks;
         uint32_t *tvs;
         radix_sort_pairs(ctx, tk0, tk1, tv0, tv1, nt, 0, 64, &tks, &tvs);
@@ -972,28 +1050,27 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *cparent = ensure<int>(ctx->cparent, ncell), *cctr = ensure<int>(ctx->cctr, ncell);
     k_children<<<nblocks(ncell), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, (int)ncell, csub, cnkids, cnkc,
                                                 cparent, cctr);
-    double *cm = ensure<double>(ctx->cm, ncell), *ccom = ensure<double>(ctx->ccom, 3 * (size_t)ncell);
-    double *cq = ensure<double>(ctx->cq, 6 * (size_t)ncell);
+    double *mo = ensure<double>(ctx->cmo, 10 * (size_t)ncell);
     float4 *cA = ensure<float4>(ctx->cA, ncell), *cB = ensure<float4>(ctx->cB, ncell);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, ncell), *cQ1 = ensure<float4>(ctx->cQ1, ncell);
-    uint32_t *lhist = ensure<uint32_t>(ctx->lhist, 2 * (kMaxLevel + 1));
-    int *llist = ensure<int>(ctx->llist, ncell);
-    BH_HIP(hipMemsetAsync(lhist, 0, (kMaxLevel + 1) * sizeof(uint32_t), st));
-    k_level_hist<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lhist);
-    uint32_t lcnt[kMaxLevel + 1], loff[kMaxLevel + 1];
-    BH_HIP(hipMemcpyAsync(lcnt, lhist, sizeof lcnt, hipMemcpyDeviceToHost, st));
+    uint64_t *lk0 = ensure<uint64_t>(ctx->lkeys0, ncell), *lk1 = ensure<uint64_t>(ctx->lkeys1, ncell);
+    uint32_t *lv0 = ensure<uint32_t>(ctx->lvals0, ncell), *lv1 = ensure<uint32_t>(ctx->lvals1, ncell);
+    k_level_keys<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lk0, lv0);
+    uint64_t *lks;
+    uint32_t *lvs;
+    radix_sort_pairs(ctx, lk0, lk1, lv0, lv1, ncell, 0, 8, &lks, &lvs);
+    const int lnblk = (int)((ncell + kSortTile - 1) / kSortTile);
+    uint32_t *loffd = ensure<uint32_t>(ctx->lhist, kMaxLevel + 2);
+    k_level_offsets<<<1, 64, 0, st>>>(ptr<uint32_t>(ctx->hist), lnblk, loffd);
+    uint32_t loff[kMaxLevel + 2], lcnt[kMaxLevel + 1];
+    BH_HIP(hipMemcpyAsync(loff, loffd, sizeof loff, hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
-    uint32_t acc = 0;
-    for (int L = 0; L <= kMaxLevel; L++) {
-        loff[L] = acc;
-        acc += lcnt[L];
-    }
-    BH_HIP(hipMemcpyAsync(lhist + kMaxLevel + 1, loff, sizeof loff, hipMemcpyHostToDevice, st));
-    k_level_scatter<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lhist + kMaxLevel + 1, llist);
+    for (int L = 0; L <= kMaxLevel; L++) lcnt[L] = loff[L + 1] - loff[L];
+    const int *llist = (const int *)lvs;
     for (int L = kMaxLevel; L >= 0; L--) {
         if (!lcnt[L]) continue;
         k_moments_level<<<nblocks(lcnt[L]), kTpb, 0, st>>>(llist + loff[L], (int)lcnt[L], csub, cfirst, clast, clevel,
-                                                           bd, cm, ccom, cq, cA, cB, cQ0, cQ1, box[3], p.usequad);
+                                                           bd, mo, cA, cB, cQ0, cQ1, box[3], p.usequad);
     }
     BH_HIP(hipGetLastError());
     T.first = cfirst;
@@ -1002,9 +1079,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.skip = cskip;
     T.sub = csub;
     T.nkids = cnkids;
-    T.cm = cm;
-    T.ccom = ccom;
-    T.cq = cq;
+    T.mo = mo;
     T.cA = cA;
     T.cB = cB;
     T.cQ0 = cQ0;

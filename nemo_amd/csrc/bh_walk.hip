@@ -181,6 +181,7 @@ __device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cn
 struct WalkArgs {
     const float4 *thi, *tlo;
     const int *tself;
+    int selfmode;
     const uint32_t *tperm;
     int64_t ntest;
     int ngroups;
@@ -302,7 +303,9 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                         const float4 bh = a.bhi[node], bl = a.blo[node];
                         P[np + mbcnt(m)] = make_float4((bh.x - cx) + bl.x, (bh.y - cy) + bl.y, (bh.z - cz) + bl.z, bh.w);
                         const int64_t rel = (int64_t)node - gbase;
-                        if (rel >= 0 && rel < gcount && a.tself[gbase + rel] == node) sSelf[w][rel] = 1;
+                        if (rel >= 0 && rel < gcount &&
+                            (a.selfmode == 1 || (a.selfmode == 2 && a.tself[gbase + rel] == node)))
+                            sSelf[w][rel] = 1;
                     }
                     np += __popcll(m);
                 }
@@ -440,10 +443,11 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     const int npl = (p.group_size == 128) ? 2 : 1;
     const int G = 64 * npl;
     WalkArgs a{};
-    a.thi = ptr<float4>(ctx->thi);
-    a.tlo = ptr<float4>(ctx->tlo);
-    a.tself = ptr<int>(ctx->tself);
-    a.tperm = ptr<uint32_t>(ctx->tperm);
+    a.thi = T.thi;
+    a.tlo = T.tlo;
+    a.tself = T.tself;
+    a.selfmode = T.selfmode;
+    a.tperm = T.tperm;
     a.ntest = nt;
     a.ngroups = ctx->ngroups;
     a.gfirst = ptr<int>(ctx->gfirst);
