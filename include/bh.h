@@ -58,6 +58,8 @@ typedef struct {
     int group_size;      /* fast mode: particles per group (0 = tuned default, else 64 or 128) */
     int learned;         /* fast mode: apply the loaded MLP correction to accepted cells */
     int keep_box;        /* bh_eval: start from the box of the previous call (hackcode1 semantics) */
+    int nleaf;           /* fast mode: a cell holding <= nleaf bodies interacts body by body (exact);
+                            0 keeps the reference's cell interactions; -1 = tuned default */
 } bh_params;
 
 typedef struct {

@@ -88,7 +88,8 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     check_params(p);
     if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_eval before bh_upload");
     bh_params q = *p;
-    if (q.group_size == 0) q.group_size = 64;
+    if (q.group_size == 0) q.group_size = 128;
+    if (q.nleaf < 0) q.nleaf = 2;
     BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
     if (q.mode == BH_MODE_DIRECT) {
         throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
@@ -139,6 +140,7 @@ void bh_default_params(bh_params *p)
     p->group_size = 0;
     p->learned = 0;
     p->keep_box = 0;
+    p->nleaf = -1;
 }
 
 const char *bh_version(void) { return "nemo_amd-bh 0.1 (gfx950)"; }
@@ -183,7 +185,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
-                          &ctx->lvals0, &ctx->lvals1};
+                          &ctx->lvals0, &ctx->lvals1, &ctx->cfnb};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)

@@ -34,7 +34,7 @@ struct bh_ctx {
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre;
-    bh::DevBuf cmo, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
+    bh::DevBuf cmo, cfnb, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
     bh::DevBuf gfirst, gcount, gscan, goff;

@@ -37,6 +37,7 @@ struct Tree {
     const float4 *blo = nullptr;    // FP32 lo part (x - hi)
     // cells
     const int *first = nullptr, *last = nullptr;   // body range [first, last]
+    const int2 *fnb = nullptr;      // (first body, body count) per cell
     const int *sub = nullptr;       // [ncell][8] child refs: >= 0 body, < 0 ~cell, -1... see kNoChild
     const int *nkids = nullptr;     // entries pushed when a cell is opened
     const uint8_t *level = nullptr;

@@ -427,7 +427,7 @@ __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restric
 __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
                         const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
                         int nb, int ncell, int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
-                        unsigned long long *deep)
+                        int2 *fnb, unsigned long long *deep)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= ncell) return;
@@ -445,6 +445,7 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restri
     clast[id] = e;
     clevel[id] = (uint8_t)L;
     if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
+    if (fnb) fnb[id] = make_int2(s, e - s + 1);
     if (pre) pre[s + id] = ~id;
     if (deep && L == kMaxLevel) atomicAdd(deep, 1ULL);
 }
@@ -808,7 +809,7 @@ struct Topo {
 
 static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, DevBuf &bcnt, DevBuf &bbase,
                      DevBuf &bm1, DevBuf &bm2, DevBuf &bfirst, DevBuf &blast, DevBuf &blevel, DevBuf *bskip,
-                     DevBuf *bpre, unsigned long long *deep)
+                     DevBuf *bpre, DevBuf *bfnb, unsigned long long *deep)
 {
     hipStream_t st = ctx->stream;
     Topo t{};
@@ -833,9 +834,10 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, Dev
     t.level = ensure<uint8_t>(blevel, ncell);
     int *cskip = bskip ? ensure<int>(*bskip, ncell) : nullptr;
     int *pre = bpre ? ensure<int>(*bpre, (size_t)n + ncell) : nullptr;
+    int2 *fnb = bfnb ? ensure<int2>(*bfnb, ncell) : nullptr;
     if (ncell > 0)
         k_cells<<<nblocks(ncell), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, (int)ncell, t.first, t.last, t.level,
-                                                 cskip, pre, deep);
+                                                 cskip, pre, fnb, deep);
     if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, pre);
     BH_HIP(hipGetLastError());
     return t;
@@ -904,7 +906,7 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const uint64_t *ks, co
         return;
     }
     Topo t = topology(ctx, tks, (int)nt, ctx->gcpl, ctx->gccnt, ctx->gcbase, ctx->gm1, ctx->gm2, ctx->gcf, ctx->gcl,
-                      ctx->gclv, nullptr, nullptr, nullptr);
+                      ctx->gclv, nullptr, nullptr, nullptr, nullptr);
     make_groups(ctx, &t, tks, nt, G);
 }
 
@@ -1035,7 +1037,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         return;
     }
     Topo tt = topology(ctx, ks, nb, ctx->cpl, ctx->ccnt, ctx->cbase, ctx->cmin1, ctx->cmin2, ctx->cfirst, ctx->clast,
-                       ctx->clevel, &ctx->cskip, &ctx->pre, cnt + 2);
+                       ctx->clevel, &ctx->cskip, &ctx->pre, &ctx->cfnb, cnt + 2);
     const uint32_t ncell = (uint32_t)tt.ncell;
     T.ncell = tt.ncell;
     int8_t *cpl = tt.cpl;
@@ -1075,6 +1077,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     BH_HIP(hipGetLastError());
     T.first = cfirst;
     T.last = clast;
+    T.fnb = ptr<int2>(ctx->cfnb);
     T.level = clevel;
     T.skip = cskip;
     T.sub = csub;

@@ -188,7 +188,9 @@ struct WalkArgs {
     const int *gfirst, *gcount;
     const float4 *bhi, *blo;
     const float4 *cA, *cB, *cQ0, *cQ1;
-    const int *sub, *nkids, *first;
+    const int *sub, *nkids;
+    const int2 *fnb;
+    int nleaf;
     const uint8_t *level;
     int nb, ncell;
     float tolsq, eps2;
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 const bool isBody = v && node >= 0;
                 const bool isCell = v && node < 0;
                 bool open = false, accept = false;
-                int c = 0, nk = 0;
+                int c = 0, nk = 0, bfirst = -1;
                 float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
                 float rx = 0.f, ry = 0.f, rz = 0.f;
                 if (isCell) {
@@ -292,9 +294,12 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     float dy = fmaxf(fmaxf(bminy - ry, ry - bmaxy), 0.f);
                     float dz = fmaxf(fmaxf(bminz - rz, rz - bmaxz), 0.f);
                     float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    open = a.tolsq * d2 < B.w;
+                    const int2 fn = a.fnb[c];
+                    const bool small = fn.y <= a.nleaf || a.level[c] == kMaxLevel;
+                    open = a.tolsq * d2 < B.w || fn.y <= a.nleaf;
                     accept = !open;
-                    if (open) nk = a.nkids[c];
+                    if (open) nk = small ? fn.y : a.nkids[c];
+                    bfirst = small ? fn.x : -1;
                 }
                 // bodies -> P2P list
                 {
@@ -332,9 +337,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     }
                     if (open) {
                         int *dst = fr1 + nn + off;
-                        if (a.level[c] == kMaxLevel) {
-                            const int f = a.first[c];
-                            for (int j = 0; j < nk; j++) dst[j] = f + j;
+                        if (bfirst >= 0) {
+                            for (int j = 0; j < nk; j++) dst[j] = bfirst + j;
                         } else {
                             const int *sub = a.sub + 8 * (int64_t)c;
                             int j = 0;
@@ -461,7 +465,8 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.cQ1 = T.cQ1;
     a.sub = T.sub;
     a.nkids = T.nkids;
-    a.first = T.first;
+    a.fnb = T.fnb;
+    a.nleaf = p.nleaf;
     a.level = T.level;
     a.nb = T.nb;
     a.ncell = T.ncell;
