@@ -67,6 +67,10 @@ void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, i
     const bool self = (tpos == nullptr || ntest == 0);
     if (!self && (ntest < 0 || ntest > (int64_t)INT32_MAX / 2)) throw ApiFail(BH_ERR_ARG, "bad ntest");
     ctx->nmass = nmass;
+    int64_t nb = 0;
+    for (int64_t i = 0; i < nmass; i++) nb += mass[i] != 0.0;
+    if (nb != ctx->nb) ctx->cell_cap = 0;
+    ctx->nb = nb;
     ctx->self_test = self;
     ctx->ntest = self ? nmass : ntest;
     double *dm = bh::ensure<double>(ctx->mass, nmass);
@@ -83,6 +87,9 @@ void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, i
     ctx->uploaded = true;
 }
 
+// One evaluation without host synchronisation, then one read-back of the
+// device status.  A capacity overflow (cells, walk frontier) or a box that
+// needed more expansion rounds grows the capacity and evaluates again.
 void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
 {
     check_params(p);
@@ -90,33 +97,61 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     bh_params q = *p;
     if (q.group_size == 0) q.group_size = 128;
     if (q.nleaf < 0) q.nleaf = 2;
-    BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (q.mode == BH_MODE_DIRECT) {
-        throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
+    if (q.mode == BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
+    if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
+    unsigned long long stat[bh::S_NSLOTS];
+    double box[4];
+    for (int attempt = 0;; attempt++) {
+        BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+        bh::build_tree(ctx, q);
+        BH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+        if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
+        else bh::walk_fast(ctx, q);
+        BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+        BH_HIP(hipMemcpyAsync(stat, ctx->counters.p, sizeof stat, hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(box, ctx->dbox.p, sizeof box, hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->box_valid = true;
+        const unsigned long long f = stat[bh::S_FLAGS];
+        if (f & bh::F_NONFINITE)
+            throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
+        if (!(f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER))) break;
+        if (attempt >= 12) throw ApiFail(BH_ERR_CAPACITY, "evaluation did not converge (capacities)");
+        if (f & bh::F_CELL_OVF)
+            ctx->cell_cap = (int)std::min<uint64_t>(stat[bh::S_NCELL_RAW] + stat[bh::S_NCELL_RAW] / 4 + 64, INT32_MAX / 2);
+        if (f & bh::F_TCELL_OVF)
+            ctx->tcell_cap = (int)std::min<uint64_t>(stat[bh::S_TNCELL_RAW] + stat[bh::S_TNCELL_RAW] / 4 + 64, INT32_MAX / 2);
+        if (f & bh::F_BOX_OPEN) {
+            ctx->box_rounds *= 4;
+            ctx->box_rounds_keep *= 4;
+        }
+        if (f & bh::F_FRONTIER) {
+            ctx->fcap *= 4;
+            if ((size_t)ctx->fcap * sizeof(int) * 2 * 4 * 8 * ctx->ncu > ((size_t)64 << 30))
+                throw ApiFail(BH_ERR_CAPACITY, "walk frontier would exceed 64 GiB; use a larger tol");
+        }
     }
-    bh::build_tree(ctx, q);
-    BH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-    if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
-    else bh::walk_fast(ctx, q);
-    BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    BH_HIP(hipEventSynchronize(ctx->ev[2]));
+    for (int k = 0; k < 4; k++) ctx->box[k] = box[k];
+    ctx->tree.ncell = (int)stat[bh::S_NCELL];
+    ctx->ngroups = (int)stat[bh::S_NGROUPS];
     bh_stats s{};
-    unsigned long long c[4];
-    BH_HIP(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
-    s.n2btot = (int64_t)c[0];
-    s.nbctot = (int64_t)c[1];
-    s.deep_buckets = (int64_t)c[2];
-    s.n_mlp = (int64_t)c[3];
+    s.n2btot = (int64_t)stat[bh::S_N2B];
+    s.nbctot = (int64_t)stat[bh::S_NBC];
+    s.deep_buckets = (int64_t)stat[bh::S_DEEP];
+    s.n_mlp = (int64_t)stat[bh::S_NMLP];
     s.ncell = ctx->tree.ncell;
-    for (int k = 0; k < 3; k++) s.rmin[k] = ctx->box[k];
-    s.rsize = ctx->box[3];
-    float t1 = 0, t2 = 0;
+    for (int k = 0; k < 3; k++) s.rmin[k] = box[k];
+    s.rsize = box[3];
+    float t1 = 0, t2 = 0, tw = 0, ts = 0;
     BH_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
     BH_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
+    BH_HIP(hipEventElapsedTime(&ts, ctx->ev[6], ctx->ev[7]));
+    if (q.mode == BH_MODE_FAST) BH_HIP(hipEventElapsedTime(&tw, ctx->ev[4], ctx->ev[5]));
+    else tw = t2;
     s.t_tree_ms = t1;
     s.t_walk_ms = t2;
-    s.t_walk_kernel_ms = q.mode == BH_MODE_FAST ? ctx->t_walk_kernel_ms : t2;
-    s.t_sort_ms = ctx->t_sort_ms;
+    s.t_walk_kernel_ms = tw;
+    s.t_sort_ms = ts;
     s.ngroups = q.mode == BH_MODE_FAST ? ctx->ngroups : 0;
     ctx->last = s;
     ctx->last_mode = q.mode;
@@ -285,7 +320,7 @@ bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, doubl
 {
     return guarded(ctx, [&] {
         if (!ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
-        const int64_t nc = ctx->tree.ncell;
+        const int64_t nc = std::min<int64_t>(ctx->tree.ncell, ctx->tree.cap);
         if (ncell) *ncell = nc;
         int64_t k = std::min<int64_t>(cap, nc);
         if (k <= 0) return;

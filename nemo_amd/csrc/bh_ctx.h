@@ -39,7 +39,7 @@ struct bh_ctx {
     // groups of the test set (fast walk)
     bh::DevBuf gfirst, gcount, gscan, goff;
     bh::DevBuf gcpl, gccnt, gcbase, gm1, gm2, gcf, gcl, gclv;
-    int ngroups = 0;
+    int ngroups = 0;          // host copy after an evaluation
     // walk scratch
     bh::DevBuf frontier, gctr, werr;
     int64_t fcap = 0;
@@ -47,6 +47,12 @@ struct bh_ctx {
     bh::DevBuf phi, acc;
 
     bh::Tree tree;
+    int64_t nb = 0;             // massive bodies (counted at upload)
+    int cell_cap = 0;           // cell capacity of the tree arrays
+    int tcell_cap = 0;          // cell capacity of a separate test-set topology
+    int box_rounds = 6;         // expansion rounds from a fresh box
+    int box_rounds_keep = 2;    // expansion rounds from the previous box
+    bool capturing = false;     // a graph is being captured: no events, no host syncs
     int last_mode = 0;
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float t_walk_kernel_ms = 0.f, t_sort_ms = 0.f;

@@ -61,10 +61,12 @@ __global__ __launch_bounds__(kTpb) void k_walk_strict(
     int64_t nt,
     const int *__restrict__ pre, const int *__restrict__ skip, const uint8_t *__restrict__ level,
     const double *__restrict__ mo,
-    const double4 *__restrict__ bd, int nnode, double rsize, double tol, double eps, int usequad, double *phi_out,
-    double *acc_out, unsigned long long *counters)
+    const double4 *__restrict__ bd, int nb, const double4 *__restrict__ box, double tol, double eps, int usequad,
+    double *phi_out, double *acc_out, unsigned long long *counters)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int nnode = nb + (int)counters[S_NCELL];
+    const double rsize = box->w;
     unsigned long long n2b = 0, nbc = 0;
     if (i < nt) {
         const double4 x0 = td[i];
@@ -121,6 +123,13 @@ __global__ __launch_bounds__(kTpb) void k_walk_strict(
     }
 }
 
+__global__ void k_zero_out(double *phi, int64_t n, double *acc)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) phi[i] = 0.0;
+    if (i < 3 * n) acc[i] = 0.0;
+}
+
 // Direct sum in input order over all n mass-set bodies (self skipped by
 // index), for the targets idx[0..nt) of the mass set.
 constexpr int kDirTile = 256;
@@ -166,14 +175,12 @@ void walk_strict(bh_ctx *ctx, const bh_params &p)
     const int64_t nt = ctx->ntest;
     double *phi = ptr<double>(ctx->phi), *acc = ptr<double>(ctx->acc);
     unsigned long long *cnt = ptr<unsigned long long>(ctx->counters);
-    const int nnode = T.nb + T.ncell;
     if (T.nb == 0) {
-        BH_HIP(hipMemsetAsync(phi, 0, nt * sizeof(double), ctx->stream));
-        BH_HIP(hipMemsetAsync(acc, 0, 3 * nt * sizeof(double), ctx->stream));
+        k_zero_out<<<std::max<int64_t>(1, (3 * nt + kTpb - 1) / kTpb), kTpb, 0, ctx->stream>>>(phi, nt, acc);
         return;
     }
     int grid = (int)std::max<int64_t>(1, (nt + kTpb - 1) / kTpb);
-    k_walk_strict<<<grid, kTpb, 0, ctx->stream>>>(T.td, T.tself, T.selfmode, T.tperm, nt, T.pre, T.skip, T.level, T.mo, T.bd, nnode, T.box[3],
+    k_walk_strict<<<grid, kTpb, 0, ctx->stream>>>(T.td, T.tself, T.selfmode, T.tperm, nt, T.pre, T.skip, T.level, T.mo, T.bd, T.nb, T.dbox,
                                                   p.tol, p.eps, p.usequad, phi, acc, cnt);
     BH_HIP(hipGetLastError());
 }

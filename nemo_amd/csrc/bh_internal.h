@@ -54,9 +54,29 @@ struct Tree {
     const uint32_t *tperm = nullptr;   // sorted test index -> input index
     const int *tself = nullptr;        // selfmode 2: tree body of each test point or -1
     int selfmode = 0;                  // 0 no self pairs, 1 test index == tree body, 2 tself[]
+    const uint64_t *keys = nullptr;    // sorted Morton keys of the mass set
+    int cap = 0;                       // cell capacity of the arrays
+    const unsigned long long *ncellp = nullptr;   // device cell count
+    const double4 *dbox = nullptr;     // device root box (rmin.xyz, rsize)
 };
 
 constexpr int kNoChild = INT32_MIN;  // empty octant in Tree::sub
+
+// Device status words (unsigned long long) written by the kernels and read
+// back once at the end of an evaluation, so the evaluation itself has no
+// host synchronisation (and can be captured in a graph).
+enum StatSlot {
+    S_N2B = 0, S_NBC = 1, S_DEEP = 2, S_NMLP = 3,
+    S_NCELL = 4,        // cells of the tree (clamped to the capacity)
+    S_NCELL_RAW = 5,    // cells needed
+    S_NGROUPS = 6,
+    S_FLAGS = 7,
+    S_TNCELL = 8, S_TNCELL_RAW = 9,   // topology of a separate test set (grouping only)
+    S_NSLOTS = 16
+};
+enum StatFlag : unsigned long long {
+    F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16
+};
 
 }  // namespace bh
 

@@ -85,7 +85,8 @@ __device__ __forceinline__ bool outside_massive(const double *__restrict__ pos, 
 }
 
 __global__ void k_box_count(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                            const double4 *__restrict__ box, int64_t chunk, uint32_t *bcount, int *boxi)
+                            const double4 *__restrict__ box, int64_t chunk, uint32_t *bcount,
+                            unsigned long long *stat)
 {
     const double4 b = *box;
     const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
@@ -94,7 +95,7 @@ __global__ void k_box_count(const double *__restrict__ pos, const double *__rest
     uint32_t tot;
     block_excl_scan((uint32_t)cnt, &tot);
     if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
-    if (bad) atomicOr(&boxi[3], 1);
+    if (bad) atomicOr(&stat[S_FLAGS], (unsigned long long)F_NONFINITE);
 }
 
 __global__ void k_box_write(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
@@ -131,13 +132,28 @@ __global__ void k_box_first_c(const double *__restrict__ pos, const int *__restr
     if (lane_id() == 0 && best != INT32_MAX) atomicMin(&boxi[0], best);
 }
 
+// Starting box: the caller's (rmin, rsize), or the previous box when keep.
+// boxi: [0] first outside candidate (min), [1] start index, [2] expansions.
+__global__ void k_box_init(double4 *box, int *boxi, double rx, double ry, double rz, double rsize, int keep)
+{
+    if (!keep) *box = make_double4(rx, ry, rz, rsize);
+    boxi[0] = INT32_MAX;
+    boxi[1] = 0;
+    boxi[2] = 0;
+}
+
+// After the expansion rounds: flag a candidate that still lies outside.
+__global__ void k_box_check(const int *boxi, unsigned long long *stat)
+{
+    if (boxi[0] != INT32_MAX) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
+}
+
 // Expand the box for the body found by k_box_first (single thread).
 __global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *boxi)
 {
     int first = boxi[0];
-    boxi[2] = 0;
-    if (first == INT32_MAX || boxi[3]) return;
-    boxi[5] += 1;
+    if (first == INT32_MAX) return;
+    boxi[2] += 1;
     const double *x = pos + 3 * (int64_t)first;
     double4 b = *box;
     for (int guard = 0; guard < 2000 && !fits(x, b); guard++) {
@@ -150,7 +166,6 @@ __global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *
     *box = b;
     boxi[0] = INT32_MAX;
     boxi[1] = first + 1;
-    boxi[2] = 1;
 }
 
 // ---- keys ------------------------------------------------------------------
@@ -193,18 +208,6 @@ __global__ void k_keys(const double *__restrict__ pos, const double *__restrict_
         keys[i] = k;
         vals[i] = (uint32_t)i;
     }
-}
-
-// Number of massive bodies = first sorted key with bit 63 set.
-__global__ void k_count_massive(const uint64_t *__restrict__ keys, int64_t n, int *nb)
-{
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) / 2;
-        if (keys[mid] >> 63) hi = mid;
-        else lo = mid + 1;
-    }
-    *nb = (int)lo;
 }
 
 // ---- exclusive scan (uint32) -------------------------------------------------
@@ -426,11 +429,11 @@ __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restric
 // s with cbase[s] <= id.
 __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
                         const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
-                        int nb, int ncell, int *cfirst, int *clast, uint8_t *clevel, int *cskip, int *pre,
-                        int2 *fnb, unsigned long long *deep)
+                        int nb, const unsigned long long *ncellp, int *cfirst, int *clast, uint8_t *clevel,
+                        int *cskip, int *pre, int2 *fnb, unsigned long long *deep)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= ncell) return;
+    if (id >= (int)*ncellp) return;
     int lo = 0, hi = nb;           // upper_bound over cbase[0..nb)
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
@@ -451,11 +454,43 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restri
 }
 
 // Pre-order position of body s: after the cells that start at s.
-__global__ void k_body_pre(const uint32_t *__restrict__ cbase, int nb, int *pre)
+__global__ void k_body_pre(const uint32_t *__restrict__ cbase, int nb, const unsigned long long *ncellp, int *pre)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb) return;
+    if (s >= nb || cbase[s + 1] > (uint32_t)*ncellp) return;
     pre[s + cbase[s + 1]] = s;
+}
+
+// Cells needed = cbase[n]; the count used is clamped to the capacity and an
+// overflow is flagged (the host then grows the arrays and evaluates again).
+__global__ void k_clamp_cells(const uint32_t *__restrict__ cbase, int n, unsigned long long cap,
+                              unsigned long long *stat, int slot, int slot_raw, unsigned long long flag)
+{
+    unsigned long long raw = n >= 2 ? cbase[n] : 0;
+    stat[slot_raw] = raw;
+    stat[slot] = raw < cap ? raw : cap;
+    if (raw > cap) atomicOr(&stat[S_FLAGS], flag);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
+__global__ void k_copy_count(const uint32_t *src, unsigned long long *dst) { *dst = *src; }
+
+// Fixed-size groups of consecutive points (sets too small to need a topology).
+__global__ void k_groups_trivial(int64_t n, int G, int *gfirst, int *gcount, unsigned long long *stat)
+{
+    int64_t ng = (n + G - 1) / G;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng; g += (int64_t)gridDim.x * blockDim.x) {
+        gfirst[g] = (int)(g * G);
+        gcount[g] = (int)min<int64_t>(G, n - g * G);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) stat[S_NGROUPS] = (unsigned long long)ng;
 }
 
 // ---- groups of test points -------------------------------------------------
@@ -550,32 +585,38 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
 __global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
-                              const uint8_t *__restrict__ clevel, int ncell, int G, uint32_t *gcnt)
+                              const uint8_t *__restrict__ clevel, int cap, const unsigned long long *ncellp, int G,
+                              uint32_t *gcnt)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= cap) return;
+    if (c >= (int)*ncellp) {
+        gcnt[c] = 0;
+        return;
+    }
     gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, nullptr, nullptr, 0);
 }
 
 __global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
-                              const uint8_t *__restrict__ clevel, int ncell, int G, const uint32_t *__restrict__ goff,
-                              int *gfirst, int *gcount)
+                              const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G,
+                              const uint32_t *__restrict__ goff, int *gfirst, int *gcount)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= (int)*ncellp) return;
     pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, gfirst, gcount, goff[c]);
 }
 
 // Children of each cell in octant order; parent links; child-cell counts.
 __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                            const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
-                           const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell,
-                           int *csub, int *cnkids, int *cnkc, int *cparent, int *cctr)
+                           const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
+                           const unsigned long long *ncellp, int *csub, int *cnkids, int *cnkc, int *cparent,
+                           int *cctr)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (c >= (int)*ncellp) return;
     int s = cfirst[c], e = clast[c], L = clevel[c];
     int *sub = csub + 8 * (int64_t)c;
 #pragma unroll
@@ -618,6 +659,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
                              const double4 *__restrict__ bd, double *mo,
                              float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad)
 {
+    // rsize: side of the root box (the cell at level L has side rsize / 2^L)
     const int L = clevel[c];
     double M = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     double q[6] = {0, 0, 0, 0, 0, 0};
@@ -734,11 +776,12 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
 // Cells bucketed by level (a one-digit radix sort of (level, cell id)) so the
 // moments can be computed level by level, deepest first, with a kernel
 // boundary between levels.
-__global__ void k_level_keys(const uint8_t *__restrict__ clevel, int ncell, uint64_t *keys, uint32_t *vals)
+__global__ void k_level_keys(const uint8_t *__restrict__ clevel, int cap, const unsigned long long *ncellp,
+                             uint64_t *keys, uint32_t *vals)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    keys[c] = clevel[c];
+    if (c >= cap) return;
+    keys[c] = c < (int)*ncellp ? clevel[c] : 255;
     vals[c] = (uint32_t)c;
 }
 
@@ -750,15 +793,17 @@ __global__ void k_level_offsets(const uint32_t *__restrict__ hist_scan, int nblk
     if (d <= kMaxLevel + 1) out[d] = hist_scan[(int64_t)d * nblk];
 }
 
-__global__ void k_moments_level(const int *__restrict__ list, int cnt, const int *__restrict__ csub,
+__global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__restrict__ loff, int L,
+                                const int *__restrict__ csub,
                                 const int *__restrict__ cfirst, const int *__restrict__ clast,
                                 const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *mo,
                                 float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
-                                double rsize, int usequad)
+                                const double4 *__restrict__ box, int usequad)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= cnt) return;
-    cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad);
+    const double rsize = box->w;
+    const uint32_t lo = loff[L], hi = loff[L + 1];
+    for (uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x; k < hi; k += gridDim.x * blockDim.x)
+        cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -798,88 +843,78 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
 }
 
 // Cell ranges of the sorted key array keys[0..n): cpl, cbase (n+1 entries),
-// block minima, and first/last/level of every cell.  n >= 2.
+// block minima, and first/last/level of every cell, launched over the cell
+// capacity with the cell count kept on the device (stat[slot]).  n >= 2.
 struct Topo {
     int8_t *cpl;
     uint32_t *cbase;
     int *first, *last;
     uint8_t *level;
-    int ncell;
+    int cap;
+    const unsigned long long *ncellp;
 };
 
-static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, DevBuf &bcpl, DevBuf &bcnt, DevBuf &bbase,
-                     DevBuf &bm1, DevBuf &bm2, DevBuf &bfirst, DevBuf &blast, DevBuf &blevel, DevBuf *bskip,
-                     DevBuf *bpre, DevBuf *bfnb, unsigned long long *deep)
+static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &bcpl, DevBuf &bcnt, DevBuf &bbase,
+                     DevBuf &bm1, DevBuf &bfirst, DevBuf &blast, DevBuf &blevel, DevBuf *bskip, DevBuf *bpre,
+                     DevBuf *bfnb, int slot, int slot_raw, unsigned long long flag)
 {
     hipStream_t st = ctx->stream;
+    unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
     Topo t{};
+    t.cap = cap;
+    t.ncellp = stat + slot;
     t.cpl = ensure<int8_t>(bcpl, n);
     uint32_t *ccnt = ensure<uint32_t>(bcnt, n + 1);
     t.cbase = ensure<uint32_t>(bbase, n + 1);
     k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, ccnt);
     exclusive_scan_u32(ctx, ccnt, t.cbase, n);
+    k_clamp_cells<<<1, 1, 0, st>>>(t.cbase, n, (unsigned long long)cap, stat, slot, slot_raw, flag);
     int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64, n3 = (n2 + 63) / 64;
     int8_t *m1 = ensure<int8_t>(bm1, n1 + n2 + n3 + 3);
     int8_t *m2 = m1 + n1 + 1, *m3 = m2 + n2 + 1;
-    (void)bm2;
     k_cmin<<<nblocks(n1), kTpb, 0, st>>>(t.cpl, n, m1, n1);
     k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
     k_cmin<<<nblocks(n3), kTpb, 0, st>>>(m2, n2, m3, n3);
-    uint32_t ncell = 0;
-    BH_HIP(hipMemcpyAsync(&ncell, t.cbase + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BH_HIP(hipStreamSynchronize(st));
-    t.ncell = (int)ncell;
-    t.first = ensure<int>(bfirst, ncell);
-    t.last = ensure<int>(blast, ncell);
-    t.level = ensure<uint8_t>(blevel, ncell);
-    int *cskip = bskip ? ensure<int>(*bskip, ncell) : nullptr;
-    int *pre = bpre ? ensure<int>(*bpre, (size_t)n + ncell) : nullptr;
-    int2 *fnb = bfnb ? ensure<int2>(*bfnb, ncell) : nullptr;
-    if (ncell > 0)
-        k_cells<<<nblocks(ncell), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, (int)ncell, t.first, t.last, t.level,
-                                                 cskip, pre, fnb, deep);
-    if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, pre);
+    t.first = ensure<int>(bfirst, cap);
+    t.last = ensure<int>(blast, cap);
+    t.level = ensure<uint8_t>(blevel, cap);
+    int *cskip = bskip ? ensure<int>(*bskip, cap) : nullptr;
+    int *pre = bpre ? ensure<int>(*bpre, (size_t)n + cap) : nullptr;
+    int2 *fnb = bfnb ? ensure<int2>(*bfnb, cap) : nullptr;
+    k_cells<<<nblocks(cap), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, t.ncellp, t.first, t.last, t.level, cskip, pre,
+                                           fnb, bskip ? stat + S_DEEP : nullptr);
+    if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, pre);
     BH_HIP(hipGetLastError());
     return t;
 }
 
-// Groups of the sorted test set for the fast walk (ctx->gfirst/gcount).
+// Groups of the sorted test set for the fast walk (ctx->gfirst/gcount, count
+// in stat[S_NGROUPS]).
 static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G)
 {
     hipStream_t st = ctx->stream;
-    if (n <= G || t == nullptr || t->ncell == 0) {
+    unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
+    int *gf = ensure<int>(ctx->gfirst, n), *gc = ensure<int>(ctx->gcount, n);
+    if (t == nullptr) {
         int64_t ng = (n + G - 1) / G;
-        std::vector<int> f(ng), c(ng);
-        for (int64_t g = 0; g < ng; g++) {
-            f[g] = (int)(g * G);
-            c[g] = (int)std::min<int64_t>(G, n - g * G);
-        }
-        int *gf = ensure<int>(ctx->gfirst, ng), *gc = ensure<int>(ctx->gcount, ng);
-        BH_HIP(hipMemcpyAsync(gf, f.data(), ng * sizeof(int), hipMemcpyHostToDevice, st));
-        BH_HIP(hipMemcpyAsync(gc, c.data(), ng * sizeof(int), hipMemcpyHostToDevice, st));
-        BH_HIP(hipStreamSynchronize(st));
-        ctx->ngroups = (int)ng;
+        k_groups_trivial<<<nblocks(ng), kTpb, 0, st>>>(n, G, gf, gc, stat);
         return;
     }
-    uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->ncell + 1);
-    uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->ncell + 1);
-    k_group_count<<<nblocks(t->ncell), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncell, G, gcnt);
-    exclusive_scan_u32(ctx, gcnt, goff, t->ncell);
-    uint32_t ng = 0;
-    BH_HIP(hipMemcpyAsync(&ng, goff + t->ncell, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    BH_HIP(hipStreamSynchronize(st));
-    int *gf = ensure<int>(ctx->gfirst, ng), *gc = ensure<int>(ctx->gcount, ng);
-    k_group_write<<<nblocks(t->ncell), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncell, G, goff,
-                                                      gf, gc);
+    uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->cap + 1);
+    uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->cap + 1);
+    k_group_count<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->cap,
+                                                    t->ncellp, G, gcnt);
+    exclusive_scan_u32(ctx, gcnt, goff, t->cap);
+    k_group_write<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncellp, G,
+                                                    goff, gf, gc);
+    k_copy_count<<<1, 1, 0, st>>>(goff + t->cap, stat + S_NGROUPS);
     BH_HIP(hipGetLastError());
-    ctx->ngroups = (int)ng;
 }
 
 // Morton keys of the mass set in input order, in the current box.
 void debug_keys(bh_ctx *ctx, uint64_t *d_out)
 {
     double4 *d_box = ensure<double4>(ctx->dbox, 1);
-    BH_HIP(hipMemcpyAsync(d_box, ctx->box, sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
     uint32_t *tmp;
     BH_HIP(hipMalloc(&tmp, ctx->nmass * sizeof(uint32_t)));
     k_keys<<<nblocks(ctx->nmass), kTpb, 0, ctx->stream>>>(ptr<double>(ctx->pos), ptr<double>(ctx->mass), ctx->nmass,
@@ -890,90 +925,78 @@ void debug_keys(bh_ctx *ctx, uint64_t *d_out)
 
 // Groups of the sorted test set.  When the test set is exactly the tree's
 // body set the tree topology is reused; otherwise the test keys get their own.
-static void build_groups(bh_ctx *ctx, const bh_params &p, const uint64_t *ks, const uint64_t *tks)
+static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo, const uint64_t *ks,
+                         const uint64_t *tks)
 {
     const int G = p.group_size > 0 ? p.group_size : 64;
     const int64_t nt = ctx->ntest;
     const Tree &T = ctx->tree;
-    if (ctx->self_test && nt == T.nb && T.ncell > 0) {
-        Topo t{ptr<int8_t>(ctx->cpl), ptr<uint32_t>(ctx->cbase), ptr<int>(ctx->cfirst), ptr<int>(ctx->clast),
-               ptr<uint8_t>(ctx->clevel), T.ncell};
-        make_groups(ctx, &t, ks, nt, G);
+    if (ctx->self_test && nt == T.nb && tree_topo) {
+        make_groups(ctx, tree_topo, ks, nt, G);
         return;
     }
-    if (nt <= G) {
+    if (nt <= G || nt < 2) {
         make_groups(ctx, nullptr, nullptr, nt, G);
         return;
     }
-    Topo t = topology(ctx, tks, (int)nt, ctx->gcpl, ctx->gccnt, ctx->gcbase, ctx->gm1, ctx->gm2, ctx->gcf, ctx->gcl,
-                      ctx->gclv, nullptr, nullptr, nullptr, nullptr);
+    if (ctx->tcell_cap == 0) ctx->tcell_cap = (int)std::min<int64_t>(2 * nt + 64, INT32_MAX / 2);
+    Topo t = topology(ctx, tks, (int)nt, ctx->tcell_cap, ctx->gcpl, ctx->gccnt, ctx->gcbase, ctx->gm1, ctx->gcf,
+                      ctx->gcl, ctx->gclv, nullptr, nullptr, nullptr, S_TNCELL, S_TNCELL_RAW, F_TCELL_OVF);
     make_groups(ctx, &t, tks, nt, G);
 }
 
-// Builds the tree of the uploaded mass set and the sorted test set.
+// Builds the tree of the uploaded mass set and the sorted test set.  No host
+// synchronisation: every count stays on the device and capacities come from
+// the host (the caller checks stat[S_FLAGS] afterwards).
 void build_tree(bh_ctx *ctx, const bh_params &p)
 {
     hipStream_t st = ctx->stream;
     const int64_t n = ctx->nmass;
+    const int nb = (int)ctx->nb;
     double *d_pos = ptr<double>(ctx->pos), *d_mass = ptr<double>(ctx->mass);
     double4 *d_box = ensure<double4>(ctx->dbox, 1);
     int *d_boxi = ensure<int>(ctx->dboxi, 8);
+    unsigned long long *stat = ensure<unsigned long long>(ctx->counters, S_NSLOTS);
+    const bool timing = !ctx->capturing;
+    k_zero_u64<<<1, 64, 0, st>>>(stat, S_NSLOTS);
 
-    // -- box: start from the caller's box (or the previous one) and expand in input order
-    double box[4];
-    if (p.keep_box && ctx->box_valid) {
-        for (int k = 0; k < 4; k++) box[k] = ctx->box[k];
-    } else {
-        for (int k = 0; k < 3; k++) box[k] = p.rmin_given ? p.rmin[k] : -p.rsize / 2.0;
-        box[3] = p.rsize;
-    }
-    // boxi: first, start, changed (last round), non-finite, ncand, expansions
-    int boxi[6] = {INT32_MAX, 0, 0, 0, 0, 0};
-    BH_HIP(hipMemcpyAsync(d_box, box, sizeof(double4), hipMemcpyHostToDevice, st));
-    BH_HIP(hipMemcpyAsync(d_boxi, boxi, sizeof(boxi), hipMemcpyHostToDevice, st));
+    // -- box: start from the caller's box (or the previous one) and expand in
+    //    input order; a fixed number of rounds, checked afterwards
+    const bool keep = p.keep_box && ctx->box_valid;
+    const double r0 = p.rmin_given ? p.rmin[0] : -p.rsize / 2.0;
+    const double r1 = p.rmin_given ? p.rmin[1] : -p.rsize / 2.0;
+    const double r2 = p.rmin_given ? p.rmin[2] : -p.rsize / 2.0;
+    k_box_init<<<1, 1, 0, st>>>(d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
     const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
     int *cand = ensure<int>(ctx->bcand, n);
     const int64_t chunk = std::max<int64_t>(kTpb, (n + box_grid - 1) / box_grid);
     const int nchunk = (int)((n + chunk - 1) / chunk);
     uint32_t *bcount = ensure<uint32_t>(ctx->bcount, 2 * (size_t)nchunk + 2);
     uint32_t *boff = bcount + nchunk + 1;
-    k_box_count<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, bcount, d_boxi);
+    k_box_count<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, bcount, stat);
     exclusive_scan_u32(ctx, bcount, boff, nchunk);
     k_box_write<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, boff, cand);
-    for (int batch = 0; batch < 512; batch++) {
-        for (int r = 0; r < 6; r++) {
-            k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
-            k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
-        }
-        BH_HIP(hipMemcpyAsync(boxi, d_boxi, sizeof(boxi), hipMemcpyDeviceToHost, st));
-        BH_HIP(hipStreamSynchronize(st));
-        if (boxi[3]) throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
-        if (!boxi[2]) break;
+    const int rounds = std::max(1, keep ? ctx->box_rounds_keep : ctx->box_rounds);
+    for (int r = 0; r < rounds; r++) {
+        k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
+        k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
     }
-    BH_HIP(hipMemcpyAsync(box, d_box, sizeof(double4), hipMemcpyDeviceToHost, st));
+    k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
+    k_box_check<<<1, 1, 0, st>>>(d_boxi, stat);
 
-    // -- keys and sort of the mass set
+    // -- keys and sort of the mass set (massless bodies sort last)
     uint64_t *k0 = ensure<uint64_t>(ctx->keys0, n), *k1 = ensure<uint64_t>(ctx->keys1, n);
     uint32_t *v0 = ensure<uint32_t>(ctx->vals0, n), *v1 = ensure<uint32_t>(ctx->vals1, n);
-    int *d_nm = ensure<int>(ctx->nmassive, 1);
     const int grid_n = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 16);
     k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0);
     uint64_t *ks;
     uint32_t *vs;
-    BH_HIP(hipEventRecord(ctx->ev[6], st));
+    if (timing) BH_HIP(hipEventRecord(ctx->ev[6], st));
     radix_sort_pairs(ctx, k0, k1, v0, v1, n, 0, 64, &ks, &vs);
-    BH_HIP(hipEventRecord(ctx->ev[7], st));
-    k_count_massive<<<1, 1, 0, st>>>(ks, n, d_nm);
-    int nb = 0;
-    BH_HIP(hipMemcpyAsync(&nb, d_nm, sizeof(int), hipMemcpyDeviceToHost, st));
-    BH_HIP(hipStreamSynchronize(st));
-    BH_HIP(hipEventElapsedTime(&ctx->t_sort_ms, ctx->ev[6], ctx->ev[7]));
-    for (int k = 0; k < 4; k++) ctx->box[k] = box[k];
-    ctx->box_valid = true;
+    if (timing) BH_HIP(hipEventRecord(ctx->ev[7], st));
 
     Tree &T = ctx->tree;
     T.nb = nb;
-    for (int k = 0; k < 4; k++) T.box[k] = box[k];
     double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
     float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
@@ -981,6 +1004,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.bd = bd;
     T.bhi = bhi;
     T.blo = blo;
+    T.keys = ks;
+    T.dbox = d_box;
 
     // -- test set, sorted by its own keys in the same box
     const int64_t nt = ctx->ntest;
@@ -996,7 +1021,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.tperm = tperm;
     T.tself = tself;
     T.selfmode = ctx->self_test ? 2 : 0;
-    if (ctx->self_test && nt == nb) {
+    if (alias) {
         // the test set is the tree's body set in the same order
         T.thi = bhi;
         T.tlo = blo;
@@ -1021,65 +1046,54 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_tpos, thi, tlo, td, tself, nb, 0);
     }
 
-    // -- topology
-    T.ncell = 0;
-    unsigned long long *cnt = ensure<unsigned long long>(ctx->counters, 16);
-    BH_HIP(hipMemsetAsync(cnt, 0, 16 * sizeof(unsigned long long), st));
-    int *pre = ensure<int>(ctx->pre, std::max(nb, 1) * 2 + 64);
+    // -- topology over the cell capacity
+    int *pre = ensure<int>(ctx->pre, std::max(nb, 1) + (size_t)std::max(ctx->cell_cap, 1));
     T.pre = pre;
+    T.cap = 0;
+    T.ncellp = stat + S_NCELL;
     if (nb <= 1) {
-        if (nb == 1) {
-            int zero = 0;
-            BH_HIP(hipMemcpyAsync(pre, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-        }
-        BH_HIP(hipStreamSynchronize(st));
-        if (p.mode == BH_MODE_FAST) build_groups(ctx, p, ks, tks_for_groups);
+        if (nb == 1) k_set_int<<<1, 1, 0, st>>>(pre, 0);
+        if (p.mode == BH_MODE_FAST) build_groups(ctx, p, nullptr, ks, tks_for_groups);
         return;
     }
-    Topo tt = topology(ctx, ks, nb, ctx->cpl, ctx->ccnt, ctx->cbase, ctx->cmin1, ctx->cmin2, ctx->cfirst, ctx->clast,
-                       ctx->clevel, &ctx->cskip, &ctx->pre, &ctx->cfnb, cnt + 2);
-    const uint32_t ncell = (uint32_t)tt.ncell;
-    T.ncell = tt.ncell;
+    const int cap = ctx->cell_cap;
+    Topo tt = topology(ctx, ks, nb, cap, ctx->cpl, ctx->ccnt, ctx->cbase, ctx->cmin1, ctx->cfirst, ctx->clast,
+                       ctx->clevel, &ctx->cskip, &ctx->pre, &ctx->cfnb, S_NCELL, S_NCELL_RAW, F_CELL_OVF);
+    T.cap = cap;
     int8_t *cpl = tt.cpl;
     uint32_t *cbase = tt.cbase;
     int *cfirst = tt.first, *clast = tt.last;
     uint8_t *clevel = tt.level;
-    int *cskip = ptr<int>(ctx->cskip);
-    pre = ptr<int>(ctx->pre);
-    T.pre = pre;
-    int *csub = ensure<int>(ctx->csub, 8 * (size_t)ncell);
-    int *cnkids = ensure<int>(ctx->cnkids, ncell), *cnkc = ensure<int>(ctx->cnkc, ncell);
-    int *cparent = ensure<int>(ctx->cparent, ncell), *cctr = ensure<int>(ctx->cctr, ncell);
-    k_children<<<nblocks(ncell), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, (int)ncell, csub, cnkids, cnkc,
-                                                cparent, cctr);
-    double *mo = ensure<double>(ctx->cmo, 10 * (size_t)ncell);
-    float4 *cA = ensure<float4>(ctx->cA, ncell), *cB = ensure<float4>(ctx->cB, ncell);
-    float4 *cQ0 = ensure<float4>(ctx->cQ0, ncell), *cQ1 = ensure<float4>(ctx->cQ1, ncell);
-    uint64_t *lk0 = ensure<uint64_t>(ctx->lkeys0, ncell), *lk1 = ensure<uint64_t>(ctx->lkeys1, ncell);
-    uint32_t *lv0 = ensure<uint32_t>(ctx->lvals0, ncell), *lv1 = ensure<uint32_t>(ctx->lvals1, ncell);
-    k_level_keys<<<nblocks(ncell), kTpb, 0, st>>>(clevel, (int)ncell, lk0, lv0);
+    T.pre = ptr<int>(ctx->pre);
+    int *csub = ensure<int>(ctx->csub, 8 * (size_t)cap);
+    int *cnkids = ensure<int>(ctx->cnkids, cap), *cnkc = ensure<int>(ctx->cnkc, cap);
+    int *cparent = ensure<int>(ctx->cparent, cap), *cctr = ensure<int>(ctx->cctr, cap);
+    k_children<<<nblocks(cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids, cnkc,
+                                              cparent, cctr);
+    double *mo = ensure<double>(ctx->cmo, 10 * (size_t)cap);
+    float4 *cA = ensure<float4>(ctx->cA, cap), *cB = ensure<float4>(ctx->cB, cap);
+    float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
+    // cells bucketed by level, then the moments level by level, deepest first
+    uint64_t *lk0 = ensure<uint64_t>(ctx->lkeys0, cap), *lk1 = ensure<uint64_t>(ctx->lkeys1, cap);
+    uint32_t *lv0 = ensure<uint32_t>(ctx->lvals0, cap), *lv1 = ensure<uint32_t>(ctx->lvals1, cap);
+    k_level_keys<<<nblocks(cap), kTpb, 0, st>>>(clevel, cap, tt.ncellp, lk0, lv0);
     uint64_t *lks;
     uint32_t *lvs;
-    radix_sort_pairs(ctx, lk0, lk1, lv0, lv1, ncell, 0, 8, &lks, &lvs);
-    const int lnblk = (int)((ncell + kSortTile - 1) / kSortTile);
+    radix_sort_pairs(ctx, lk0, lk1, lv0, lv1, cap, 0, 8, &lks, &lvs);
+    const int lnblk = (int)((cap + kSortTile - 1) / kSortTile);
     uint32_t *loffd = ensure<uint32_t>(ctx->lhist, kMaxLevel + 2);
     k_level_offsets<<<1, 64, 0, st>>>(ptr<uint32_t>(ctx->hist), lnblk, loffd);
-    uint32_t loff[kMaxLevel + 2], lcnt[kMaxLevel + 1];
-    BH_HIP(hipMemcpyAsync(loff, loffd, sizeof loff, hipMemcpyDeviceToHost, st));
-    BH_HIP(hipStreamSynchronize(st));
-    for (int L = 0; L <= kMaxLevel; L++) lcnt[L] = loff[L + 1] - loff[L];
     const int *llist = (const int *)lvs;
-    for (int L = kMaxLevel; L >= 0; L--) {
-        if (!lcnt[L]) continue;
-        k_moments_level<<<nblocks(lcnt[L]), kTpb, 0, st>>>(llist + loff[L], (int)lcnt[L], csub, cfirst, clast, clevel,
-                                                           bd, mo, cA, cB, cQ0, cQ1, box[3], p.usequad);
-    }
+    const int mgrid = ctx->ncu * 4;
+    for (int L = kMaxLevel; L >= 0; L--)
+        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1,
+                                                d_box, p.usequad);
     BH_HIP(hipGetLastError());
     T.first = cfirst;
     T.last = clast;
     T.fnb = ptr<int2>(ctx->cfnb);
     T.level = clevel;
-    T.skip = cskip;
+    T.skip = ptr<int>(ctx->cskip);
     T.sub = csub;
     T.nkids = cnkids;
     T.mo = mo;
@@ -1087,7 +1101,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cB = cB;
     T.cQ0 = cQ0;
     T.cQ1 = cQ1;
-    if (p.mode == BH_MODE_FAST) build_groups(ctx, p, ks, tks_for_groups);
+    if (p.mode == BH_MODE_FAST) build_groups(ctx, p, &tt, ks, tks_for_groups);
 }
 
 }  // namespace bh

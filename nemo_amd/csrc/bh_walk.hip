@@ -184,7 +184,7 @@ struct WalkArgs {
     int selfmode;
     const uint32_t *tperm;
     int64_t ntest;
-    int ngroups;
+    const unsigned long long *stat;
     const int *gfirst, *gcount;
     const float4 *bhi, *blo;
     const float4 *cA, *cB, *cQ0, *cQ1;
@@ -192,12 +192,12 @@ struct WalkArgs {
     const int2 *fnb;
     int nleaf;
     const uint8_t *level;
-    int nb, ncell;
+    int nb;
+    const unsigned long long *ncellp;
     float tolsq, eps2;
     int *frontier;
     int fcap;
     int *gctr;
-    int *werr;
     unsigned long long *counters;
     double *phi, *acc;
 };
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         int g = 0;
         if (lane == 0) g = atomicAdd(a.gctr, 1);
         g = __shfl(g, 0, 64);
-        if (g >= a.ngroups) break;
+        if (g >= (int)a.stat[S_NGROUPS]) break;
         const int64_t gbase = a.gfirst[g];
         const int gcount = a.gcount[g];
 
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
 
         int np = 0, nc = 0, nf = 0;
         long long tot_pp = 0, tot_pc = 0;
-        if (a.ncell == 0) {
+        if (*a.ncellp == 0) {
             if (a.nb == 1 && lane == 0) fr0[0] = 0;
             nf = a.nb == 1 ? 1 : 0;
         } else {
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
         if (overflow) {
-            if (lane == 0) atomicOr(a.werr, 1);
+            if (lane == 0) atomicOr(&a.counters[S_FLAGS], (unsigned long long)F_FRONTIER);
             continue;
         }
         wave_lds_sync();
@@ -426,6 +426,15 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
     }
 }
 
+__global__ void k_fill_zero(double *phi, int64_t n, double *acc)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) phi[i] = 0.0;
+    if (i < 3 * n) acc[i] = 0.0;
+}
+
+__global__ void k_zero_i32(int *p) { *p = 0; }
+
 template <int NPL, bool QUAD, bool SOFT0>
 void launch(bh_ctx *ctx, WalkArgs &a, int grid)
 {
@@ -440,12 +449,10 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     const int64_t nt = ctx->ntest;
     double *phi = ptr<double>(ctx->phi), *acc = ptr<double>(ctx->acc);
     if (T.nb == 0) {
-        BH_HIP(hipMemsetAsync(phi, 0, nt * sizeof(double), ctx->stream));
-        BH_HIP(hipMemsetAsync(acc, 0, 3 * nt * sizeof(double), ctx->stream));
+        k_fill_zero<<<std::max<int64_t>(1, (4 * nt + 255) / 256), 256, 0, ctx->stream>>>(phi, nt, acc);
         return;
     }
     const int npl = (p.group_size == 128) ? 2 : 1;
-    const int G = 64 * npl;
     WalkArgs a{};
     a.thi = T.thi;
     a.tlo = T.tlo;
@@ -453,10 +460,9 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.selfmode = T.selfmode;
     a.tperm = T.tperm;
     a.ntest = nt;
-    a.ngroups = ctx->ngroups;
+    a.stat = ptr<unsigned long long>(ctx->counters);
     a.gfirst = ptr<int>(ctx->gfirst);
     a.gcount = ptr<int>(ctx->gcount);
-    (void)G;
     a.bhi = T.bhi;
     a.blo = T.blo;
     a.cA = T.cA;
@@ -469,48 +475,32 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.nleaf = p.nleaf;
     a.level = T.level;
     a.nb = T.nb;
-    a.ncell = T.ncell;
+    a.ncellp = T.ncellp;
     a.tolsq = (float)(p.tol * p.tol);
     a.eps2 = (float)(p.eps * p.eps);
     a.counters = ptr<unsigned long long>(ctx->counters);
     a.phi = phi;
     a.acc = acc;
-    // persistent grid: enough blocks to fill the device, each wave loops over groups
-    int grid = std::min<int>((a.ngroups + kWaves - 1) / kWaves, ctx->ncu * 8);
+    // persistent grid sized for the device; each wave loops over groups
+    const int64_t max_groups = (nt + 1) / 1;
+    int grid = (int)std::min<int64_t>((max_groups + kWaves - 1) / kWaves, (int64_t)ctx->ncu * 8);
     grid = std::max(grid, 1);
     if (ctx->fcap == 0) ctx->fcap = 1 << 15;
-    for (int attempt = 0; attempt < 6; attempt++) {
-        a.fcap = (int)ctx->fcap;
-        a.frontier = ensure<int>(ctx->frontier, (size_t)grid * kWaves * 2 * ctx->fcap);
-        a.gctr = ensure<int>(ctx->gctr, 1);
-        a.werr = ensure<int>(ctx->werr, 1);
-        BH_HIP(hipMemsetAsync(a.gctr, 0, sizeof(int), ctx->stream));
-        BH_HIP(hipMemsetAsync(a.werr, 0, sizeof(int), ctx->stream));
-        const bool soft0 = p.eps == 0.0;
-        BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
-        if (npl == 2) {
-            if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);
-            else soft0 ? launch<2, false, true>(ctx, a, grid) : launch<2, false, false>(ctx, a, grid);
-        } else {
-            if (p.usequad) soft0 ? launch<1, true, true>(ctx, a, grid) : launch<1, true, false>(ctx, a, grid);
-            else soft0 ? launch<1, false, true>(ctx, a, grid) : launch<1, false, false>(ctx, a, grid);
-        }
-        BH_HIP(hipGetLastError());
-        BH_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
-        int werr = 0;
-        BH_HIP(hipMemcpyAsync(&werr, a.werr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        BH_HIP(hipStreamSynchronize(ctx->stream));
-        if (!werr) {
-            BH_HIP(hipEventElapsedTime(&ctx->t_walk_kernel_ms, ctx->ev[4], ctx->ev[5]));
-            return;
-        }
-        // frontier overflow: counters of the failed pass are discarded, retry with 4x capacity
-        BH_HIP(hipMemsetAsync(a.counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
-        ctx->fcap *= 4;
-        if ((size_t)grid * kWaves * 2 * ctx->fcap * sizeof(int) > ((size_t)64 << 30))
-            throw ApiFail(BH_ERR_CAPACITY, "walk frontier would exceed 64 GiB; use a larger tol");
+    a.fcap = (int)ctx->fcap;
+    a.frontier = ensure<int>(ctx->frontier, (size_t)grid * kWaves * 2 * ctx->fcap);
+    a.gctr = ensure<int>(ctx->gctr, 1);
+    k_zero_i32<<<1, 1, 0, ctx->stream>>>(a.gctr);
+    const bool soft0 = p.eps == 0.0;
+    if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    if (npl == 2) {
+        if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);
+        else soft0 ? launch<2, false, true>(ctx, a, grid) : launch<2, false, false>(ctx, a, grid);
+    } else {
+        if (p.usequad) soft0 ? launch<1, true, true>(ctx, a, grid) : launch<1, true, false>(ctx, a, grid);
+        else soft0 ? launch<1, false, true>(ctx, a, grid) : launch<1, false, false>(ctx, a, grid);
     }
-    throw ApiFail(BH_ERR_CAPACITY, "walk frontier capacity exceeded");
+    BH_HIP(hipGetLastError());
+    if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[5], ctx->stream));
 }
 
 }  // namespace bh
