@@ -115,6 +115,26 @@ bh_status bh_debug_groups(bh_ctx *ctx, int64_t cap, int *first, int *count, int6
 /* Force the next call to fail with BH_ERR_INJECTED (error-path tests). */
 void bh_debug_inject(bh_ctx *ctx, int code);
 
+/* hackcode1 time integration on the device (stepsystem, code.c:159-191;
+ * diagnostics, code_io.c:351-388).  State in FP64, input order.  The box
+ * starts from p's rmin/rsize (hackcode1: -2, 4; code.c:119-120) and then only
+ * grows.  A step = force pass (tree + walk), second-order velocity
+ * correction when nstep > 0, diagnostics of that state, drift.
+ * diag: 36 doubles per step: Energy[3] (T+U, T, U), KETensor[9],
+ * PETensor[9], AMTensor[9], CMPhaseSpace[6]. */
+bh_status bh_lf_init(bh_ctx *ctx, const bh_params *p, int64_t n, const double *mass, const double *pos,
+                     const double *vel, double dt);
+/* Force pass + velocity correction + diagnostics; the state is then at output time. */
+bh_status bh_lf_force(bh_ctx *ctx, double *diag, int64_t *n2b, int64_t *nbc);
+/* Half-kick, drift, half-kick; nstep += 1. */
+bh_status bh_lf_drift(bh_ctx *ctx);
+/* nsteps full steps with one synchronisation at the end; use_graph: after one
+ * eager step, the step is captured in a hipGraph and replayed.  diag
+ * [nsteps][36], n2b/nbc [nsteps] may be NULL. */
+bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, int64_t *n2b, int64_t *nbc);
+/* Copy the current state to the host; any pointer may be NULL. */
+bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *phi, int64_t *nstep);
+
 /* Synthetic inputs: equal-mass Plummer sphere in virial units (G=M=1,
  * E=-1/4) following mkplummer's recipe with mfrac=0.999, rfrac=22.8042468,
  * zero centre of mass; counter-based RNG keyed by seed, so the result does

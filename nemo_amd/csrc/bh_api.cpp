@@ -158,6 +158,36 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     if (st) *st = s;
 }
 
+void lf_upload_state(bh_ctx *ctx, int64_t n, const double *vel)
+{
+    double *dv = bh::ensure<double>(ctx->vel, 3 * n);
+    double *da = bh::ensure<double>(ctx->acc_old, 3 * n);
+    BH_HIP(hipMemcpyAsync(dv, vel, 3 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipMemsetAsync(da, 0, 3 * n * sizeof(double), ctx->stream));
+    bh::ensure<int>(ctx->lfstep, 2);
+    bh::lf_reset(ctx);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void lf_drop_graph(bh_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_cap = 0;
+}
+
+// Parameters of one leapfrog force pass: after the first step the box is kept.
+bh_params lf_params(const bh_ctx *ctx)
+{
+    bh_params q = ctx->lfp;
+    if (q.group_size == 0) q.group_size = 128;
+    if (q.nleaf < 0) q.nleaf = 2;
+    q.keep_box = ctx->box_valid ? 1 : 0;
+    return q;
+}
+
 }  // namespace
 
 extern "C" {
@@ -209,6 +239,7 @@ void bh_destroy(bh_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    lf_drop_graph(ctx);
     bh::DevBuf *bufs[] = {&ctx->mass,   &ctx->pos,    &ctx->tpos,   &ctx->dbox,    &ctx->dboxi,  &ctx->keys0,
                           &ctx->keys1,  &ctx->vals0,  &ctx->vals1,  &ctx->hist,    &ctx->scan_tmp, &ctx->tkeys0,
                           &ctx->tkeys1, &ctx->tvals0, &ctx->tvals1, &ctx->nmassive, &ctx->bd,    &ctx->bhi,
@@ -220,7 +251,9 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
-                          &ctx->lvals0, &ctx->lvals1, &ctx->cfnb};
+                          &ctx->lvals0, &ctx->lvals1, &ctx->cfnb,
+                          &ctx->vel,    &ctx->acc_old, &ctx->lfstep, &ctx->diag_partial, &ctx->diag_buf,
+                          &ctx->n2b_buf, &ctx->nbc_buf};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
@@ -362,6 +395,153 @@ bh_status bh_debug_groups(bh_ctx *ctx, int64_t cap, int *first, int *count, int6
         if (k <= 0) return;
         if (first) BH_HIP(hipMemcpy(first, ctx->gfirst.p, k * sizeof(int), hipMemcpyDeviceToHost));
         if (count) BH_HIP(hipMemcpy(count, ctx->gcount.p, k * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_lf_init(bh_ctx *ctx, const bh_params *p, int64_t n, const double *mass, const double *pos,
+                     const double *vel, double dt)
+{
+    return guarded(ctx, [&] {
+        check_params(p);
+        if (!vel) throw ApiFail(BH_ERR_ARG, "vel is NULL");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw ApiFail(BH_ERR_ARG, "dt must be finite and > 0");
+        if (p->mode == BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "leapfrog needs the fast or strict walk");
+        lf_drop_graph(ctx);
+        upload(ctx, n, mass, pos, 0, nullptr);
+        ctx->box_valid = false;
+        lf_upload_state(ctx, n, vel);
+        ctx->lfp = *p;
+        ctx->lf_dt = dt;
+        ctx->lf_nstep = 0;
+        ctx->lf_ready = true;
+        bh::ensure<double>(ctx->diag_buf, 36);
+        bh::ensure<long long>(ctx->n2b_buf, 1);
+        bh::ensure<long long>(ctx->nbc_buf, 1);
+    });
+}
+
+bh_status bh_lf_force(bh_ctx *ctx, double *diag, int64_t *n2b, int64_t *nbc)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_force before bh_lf_init");
+        bh_params q = lf_params(ctx);
+        bh::lf_reset_record(ctx);
+        eval(ctx, &q, nullptr);
+        double *dd = bh::ensure<double>(ctx->diag_buf, 36);
+        long long *d2 = bh::ensure<long long>(ctx->n2b_buf, 1), *dc = bh::ensure<long long>(ctx->nbc_buf, 1);
+        bh::lf_after_force(ctx, dd, d2, dc);
+        double h[36];
+        long long c[2];
+        BH_HIP(hipMemcpyAsync(h, dd, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(&c[0], d2, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(&c[1], dc, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        if (diag) memcpy(diag, h, sizeof h);
+        if (n2b) *n2b = c[0];
+        if (nbc) *nbc = c[1];
+    });
+}
+
+bh_status bh_lf_drift(bh_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_drift before bh_lf_init");
+        bh::lf_drift(ctx);
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->lf_nstep++;
+    });
+}
+
+bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, int64_t *n2b, int64_t *nbc)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_run before bh_lf_init");
+        if (nsteps < 0) throw ApiFail(BH_ERR_ARG, "nsteps < 0");
+        if (nsteps == 0) return;
+        hipStream_t st = ctx->stream;
+        const int64_t cap = std::max<int64_t>(nsteps, ctx->graph_cap);
+        double *dd = bh::ensure<double>(ctx->diag_buf, 36 * (size_t)cap);
+        long long *d2 = bh::ensure<long long>(ctx->n2b_buf, cap), *dc = bh::ensure<long long>(ctx->nbc_buf, cap);
+        bh::lf_reset_record(ctx);
+        unsigned long long *stat = bh::ptr<unsigned long long>(ctx->counters);
+        int64_t done = 0;
+        // one eager step first when the box or the capacities are not settled
+        if (!ctx->box_valid || !use_graph) {
+            bh_params q = lf_params(ctx);
+            eval(ctx, &q, nullptr);
+            bh::lf_after_force(ctx, dd, d2, dc);
+            bh::lf_drift(ctx);
+            done = 1;
+        }
+        bh_params q = lf_params(ctx);
+        q.keep_box = 1;
+        if (use_graph) {
+            const int key[4] = {ctx->cell_cap, ctx->tcell_cap, (int)ctx->fcap, (int)(ctx->box_rounds_keep)};
+            bool stale = ctx->graph_exec == nullptr || ctx->graph_cap < nsteps || memcmp(key, ctx->graph_key, sizeof key);
+            if (stale) {
+                lf_drop_graph(ctx);
+                BH_HIP(hipStreamSynchronize(st));
+                BH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                ctx->capturing = true;
+                try {
+                    bh::build_tree(ctx, q);
+                    if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
+                    else bh::walk_fast(ctx, q);
+                    bh::lf_after_force(ctx, dd, d2, dc);
+                    bh::lf_drift(ctx);
+                } catch (...) {
+                    ctx->capturing = false;
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                ctx->capturing = false;
+                BH_HIP(hipStreamEndCapture(st, &ctx->graph));
+                BH_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+                ctx->graph_cap = cap;
+                memcpy(ctx->graph_key, key, sizeof key);
+            }
+            for (int64_t s = done; s < nsteps; s++) BH_HIP(hipGraphLaunch(ctx->graph_exec, st));
+        } else {
+            for (int64_t s = done; s < nsteps; s++) {
+                bh::build_tree(ctx, q);
+                if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
+                else bh::walk_fast(ctx, q);
+                bh::lf_after_force(ctx, dd, d2, dc);
+                bh::lf_drift(ctx);
+            }
+        }
+        unsigned long long h[bh::S_NSLOTS];
+        double box[4];
+        BH_HIP(hipMemcpyAsync(h, stat, sizeof h, hipMemcpyDeviceToHost, st));
+        BH_HIP(hipMemcpyAsync(box, ctx->dbox.p, sizeof box, hipMemcpyDeviceToHost, st));
+        if (diag) BH_HIP(hipMemcpyAsync(diag, dd, 36 * nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (n2b) BH_HIP(hipMemcpyAsync(n2b, d2, nsteps * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        if (nbc) BH_HIP(hipMemcpyAsync(nbc, dc, nsteps * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        BH_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < 4; k++) ctx->box[k] = box[k];
+        ctx->lf_nstep += nsteps;
+        const unsigned long long f = h[bh::S_STICKY] | h[bh::S_FLAGS];
+        if (f & bh::F_NONFINITE) throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body during the run");
+        if (f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER)) {
+            lf_drop_graph(ctx);
+            throw ApiFail(BH_ERR_CAPACITY, "a step of the batch exceeded a capacity (cells, frontier or box rounds); "
+                                           "the state is invalid: restart with bh_lf_init");
+        }
+    });
+}
+
+bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *phi, int64_t *nstep)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_get before bh_lf_init");
+        const int64_t n = ctx->nmass;
+        if (pos) BH_HIP(hipMemcpy(pos, ctx->pos.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        if (vel) BH_HIP(hipMemcpy(vel, ctx->vel.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        if (acc) BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        if (phi) BH_HIP(hipMemcpy(phi, ctx->phi.p, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (nstep) *nstep = ctx->lf_nstep;
     });
 }
 

@@ -46,6 +46,18 @@ struct bh_ctx {
     // outputs (input order of the test set)
     bh::DevBuf phi, acc;
 
+    // leapfrog state (input order): velocities, previous accelerations,
+    // device step counters {nstep, record}, per-step outputs, step graph
+    bh::DevBuf vel, acc_old, lfstep, diag_partial, diag_buf, n2b_buf, nbc_buf;
+    double lf_dt = 0.0;
+    bh_params lfp{};
+    bool lf_ready = false;
+    int64_t lf_nstep = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_cap = 0;         // steps the captured output buffers hold
+    int graph_key[4] = {0, 0, 0, 0};
+
     bh::Tree tree;
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays
@@ -94,5 +106,10 @@ void walk_strict(bh_ctx *ctx, const bh_params &p);
 void direct_sum(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *d_idx, double *d_phi, double *d_acc);
 // bh_walk.hip
 void walk_fast(bh_ctx *ctx, const bh_params &p);
+// bh_lf.hip
+void lf_reset(bh_ctx *ctx);
+void lf_reset_record(bh_ctx *ctx);
+void lf_after_force(bh_ctx *ctx, double *diag_out, long long *n2b_out, long long *nbc_out);
+void lf_drift(bh_ctx *ctx);
 
 }  // namespace bh

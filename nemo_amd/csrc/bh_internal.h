@@ -72,6 +72,8 @@ enum StatSlot {
     S_NGROUPS = 6,
     S_FLAGS = 7,
     S_TNCELL = 8, S_TNCELL_RAW = 9,   // topology of a separate test set (grouping only)
+    S_ZERO = 10,        // slots below are cleared at the start of every evaluation
+    S_STICKY = 10,      // flags or-ed over the steps of a leapfrog batch
     S_NSLOTS = 16
 };
 enum StatFlag : unsigned long long {

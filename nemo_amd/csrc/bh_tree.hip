@@ -958,7 +958,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *d_boxi = ensure<int>(ctx->dboxi, 8);
     unsigned long long *stat = ensure<unsigned long long>(ctx->counters, S_NSLOTS);
     const bool timing = !ctx->capturing;
-    k_zero_u64<<<1, 64, 0, st>>>(stat, S_NSLOTS);
+    k_zero_u64<<<1, 64, 0, st>>>(stat, S_ZERO);
 
     // -- box: start from the caller's box (or the previous one) and expand in
     //    input order; a fixed number of rounds, checked afterwards

@@ -72,6 +72,11 @@ SIGNATURES = {
     "bh_debug_perm": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
     "bh_debug_groups": (C.c_int, [_vp, C.c_int64, _i32, _i32, _i64]),
     "bh_mkplummer": (None, [C.c_int64, C.c_uint64, _d, _d, _d]),
+    "bh_lf_init": (C.c_int, [_vp, C.POINTER(Params), C.c_int64, _d, _d, _d, C.c_double]),
+    "bh_lf_force": (C.c_int, [_vp, _d, _i64, _i64]),
+    "bh_lf_drift": (C.c_int, [_vp]),
+    "bh_lf_run": (C.c_int, [_vp, C.c_int64, C.c_int, _d, _i64, _i64]),
+    "bh_lf_get": (C.c_int, [_vp, _d, _d, _d, _d, _i64]),
 }
 
 
@@ -224,6 +229,63 @@ class Engine:
 
     def inject(self, code):
         lib().bh_debug_inject(self._h, code)
+
+
+class Leapfrog:
+    """hackcode1 time integration on the device (bh_lf_*).  Implements the
+    stepper interface of nemo_amd.hackcode1.run: force(), diagnostics(),
+    drift(), and mass/pos/vel (downloaded on access)."""
+
+    def __init__(self, engine: Engine, mass, pos, vel, dt, **params):
+        params.setdefault("rsize", 4.0)          # hackcode1 box: rmin=-2, rsize=4 (code.c:119-120)
+        self.engine = engine
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        self.n = len(self.mass)
+        self.dt = dt
+        self.params = default_params(**params)
+        engine._check(lib().bh_lf_init(engine._h, C.byref(self.params), self.n, _p(self.mass), _p(pos), _p(vel), dt))
+        self._diag = None
+
+    def force(self):
+        d = np.zeros(36)
+        a = np.zeros(1, np.int64)
+        b = np.zeros(1, np.int64)
+        self.engine._check(lib().bh_lf_force(self.engine._h, _p(d), _p(a, _i64), _p(b, _i64)))
+        self._diag = d
+        return int(a[0]), int(b[0])
+
+    def diagnostics(self):
+        return self._diag
+
+    def drift(self):
+        self.engine._check(lib().bh_lf_drift(self.engine._h))
+
+    def run(self, nsteps, graph=True):
+        """nsteps full steps; returns (diag[nsteps,36], n2b[nsteps], nbc[nsteps])."""
+        d = np.zeros((nsteps, 36))
+        a = np.zeros(nsteps, np.int64)
+        b = np.zeros(nsteps, np.int64)
+        self.engine._check(lib().bh_lf_run(self.engine._h, nsteps, int(bool(graph)), _p(d), _p(a, _i64), _p(b, _i64)))
+        return d, a, b
+
+    def state(self):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        acc = np.zeros((self.n, 3))
+        phi = np.zeros(self.n)
+        ns = np.zeros(1, np.int64)
+        self.engine._check(lib().bh_lf_get(self.engine._h, _p(pos), _p(vel), _p(acc), _p(phi), _p(ns, _i64)))
+        return pos, vel, acc, phi, int(ns[0])
+
+    @property
+    def pos(self):
+        return self.state()[0]
+
+    @property
+    def vel(self):
+        return self.state()[1]
 
 
 @dataclass

@@ -54,7 +54,9 @@ def run(stepper, tstop=2.0, freq=32.0, freqout=4.0, minor_freqout=32.0, options=
                 write_mass = True
                 firstmass = False
         if (time_bit or major) and out is not None:
-            snapshot.put_snapshot(out, stepper.mass, stepper.pos, stepper.vel, tnow,
+            pos = stepper.pos if major else None
+            vel = stepper.vel if major else None
+            snapshot.put_snapshot(out, stepper.mass, pos, vel, tnow,
                                   write_mass=write_mass, write_phase=major, diag=diag)
         stepper.drift()
         tnow = tnow + 1.0 / freq

@@ -1,0 +1,80 @@
+"""hackcode1 time integration on the device against the oracle and the
+reference's own known answers."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from nemo_amd import engine as E
+from nemo_amd import hackcode1, snapshot
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def test_default_run_strict_matches_reference_kats(eng, golden):
+    """The 64-step default run (tol=1, eps=0.05, monopole) from the t=0
+    fixture, strict walk: same interaction totals as the reference log, same
+    bsf five-number summary as the reference Testfile, and a trajectory
+    bit-identical to the oracle's."""
+    s = snapshot.read(os.path.join(golden, "hack128.sun4"))[0]
+    kats = json.load(open(os.path.join(golden, "kats.json")))
+    lf = E.Leapfrog(eng, s.mass, s.pos, s.vel, 1.0 / 32.0, tol=1.0, eps=0.05, usequad=0, mode="strict")
+    rows, out = hackcode1.run(lf, out=bytearray())
+    assert rows[0].nttot == kats["hackcode1_t0"]["nttot"]
+    assert rows[-1].nttot == kats["hackcode1_t2"]["nttot"]
+    assert snapshot.bsf(bytes(out)) == kats["hackcode1_default_bsf"]["value"]
+    ref = oracle.Leapfrog(s.mass, s.pos, s.vel, tol=1.0, eps=0.05)
+    rrows, _ = hackcode1.run(ref)
+    pos, vel, acc, phi, nstep = lf.state()
+    assert nstep == 65
+    assert np.array_equal(pos, ref.pos) and np.array_equal(vel, ref.vel)
+    assert np.array_equal(acc, ref.acc) and np.array_equal(phi, ref.phi)
+    for a, b in zip(rows, rrows):
+        assert a.nttot == b.nttot
+        assert abs(a.etot - b.etot) < 1e-14
+
+
+@pytest.mark.parametrize("mode", ["strict", "fast"])
+def test_graph_replay_equals_eager(eng, mode):
+    """Steps captured in a hipGraph and replayed give bit-identical states to
+    eager steps, replay after replay (with eager work in between)."""
+    m, x, v = E.mkplummer(20000, seed=9)
+    kw = dict(tol=0.7, eps=0.02, usequad=1, mode=mode)
+    a = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, **kw)
+    da, na, ca = a.run(12, graph=False)
+    sa = a.state()
+    for rep in range(3):
+        b = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, **kw)
+        db, nb_, cb = b.run(5, graph=True)
+        db2, nb2, cb2 = b.run(7, graph=True)
+        sb = b.state()
+        assert np.array_equal(sa[0], sb[0]) and np.array_equal(sa[1], sb[1])
+        assert np.array_equal(np.concatenate([db, db2]), da)
+        assert np.array_equal(np.concatenate([nb_, nb2]), na)
+        eng.force(m[:100], x[:100], E.default_params(tol=0.0))   # eager work between replays
+
+
+def test_fast_leapfrog_energy_and_oracle(eng):
+    """Fast walk + leapfrog: close to the oracle's trajectory and energy."""
+    m, x, v = E.mkplummer(4000, seed=21)
+    lf = E.Leapfrog(eng, m, x, v, 1.0 / 32.0, tol=0.7, eps=0.05, usequad=1)
+    d, n2b, nbc = lf.run(33, graph=True)
+    ref = oracle.Leapfrog(m, x, v, tol=0.7, eps=0.05, usequad=True, fcells=2.0)
+    for _ in range(33):
+        rd, _, _ = ref.step()
+    e0, e1 = d[0, 0], d[-1, 0]
+    assert abs(e1 - e0) / abs(e0) < 2e-3
+    assert abs(d[-1, 0] - rd[0]) / abs(rd[0]) < 1e-3
+    pos = lf.state()[0]
+    # different force errors (FP32 group walk vs FP64 per-particle walk) in a
+    # chaotic system: the trajectories agree to the force-error level
+    assert np.median(np.linalg.norm(pos - ref.pos, axis=1)) < 5e-3
