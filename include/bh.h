@@ -135,6 +135,28 @@ bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, in
 /* Copy the current state to the host; any pointer may be NULL. */
 bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *phi, int64_t *nstep);
 
+/* Learned far-field correction (no reference counterpart).  A 12-64-64-64-3
+ * SiLU MLP on BF16 MFMA predicts, for a cell accepted by the group walk whose
+ * size/distance lies in the trained range, the residual between the exact
+ * force of the cell's bodies and its multipole force at the group centre;
+ * the sum over a group's cells is added to every member (params.learned).
+ * params: kMlpParams = 9347 floats (W1[64][12] b1 W2[64][64] b2 W3[64][64] b3
+ * W4[3][64] b4); norm[4] = output scale, size/distance range lo, hi, 0. */
+bh_status bh_mlp_init(bh_ctx *ctx, uint64_t seed);
+bh_status bh_mlp_set(bh_ctx *ctx, const float *params, const float *norm);
+bh_status bh_mlp_get(bh_ctx *ctx, float *params, float *norm);
+/* Batch inference / mean-squared-error gradient on host buffers (tests). */
+bh_status bh_mlp_forward(bh_ctx *ctx, int64_t n, const float *x, float *y);
+bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, float *grad, float *loss);
+/* Training samples from the uploaded snapshot: a group walk that records, for
+ * every stride-th group, its accepted cells with size/distance in [rlo, rhi];
+ * each gives (features, exact FP64 residual at the group centre). */
+bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo, double rhi, int64_t *nadded);
+bh_status bh_mlp_samples(bh_ctx *ctx, int64_t cap, float *x, float *y, int64_t *n);
+bh_status bh_mlp_clear_samples(bh_ctx *ctx);
+/* Adam on the collected samples (mean squared error of the scaled residual). */
+bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss);
+
 /* Synthetic inputs: equal-mass Plummer sphere in virial units (G=M=1,
  * E=-1/4) following mkplummer's recipe with mfrac=0.999, rfrac=22.8042468,
  * zero centre of mass; counter-based RNG keyed by seed, so the result does

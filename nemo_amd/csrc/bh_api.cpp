@@ -105,8 +105,15 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         bh::build_tree(ctx, q);
         BH_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-        if (q.mode == BH_MODE_STRICT) bh::walk_strict(ctx, q);
-        else bh::walk_fast(ctx, q);
+        if (q.mode == BH_MODE_STRICT) {
+            bh::walk_strict(ctx, q);
+        } else if (q.learned) {
+            if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "learned correction requested but no model is loaded");
+            bh::walk_fast(ctx, q, 1, 1);
+            bh::mlp_correct(ctx, q);
+        } else {
+            bh::walk_fast(ctx, q);
+        }
         BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
         BH_HIP(hipMemcpyAsync(stat, ctx->counters.p, sizeof stat, hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipMemcpyAsync(box, ctx->dbox.p, sizeof box, hipMemcpyDeviceToHost, ctx->stream));
@@ -115,7 +122,8 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         const unsigned long long f = stat[bh::S_FLAGS];
         if (f & bh::F_NONFINITE)
             throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
-        if (!(f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER))) break;
+        if (!(f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER | bh::F_LREC_OVF))) break;
+        if (f & bh::F_LREC_OVF) ctx->lcap *= 4;
         if (attempt >= 12) throw ApiFail(BH_ERR_CAPACITY, "evaluation did not converge (capacities)");
         if (f & bh::F_CELL_OVF)
             ctx->cell_cap = (int)std::min<uint64_t>(stat[bh::S_NCELL_RAW] + stat[bh::S_NCELL_RAW] / 4 + 64, INT32_MAX / 2);
@@ -253,7 +261,10 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
                           &ctx->lvals0, &ctx->lvals1, &ctx->cfnb,
                           &ctx->vel,    &ctx->acc_old, &ctx->lfstep, &ctx->diag_partial, &ctx->diag_buf,
-                          &ctx->n2b_buf, &ctx->nbc_buf};
+                          &ctx->n2b_buf, &ctx->nbc_buf, &ctx->lrec,   &ctx->cF0,    &ctx->cF1,
+                          &ctx->mlp.params, &ctx->mlp.wb, &ctx->mlp.bias, &ctx->mlp.gpart, &ctx->mlp.lpart,
+                          &ctx->mlp.adam_m, &ctx->mlp.adam_v, &ctx->mlp.grad, &ctx->mlp.loss, &ctx->mlp.sx,
+                          &ctx->mlp.sy, &ctx->mlp.snew_x, &ctx->mlp.snew_y, &ctx->mlp.soff, &ctx->mlp.corr};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
@@ -542,6 +553,204 @@ bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *
         if (acc) BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
         if (phi) BH_HIP(hipMemcpy(phi, ctx->phi.p, n * sizeof(double), hipMemcpyDeviceToHost));
         if (nstep) *nstep = ctx->lf_nstep;
+    });
+}
+
+// ---- learned correction ----------------------------------------------------------
+
+bh_status bh_mlp_init(bh_ctx *ctx, uint64_t seed)
+{
+    return guarded(ctx, [&] {
+        std::vector<float> P(bh::kMlpParams, 0.f);
+        uint64_t st = seed * 0x9e3779b97f4a7c15ULL + 1;
+        auto u = [&]() {
+            st ^= st << 13;
+            st ^= st >> 7;
+            st ^= st << 17;
+            return (double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+        };
+        auto fill = [&](int off, int rows, int cols) {
+            const double lim = std::sqrt(6.0 / cols);
+            for (int i = 0; i < rows * cols; i++) P[off + i] = (float)(lim * u());
+        };
+        fill(bh::kOffW1, bh::kMlpHid, bh::kMlpIn);
+        fill(bh::kOffW2, bh::kMlpHid, bh::kMlpHid);
+        fill(bh::kOffW3, bh::kMlpHid, bh::kMlpHid);
+        fill(bh::kOffW4, bh::kMlpOut, bh::kMlpHid);
+        for (int i = 0; i < bh::kMlpOut * bh::kMlpHid; i++) P[bh::kOffW4 + i] *= 0.1f;
+        bh::mlp_set(ctx, P.data(), nullptr);
+        ctx->mlp.adam_step = 0;
+    });
+}
+
+bh_status bh_mlp_set(bh_ctx *ctx, const float *params, const float *norm)
+{
+    return guarded(ctx, [&] {
+        if (!params) throw ApiFail(BH_ERR_ARG, "params is NULL");
+        bh::mlp_set(ctx, params, norm);
+        ctx->mlp.adam_step = 0;
+    });
+}
+
+bh_status bh_mlp_get(bh_ctx *ctx, float *params, float *norm)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
+        if (params)
+            BH_HIP(hipMemcpy(params, ctx->mlp.params.p, bh::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+        if (norm) memcpy(norm, ctx->mlp.norm, sizeof ctx->mlp.norm);
+    });
+}
+
+bh_status bh_mlp_forward(bh_ctx *ctx, int64_t n, const float *x, float *y)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
+        if (n < 1 || !x || !y) throw ApiFail(BH_ERR_ARG, "bad arguments");
+        float *dx, *dy;
+        BH_HIP(hipMalloc(&dx, n * bh::kMlpIn * sizeof(float)));
+        BH_HIP(hipMalloc(&dy, n * 3 * sizeof(float)));
+        BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+        bh::mlp_forward(ctx, dx, n, dy);
+        BH_HIP(hipMemcpyAsync(y, dy, n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        BH_HIP(hipFree(dx));
+        BH_HIP(hipFree(dy));
+    });
+}
+
+bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, float *grad, float *loss)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
+        if (n < 1 || !x || !y || !grad) throw ApiFail(BH_ERR_ARG, "bad arguments");
+        float *dx, *dy, *dg;
+        BH_HIP(hipMalloc(&dx, n * bh::kMlpIn * sizeof(float)));
+        BH_HIP(hipMalloc(&dy, n * 3 * sizeof(float)));
+        BH_HIP(hipMalloc(&dg, (bh::kMlpParams + 1) * sizeof(float)));
+        BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+        BH_HIP(hipMemcpy(dy, y, n * 3 * sizeof(float), hipMemcpyHostToDevice));
+        bh::mlp_grad(ctx, dx, dy, n, dg, dg + bh::kMlpParams);
+        BH_HIP(hipMemcpyAsync(grad, dg, bh::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        float l = 0;
+        BH_HIP(hipMemcpyAsync(&l, dg + bh::kMlpParams, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+        if (loss) *loss = l;
+        BH_HIP(hipFree(dx));
+        BH_HIP(hipFree(dy));
+        BH_HIP(hipFree(dg));
+    });
+}
+
+bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo, double rhi, int64_t *nadded)
+{
+    return guarded(ctx, [&] {
+        check_params(p);
+        if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_mlp_collect before bh_upload");
+        if (!(rlo >= 0.0 && rhi > rlo)) throw ApiFail(BH_ERR_ARG, "bad size/distance range");
+        ctx->mlp.norm[1] = (float)rlo;
+        ctx->mlp.norm[2] = (float)rhi;
+        bh_params q = *p;
+        if (q.group_size == 0) q.group_size = 128;
+        if (q.nleaf < 0) q.nleaf = 2;
+        q.mode = BH_MODE_FAST;
+        q.learned = 0;
+        if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
+        for (int attempt = 0;; attempt++) {
+            bh::build_tree(ctx, q);
+            bh::walk_fast(ctx, q, 2, std::max(1, stride));
+            unsigned long long stat[bh::S_NSLOTS];
+            BH_HIP(hipMemcpyAsync(stat, ctx->counters.p, sizeof stat, hipMemcpyDeviceToHost, ctx->stream));
+            BH_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->box_valid = true;
+            const unsigned long long f = stat[bh::S_FLAGS];
+            if (f & bh::F_NONFINITE) throw ApiFail(BH_ERR_NONFINITE, "non-finite position");
+            if (!f) break;
+            if (attempt > 12) throw ApiFail(BH_ERR_CAPACITY, "sample collection did not converge");
+            if (f & bh::F_CELL_OVF) ctx->cell_cap = (int)std::min<uint64_t>(stat[bh::S_NCELL_RAW] * 5 / 4 + 64, INT32_MAX / 2);
+            if (f & bh::F_TCELL_OVF) ctx->tcell_cap = (int)std::min<uint64_t>(stat[bh::S_TNCELL_RAW] * 5 / 4 + 64, INT32_MAX / 2);
+            if (f & bh::F_BOX_OPEN) ctx->box_rounds *= 4;
+            if (f & bh::F_FRONTIER) ctx->fcap *= 4;
+            if (f & bh::F_LREC_OVF) ctx->lcap *= 4;
+        }
+        const int64_t got = bh::mlp_collect(ctx, q);
+        if (nadded) *nadded = got;
+    });
+}
+
+bh_status bh_mlp_samples(bh_ctx *ctx, int64_t cap, float *x, float *y, int64_t *n)
+{
+    return guarded(ctx, [&] {
+        const int64_t ns = ctx->mlp.nsamples;
+        if (n) *n = ns;
+        const int64_t k = std::min(cap, ns);
+        if (k > 0 && x) BH_HIP(hipMemcpy(x, ctx->mlp.sx.p, k * bh::kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
+        if (k > 0 && y) BH_HIP(hipMemcpy(y, ctx->mlp.sy.p, k * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_mlp_clear_samples(bh_ctx *ctx)
+{
+    return guarded(ctx, [&] { ctx->mlp.nsamples = 0; });
+}
+
+bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss)
+{
+    return guarded(ctx, [&] {
+        bh::Mlp &M = ctx->mlp;
+        if (!M.ready) throw ApiFail(BH_ERR_STATE, "bh_mlp_train before bh_mlp_init");
+        const int64_t n = M.nsamples;
+        if (n < 64) throw ApiFail(BH_ERR_STATE, "too few training samples");
+        if (epochs < 1 || batch < 64 || !(lr > 0.f)) throw ApiFail(BH_ERR_ARG, "bad training arguments");
+        hipStream_t st = ctx->stream;
+        // output scale: rms of the targets on a sample of the set
+        if (M.adam_step == 0) {
+            const int64_t k = std::min<int64_t>(n, 1 << 20);
+            std::vector<float> h(3 * k);
+            BH_HIP(hipMemcpy(h.data(), M.sy.p, 3 * k * sizeof(float), hipMemcpyDeviceToHost));
+            double s2 = 0.0;
+            for (float v : h) s2 += (double)v * v;
+            M.norm[0] = (float)std::max(1e-12, std::sqrt(s2 / (3.0 * k)));
+        }
+        float *bx = bh::ensure<float>(M.snew_x, (size_t)batch * bh::kMlpIn);
+        float *by = bh::ensure<float>(M.snew_y, (size_t)batch * 3);
+        float *grad = bh::ensure<float>(M.grad, bh::kMlpParams);
+        const int64_t nb = n / batch;
+        float *lossb = bh::ensure<float>(M.loss, std::max<int64_t>(nb, 1));
+        uint64_t rs = seed * 0x9e3779b97f4a7c15ULL + 7;
+        auto rnd = [&]() {
+            rs ^= rs << 13;
+            rs ^= rs >> 7;
+            rs ^= rs << 17;
+            return rs;
+        };
+        auto gcd = [](uint64_t a, uint64_t b) {
+            while (b) {
+                uint64_t t = a % b;
+                a = b;
+                b = t;
+            }
+            return a;
+        };
+        for (int e = 0; e < epochs; e++) {
+            uint64_t a = (rnd() % (uint64_t)n) | 1;
+            while (gcd(a, (uint64_t)n) != 1) a = (a + 2) % (uint64_t)n;
+            const uint64_t c = rnd() % (uint64_t)n;
+            for (int64_t b = 0; b < nb; b++) {
+                bh::mlp_gather(ctx, a, c, b * batch, batch, bx, by);
+                bh::mlp_grad(ctx, bx, by, batch, grad, lossb + b);
+                bh::mlp_adam(ctx, grad, lr, ++M.adam_step);
+            }
+            if (epoch_loss) {
+                std::vector<float> lh(std::max<int64_t>(nb, 1));
+                BH_HIP(hipMemcpyAsync(lh.data(), lossb, nb * sizeof(float), hipMemcpyDeviceToHost, st));
+                BH_HIP(hipStreamSynchronize(st));
+                double s = 0.0;
+                for (int64_t b = 0; b < nb; b++) s += lh[b];
+                epoch_loss[e] = (float)(s / std::max<int64_t>(nb, 1));
+            }
+        }
+        BH_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -2,8 +2,22 @@
 #pragma once
 
 #include "bh_internal.h"
+#include "bh_mlp.h"
 
 #include <vector>
+
+namespace bh {
+// Learned-correction state: parameters (FP32 master, BF16 MFMA copy, biases),
+// optimiser moments, training samples and the per-group corrections.
+struct Mlp {
+    DevBuf params, wb, bias, gpart, lpart, adam_m, adam_v, grad, loss;
+    DevBuf sx, sy, snew_x, snew_y, soff, corr;
+    int64_t nsamples = 0;
+    float norm[kMlpNorm] = {1.f, 0.25f, 1.6f, 0.f};
+    bool ready = false;
+    int adam_step = 0;
+};
+}  // namespace bh
 
 struct bh_ctx {
     int device = 0;
@@ -58,6 +72,11 @@ struct bh_ctx {
     int64_t graph_cap = 0;         // steps the captured output buffers hold
     int graph_key[4] = {0, 0, 0, 0};
 
+    bh::Mlp mlp;
+    bh::DevBuf lrec;             // per-wave learned-correction records of the walk
+    int64_t lcap = 0;            // ints per wave region
+    int lnwave = 0;              // waves (regions) of the last recording walk
+    bh::DevBuf cF0, cF1;         // per-cell learned features
     bh::Tree tree;
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays
@@ -105,7 +124,16 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
 void walk_strict(bh_ctx *ctx, const bh_params &p);
 void direct_sum(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *d_idx, double *d_phi, double *d_acc);
 // bh_walk.hip
-void walk_fast(bh_ctx *ctx, const bh_params &p);
+void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode = 0, int lstride = 1);
+// bh_mlp.hip
+void mlp_set(bh_ctx *ctx, const float *host_params, const float *norm);
+void mlp_pack(bh_ctx *ctx);
+void mlp_forward(bh_ctx *ctx, const float *d_x, int64_t n, float *d_y);
+void mlp_grad(bh_ctx *ctx, const float *d_x, const float *d_y, int64_t n, float *d_grad, float *d_loss);
+void mlp_adam(bh_ctx *ctx, const float *d_grad, float lr, int step);
+void mlp_gather(bh_ctx *ctx, uint64_t a, uint64_t c, int64_t first, int64_t cnt, float *bx, float *by);
+void mlp_correct(bh_ctx *ctx, const bh_params &p);
+int64_t mlp_collect(bh_ctx *ctx, const bh_params &p);
 // bh_lf.hip
 void lf_reset(bh_ctx *ctx);
 void lf_reset_record(bh_ctx *ctx);

@@ -58,6 +58,8 @@ struct Tree {
     int cap = 0;                       // cell capacity of the arrays
     const unsigned long long *ncellp = nullptr;   // device cell count
     const double4 *dbox = nullptr;     // device root box (rmin.xyz, rsize)
+    const float4 *cF0 = nullptr;       // learned features: (com - centre)/size xyz, Q_yz/(M size^2)
+    const float4 *cF1 = nullptr;       // Q_xx, Q_yy, Q_xy, Q_xz over M size^2
 };
 
 constexpr int kNoChild = INT32_MIN;  // empty octant in Tree::sub
@@ -77,7 +79,7 @@ enum StatSlot {
     S_NSLOTS = 16
 };
 enum StatFlag : unsigned long long {
-    F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16
+    F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16, F_LREC_OVF = 32
 };
 
 }  // namespace bh

@@ -1,0 +1,727 @@
+// Learned far-field correction: a 12 -> 64 -> 64 -> 64 -> 3 SiLU MLP on BF16
+// MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) that predicts the
+// residual between the exact force of a cell's bodies and the cell's
+// multipole (monopole + quadrupole, softened as in grav.c:42-73) force.
+//
+// One wavefront handles 64 samples (rows).  A layer is a 64 x out x in GEMM
+// cut into 16x16 output tiles, each the sum of in/32 MFMAs.  Operand layout
+// of the 16x16x32 instruction (wave64): lane l holds A[l%16][8*(l/16)+e] and
+// B[8*(l/16)+e][l%16] for e = 0..7, and D[4*(l/16)+e][l%16] for e = 0..3.
+//
+// Features of a (cell, point) pair, all O(1): d/r (3, d = com - point),
+// size/r (1), (com - cell centre)/size (3), Q/(M size^2) components xx, yy,
+// xy, xz, yz (5).  Target: (a_exact - a_multipole) * r^2 / M / yscale.
+//
+// Parameters: FP32 master copy (W1[64][12] b1[64] W2[64][64] b2[64]
+// W3[64][64] b3[64] W4[3][64] b4[3]) and BF16 copies padded for the MFMA
+// operands (W1[64][32], W2[64][64], W3[64][64], W4[16][64]).
+#include "bh_ctx.h"
+#include "bh_mlp.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace bh {
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z)
+{
+    float s = 1.f / (1.f + __expf(-z));
+    return s * (1.f + z * (1.f - s));
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// A operand of tile (ti, kk) from a row-major bf16 matrix (row stride lda),
+// 16-byte aligned rows: rows ti*16.., columns kk..kk+31.
+__device__ __forceinline__ bf16x8 load_rows(const bf16 *M, int lda, int r0, int k0)
+{
+    const int l = lane_id();
+    return *reinterpret_cast<const bf16x8 *>(M + (r0 + (l & 15)) * lda + k0 + 8 * (l >> 4));
+}
+
+// A operand element-wise: A[i][k] = M[k*ld + i] (transposed access).
+__device__ __forceinline__ bf16x8 load_cols(const bf16 *M, int ld, int r0, int k0)
+{
+    const int l = lane_id();
+    bf16x8 v;
+    const int i = r0 + (l & 15), kb = k0 + 8 * (l >> 4);
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] = M[(kb + e) * ld + i];
+    return v;
+}
+
+// Weight offsets inside the BF16 parameter block.
+// W4 is padded to 32 zero rows so that the transposed product in the backward
+// pass (K = 32) stays inside the block.
+constexpr int kW1b = 0, kW2b = kW1b + 64 * 32, kW3b = kW2b + 64 * 64, kW4b = kW3b + 64 * 64, kWbTotal = kW4b + 32 * 64;
+
+struct Acts {
+    bf16 *x;            // [64][32] inputs
+    bf16 *z[3];         // pre-activations of the hidden layers [64][64] (training)
+    bf16 *h[3];         // hidden activations [64][64]; may all alias (inference)
+};
+
+// One hidden layer: out[64][64] = silu(in[64][K] W^T + b).  in/out may alias
+// (all reads happen before the writes).  zsave != nullptr keeps Z (bf16).
+template <int K>
+__device__ void layer_hidden(const bf16 *in, const bf16 *W, const float *b, bf16 *out, bf16 *zsave)
+{
+    const int l = lane_id();
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            acc[ti][tj] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < K; kk += 32)
+                acc[ti][tj] = mfma(load_rows(in, K, ti * 16, kk), load_rows(W, K, tj * 16, kk), acc[ti][tj]);
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            const int col = tj * 16 + (l & 15);
+            const float bias = b[col];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = ti * 16 + 4 * (l >> 4) + e;
+                const float z = acc[ti][tj][e] + bias;
+                if (zsave) zsave[row * 64 + col] = (bf16)z;
+                out[row * 64 + col] = (bf16)silu(z);
+            }
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Output layer: y[64][3] (rows in lanes as D fragments) = h W4^T + b4.
+// Returns for lane l the 4 rows 4*(l/16)+e of tile ti in column l%16.
+__device__ void layer_out(const bf16 *h, const bf16 *W4, const float *b4, float y[4][4])
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 32) acc = mfma(load_rows(h, 64, ti * 16, kk), load_rows(W4, 64, 0, kk), acc);
+        const int col = l & 15;
+        const float bias = col < 3 ? b4[col] : 0.f;
+#pragma unroll
+        for (int e = 0; e < 4; e++) y[ti][e] = acc[e] + bias;
+    }
+}
+
+// Forward pass of 64 rows already in a.x; y fragments as in layer_out.
+__device__ void mlp_forward_wave(const Acts &a, const bf16 *Wb, const float *bias, float y[4][4])
+{
+    layer_hidden<32>(a.x, Wb + kW1b, bias + 0, a.h[0], a.z[0]);
+    layer_hidden<64>(a.h[0], Wb + kW2b, bias + 64, a.h[1], a.z[1]);
+    layer_hidden<64>(a.h[1], Wb + kW3b, bias + 128, a.h[2], a.z[2]);
+    layer_out(a.h[2], Wb + kW4b, bias + 192, y);
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stage 64 rows of 12 FP32 features (rows >= nrow are zero) into a.x.
+__device__ void stage_features(const Acts &a, const float *f, int nrow)
+{
+    const int l = lane_id();
+    for (int c = 0; c < 32; c++) {
+        float v = (l < nrow && c < kMlpIn) ? f[(int64_t)l * kMlpIn + c] : 0.f;
+        a.x[l * 32 + c] = (bf16)v;
+    }
+    wave_sync_lds();
+}
+
+// ---- batch inference ---------------------------------------------------------
+
+constexpr int kActsBytes = 64 * 32 * 2 + 64 * 64 * 2;
+
+__global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ X, int64_t n, const bf16 *__restrict__ Wb,
+                                                    const float *__restrict__ bias, float *__restrict__ Y)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sh[64 * 64];
+    Acts a{sx, {nullptr, nullptr, nullptr}, {sh, sh, sh}};
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    stage_features(a, X + r0 * kMlpIn, (int)std::min<int64_t>(64, n - r0));
+    float y[4][4];
+    mlp_forward_wave(a, Wb, bias, y);
+    const int l = lane_id(), col = l & 15;
+    if (col < 3)
+#pragma unroll
+        for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int64_t row = r0 + ti * 16 + 4 * (l >> 4) + e;
+                if (row < n) Y[row * 3 + col] = y[ti][e];
+            }
+}
+
+// ---- training: forward + backward, per-wave partial gradients ----------------
+
+// dW[out][in] += sum_s dZ[s][out] * H[s][in]: A = dZ^T (transposed access of the
+// [64][ldz] dZ block), B = H rows.  Tiles: out 16*ti.., in 16*tj.., K = 64 samples.
+template <int OUT_T, int IN_T>
+__device__ void grad_w(const bf16 *dZ, int ldz, const bf16 *H, int ldh, float *gW, int in_real, int out_real)
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int ti = 0; ti < OUT_T; ti++)
+#pragma unroll
+        for (int tj = 0; tj < IN_T; tj++) {
+            f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 64; kk += 32) {
+                bf16x8 av = load_cols(dZ, ldz, ti * 16, kk);
+                bf16x8 bv;
+                const int j = tj * 16 + (l & 15), kb = kk + 8 * (l >> 4);
+#pragma unroll
+                for (int e = 0; e < 8; e++) bv[e] = H[(kb + e) * ldh + j];
+                acc = mfma(av, bv, acc);
+            }
+            const int in = tj * 16 + (l & 15);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int out = ti * 16 + 4 * (l >> 4) + e;
+                if (out < out_real && in < in_real) gW[out * in_real + in] = acc[e];
+            }
+        }
+}
+
+// dH[s][in] = sum_out dZ[s][out] W[out][in]: A = dZ rows (K = out), B[k=out][j=in]
+// = W[out][in] read element-wise from the BF16 weights (row stride ldw).
+template <int KOUT>
+__device__ void grad_h(const bf16 *dZ, int ldz, const bf16 *W, int ldw, float dH[4][4][4])
+{
+    const int l = lane_id();
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < KOUT; kk += 32) {
+                bf16x8 av = load_rows(dZ, ldz, ti * 16, kk);
+                bf16x8 bv;
+                const int j = tj * 16 + (l & 15), kb = kk + 8 * (l >> 4);
+#pragma unroll
+                for (int e = 0; e < 8; e++) bv[e] = W[(kb + e) * ldw + j];
+                acc = mfma(av, bv, acc);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) dH[ti][tj][e] = acc[e];
+        }
+}
+
+// Writes dZ = dH * silu'(Z) (bf16) into dz[64][64] and db[64] partial sums.
+__device__ void backprop_silu(const float dH[4][4][4], const bf16 *Z, bf16 *dz, float *gb)
+{
+    const int l = lane_id();
+    wave_sync_lds();
+    float colsum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++) {
+            const int col = tj * 16 + (l & 15);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = ti * 16 + 4 * (l >> 4) + e;
+                const float g = dH[ti][tj][e] * silu_grad((float)Z[row * 64 + col]);
+                const bf16 gb16 = (bf16)g;
+                dz[row * 64 + col] = gb16;
+                colsum[tj] += (float)gb16;
+            }
+        }
+    // column sums over the 64 rows: lanes with the same l%16 hold the same columns
+#pragma unroll
+    for (int tj = 0; tj < 4; tj++) {
+        float v = colsum[tj];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (l < 16) gb[tj * 16 + l] = v;
+    }
+    wave_sync_lds();
+}
+
+// One wave: 64 samples -> partial gradient (kMlpParams floats, FP32 master
+// layout) and partial loss.  rows >= nrow contribute nothing.
+__global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ X, const float *__restrict__ Yt, int64_t n,
+                                                 float inv_batch, const bf16 *__restrict__ Wb,
+                                                 const float *__restrict__ bias, float *__restrict__ gpart,
+                                                 float *__restrict__ lpart)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sz[3][64 * 64];
+    __shared__ __attribute__((aligned(16))) bf16 sh[3][64 * 64];
+    __shared__ __attribute__((aligned(16))) bf16 sd[64 * 64];
+    Acts a{sx, {sz[0], sz[1], sz[2]}, {sh[0], sh[1], sh[2]}};
+    const int l = lane_id();
+    const int64_t r0 = (int64_t)blockIdx.x * 64;
+    const int nrow = (int)std::min<int64_t>(64, n - r0);
+    stage_features(a, X + r0 * kMlpIn, nrow);
+    float y[4][4];
+    mlp_forward_wave(a, Wb, bias, y);
+    float *g = gpart + (int64_t)blockIdx.x * kMlpParams;
+    // dY = 2 (y - t) / batch on real rows/columns; kept as bf16 dZ4 [64][64] (cols 0..2)
+    const int col = l & 15;
+    float lsum = 0.f, b4s = 0.f;
+    for (int i = l; i < 64 * 64; i += 64) sd[i] = (bf16)0.f;
+    wave_sync_lds();
+#pragma unroll
+    for (int ti = 0; ti < 4; ti++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int row = ti * 16 + 4 * (l >> 4) + e;
+            if (col < 3 && row < nrow) {
+                const float d = y[ti][e] - Yt[(r0 + row) * 3 + col];
+                lsum += d * d;
+                const bf16 gd = (bf16)(2.f * d * inv_batch);
+                sd[row * 64 + col] = gd;
+                b4s += (float)gd;
+            }
+        }
+    wave_sync_lds();
+    // loss and db4
+    for (int d = 32; d >= 1; d >>= 1) lsum += __shfl_xor(lsum, d, 64);
+    if (l == 0) lpart[blockIdx.x] = lsum * inv_batch;
+    b4s += __shfl_xor(b4s, 16, 64);
+    b4s += __shfl_xor(b4s, 32, 64);
+    if (l < 3) g[kOffB4 + l] = b4s;
+    // layer 4 (W4 rows 3..31 are zero, dZ4 columns 3..63 are zero)
+    grad_w<1, 4>(sd, 64, sh[2], 64, g + kOffW4, 64, 3);
+    float dH[4][4][4];
+    grad_h<32>(sd, 64, Wb + kW4b, 64, dH);
+    // layer 3
+    backprop_silu(dH, sz[2], sd, g + kOffB3);
+    grad_w<4, 4>(sd, 64, sh[1], 64, g + kOffW3, 64, 64);
+    grad_h<64>(sd, 64, Wb + kW3b, 64, dH);
+    // layer 2
+    backprop_silu(dH, sz[1], sd, g + kOffB2);
+    grad_w<4, 4>(sd, 64, sh[0], 64, g + kOffW2, 64, 64);
+    grad_h<64>(sd, 64, Wb + kW2b, 64, dH);
+    // layer 1
+    backprop_silu(dH, sz[0], sd, g + kOffB1);
+    grad_w<4, 1>(sd, 64, sx, 32, g + kOffW1, kMlpIn, 64);
+}
+
+// Sum of the per-wave partial gradients in wave order (deterministic).
+__global__ void k_mlp_reduce(const float *__restrict__ gpart, int nw, float *__restrict__ grad, const float *lpart,
+                             float *loss)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < kMlpParams) {
+        float s = 0.f;
+        for (int w = 0; w < nw; w++) s += gpart[(int64_t)w * kMlpParams + p];
+        grad[p] = s;
+    }
+    if (p == 0) {
+        float s = 0.f;
+        for (int w = 0; w < nw; w++) s += lpart[w];
+        *loss = s;
+    }
+}
+
+// Adam step on the FP32 master parameters, then refresh the BF16 copies.
+__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ m, float *__restrict__ v,
+                       float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float g = G[p];
+    float mm = b1 * m[p] + (1.f - b1) * g;
+    float vv = b2 * v[p] + (1.f - b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    P[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+
+// ---- features and the learned correction of the walk ---------------------------
+
+// Features of cell c seen from point p (FP32, relative coordinates: rel = com -
+// p).  Returns scale = M / r^2 (the output unit) or 0 if r == 0.
+__device__ __forceinline__ float cell_features(float rx, float ry, float rz, float4 A, float4 B, float4 F0, float4 F1,
+                                               float *f)
+{
+    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+    if (!(r2 > 0.f)) {
+        for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+        return 0.f;
+    }
+    const float inv = rsqrtf(r2);
+    f[0] = rx * inv;
+    f[1] = ry * inv;
+    f[2] = rz * inv;
+    f[3] = sqrtf(B.w) * inv;   // size / r
+    f[4] = F0.x;
+    f[5] = F0.y;
+    f[6] = F0.z;
+    f[7] = F1.x;               // Qxx
+    f[8] = F1.y;               // Qyy
+    f[9] = F1.z;               // Qxy
+    f[10] = F1.w;              // Qxz
+    f[11] = F0.w;              // Qyz
+    return A.w / r2;
+}
+
+// Sum over lanes in a fixed xor-tree order (same result in every lane).
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// One wave per record region of the walk: for every group record, the MLP
+// over its cells (64 at a time) and the sum of the predicted residual
+// accelerations, written to corr[g].
+__global__ __launch_bounds__(64) void k_learned(const int *__restrict__ lrec, int lcap, const float4 *__restrict__ cA,
+                                                const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
+                                                const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
+                                                const float *__restrict__ bias, float yscale, float4 *corr,
+                                                unsigned long long *stat)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sh[64 * 64];
+    __shared__ float sy[64 * 4];
+    Acts a{sx, {nullptr, nullptr, nullptr}, {sh, sh, sh}};
+    const int l = lane_id();
+    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
+    const int used = reg[0];
+    unsigned long long napplied = 0;
+    for (int p = 1; p < used;) {
+        const int g = reg[p], n = reg[p + 1];
+        const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        for (int base = 0; base < n; base += 64) {
+            const bool ok = base + l < n;
+            float f[kMlpIn];
+            float scale = 0.f;
+            if (ok) {
+                const int c = reg[p + 5 + base + l];
+                const float4 A = cA[c], B = cB[c];
+                scale = cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c], cF1[c], f);
+            } else {
+                for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+            }
+            for (int k = 0; k < 32; k++) sx[l * 32 + k] = (bf16)(k < kMlpIn ? f[k] : 0.f);
+            wave_sync_lds();
+            float y[4][4];
+            mlp_forward_wave(a, Wb, bias, y);
+            const int col = l & 15;
+            if (col < 3)
+                for (int ti = 0; ti < 4; ti++)
+                    for (int e = 0; e < 4; e++) sy[(ti * 16 + 4 * (l >> 4) + e) * 4 + col] = y[ti][e];
+            wave_sync_lds();
+            const float w = yscale * scale;
+            ax += wave_sum(sy[l * 4 + 0] * w);
+            ay += wave_sum(sy[l * 4 + 1] * w);
+            az += wave_sum(sy[l * 4 + 2] * w);
+            wave_sync_lds();
+        }
+        if (l == 0) corr[g] = make_float4(ax, ay, az, 0.f);
+        napplied += (unsigned long long)n;
+        p += 5 + n;
+    }
+    if (l == 0 && napplied) atomicAdd(&stat[S_NMLP], napplied);
+}
+
+// Adds the group corrections to the accelerations of the group members.
+__global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
+                             const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
+                             const float4 *__restrict__ corr, double *acc)
+{
+    const int g = blockIdx.x;
+    if (g >= (int)stat[S_NGROUPS]) return;
+    const float4 c = corr[g];
+    const int f = gfirst[g], n = gcount[g];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int64_t o = tperm[f + k];
+        acc[3 * o] += (double)c.x;
+        acc[3 * o + 1] += (double)c.y;
+        acc[3 * o + 2] += (double)c.z;
+    }
+}
+
+__global__ void k_zero_f4(float4 *p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Cells recorded per region (sample mode), for the sample offsets.
+__global__ void k_region_cells(const int *__restrict__ lrec, int lcap, int nreg, uint32_t *cnt)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreg) return;
+    const int *reg = lrec + (int64_t)r * lcap;
+    uint32_t c = 0;
+    for (int p = 1; p < reg[0]; p += 5 + reg[p + 1]) c += reg[p + 1];
+    cnt[r] = c;
+}
+
+// Training samples: for every recorded (group centre, cell), the features and
+// the exact residual (FP64 softened direct sum over the cell's bodies minus
+// the FP64 multipole force of the walk's formula) in units of M / r^2.
+__global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ off,
+                                                const float4 *__restrict__ cA, const float4 *__restrict__ cB,
+                                                const float4 *__restrict__ cF0, const float4 *__restrict__ cF1,
+                                                const int2 *__restrict__ fnb, const double *__restrict__ mo,
+                                                const double4 *__restrict__ bd, double eps, int usequad,
+                                                float *__restrict__ X, float *__restrict__ Y)
+{
+    const int l = lane_id();
+    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
+    const int used = reg[0];
+    int64_t j = off[blockIdx.x];
+    const double e2 = eps * eps;
+    for (int p = 1; p < used;) {
+        const int n = reg[p + 1];
+        const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
+        const double tx = cx, ty = cy, tz = cz;
+        for (int k = 0; k < n; k++, j++) {
+            const int c = reg[p + 5 + k];
+            const int2 fn = fnb[c];
+            double ex = 0.0, ey = 0.0, ez = 0.0;
+            for (int b = l; b < fn.y; b += 64) {
+                const double4 q = bd[fn.x + b];
+                const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+                const double r2 = dx * dx + dy * dy + dz * dz + e2;
+                const double w = q.w / (r2 * sqrt(r2));
+                ex += dx * w;
+                ey += dy * w;
+                ez += dz * w;
+            }
+            for (int d = 32; d >= 1; d >>= 1) {
+                ex += __shfl_xor(ex, d, 64);
+                ey += __shfl_xor(ey, d, 64);
+                ez += __shfl_xor(ez, d, 64);
+            }
+            if (l == 0) {
+                const double *m = mo + 10 * (int64_t)c;
+                const double dx = m[1] - tx, dy = m[2] - ty, dz = m[3] - tz;
+                const double r2 = dx * dx + dy * dy + dz * dz + e2;
+                const double ri = 1.0 / sqrt(r2), ri2 = ri * ri;
+                const double mr3 = m[0] * ri * ri2;
+                double mx = dx * mr3, my = dy * mr3, mz = dz * mr3;
+                if (usequad) {
+                    const double *Q = m + 4;   // xx xy xz yy yz zz
+                    const double qx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+                    const double qy = Q[1] * dx + Q[3] * dy + Q[4] * dz;
+                    const double qz = Q[2] * dx + Q[4] * dy + Q[5] * dz;
+                    const double r5 = ri2 * ri2 * ri;
+                    const double phq = -0.5 * r5 * (dx * qx + dy * qy + dz * qz);
+                    const double cf = 5.0 * phq * ri2;
+                    mx -= dx * cf + qx * r5;
+                    my -= dy * cf + qy * r5;
+                    mz -= dz * cf + qz * r5;
+                }
+                float f[kMlpIn];
+                const float4 A = cA[c], B = cB[c];
+                const float scale =
+                    cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c], cF1[c], f);
+                for (int t = 0; t < kMlpIn; t++) X[j * kMlpIn + t] = f[t];
+                const double inv = scale > 0.f ? 1.0 / (double)scale : 0.0;
+                Y[j * 3] = (float)((ex - mx) * inv);
+                Y[j * 3 + 1] = (float)((ey - my) * inv);
+                Y[j * 3 + 2] = (float)((ez - mz) * inv);
+            }
+        }
+        p += 5 + n;
+    }
+}
+
+}  // namespace
+
+// Training batch b of an epoch: rows (a*i + c) mod n (a coprime with n), targets
+// divided by yscale.
+__global__ void k_gather_batch(const float *__restrict__ X, const float *__restrict__ Y, int64_t n, uint64_t a,
+                               uint64_t c, int64_t first, int64_t cnt, float inv_yscale, float *bx, float *by)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const uint64_t r = (uint64_t)((a * (uint64_t)(first + i) + c) % (uint64_t)n);
+    for (int k = 0; k < kMlpIn; k++) bx[i * kMlpIn + k] = X[r * kMlpIn + k];
+    for (int k = 0; k < 3; k++) by[i * 3 + k] = Y[r * 3 + k] * inv_yscale;
+}
+
+// Converts the FP32 master parameters to the padded BF16 MFMA layout and the
+// FP32 bias block (b1 | b2 | b3 | b4 padded to 16).
+__global__ void k_mlp_pack(const float *__restrict__ P, bf16 *__restrict__ Wb, float *__restrict__ bias)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < kWbTotal) {
+        float v = 0.f;
+        if (t < kW2b) {
+            int o = t / 32, i = t % 32;
+            v = i < kMlpIn ? P[kOffW1 + o * kMlpIn + i] : 0.f;
+        } else if (t < kW3b) {
+            v = P[kOffW2 + (t - kW2b)];
+        } else if (t < kW4b) {
+            v = P[kOffW3 + (t - kW3b)];
+        } else {
+            int o = (t - kW4b) / 64, i = (t - kW4b) % 64;
+            v = o < 3 ? P[kOffW4 + o * 64 + i] : 0.f;
+        }
+        Wb[t] = (bf16)v;
+    }
+    if (t < 64) {
+        bias[t] = P[kOffB1 + t];
+        bias[64 + t] = P[kOffB2 + t];
+        bias[128 + t] = P[kOffB3 + t];
+    }
+    if (t < 16) bias[192 + t] = t < 3 ? P[kOffB4 + t] : 0.f;
+}
+
+// ---- host side ---------------------------------------------------------------
+
+void mlp_pack(bh_ctx *ctx)
+{
+    Mlp &M = ctx->mlp;
+    k_mlp_pack<<<(kWbTotal + 255) / 256, 256, 0, ctx->stream>>>(ptr<float>(M.params), ptr<bf16>(M.wb), ptr<float>(M.bias));
+    BH_HIP(hipGetLastError());
+}
+
+void mlp_set(bh_ctx *ctx, const float *host_params, const float *norm)
+{
+    Mlp &M = ctx->mlp;
+    float *P = ensure<float>(M.params, kMlpParams);
+    ensure<bf16>(M.wb, kWbTotal);
+    ensure<float>(M.bias, 208);
+    BH_HIP(hipMemcpyAsync(P, host_params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    if (norm) {
+        for (int k = 0; k < kMlpNorm; k++) M.norm[k] = norm[k];
+    }
+    mlp_pack(ctx);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    M.ready = true;
+}
+
+
+// Learned correction after a walk with lmode = 1: per-group corrections from
+// the recorded cells, added to the members' accelerations.
+void mlp_correct(bh_ctx *ctx, const bh_params &p)
+{
+    Mlp &M = ctx->mlp;
+    const Tree &T = ctx->tree;
+    hipStream_t st = ctx->stream;
+    const int64_t ngcap = ctx->ntest;
+    float4 *corr = ensure<float4>(M.corr, ngcap);
+    k_zero_f4<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(corr, (int)ngcap);
+    k_learned<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.cA, T.cB, T.cF0, T.cF1,
+                                          ptr<bf16>(M.wb), ptr<float>(M.bias), M.norm[0], corr,
+                                          ptr<unsigned long long>(ctx->counters));
+    k_apply_corr<<<(int)ngcap, 64, 0, st>>>(ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
+                                            ptr<unsigned long long>(ctx->counters), T.tperm, corr,
+                                            ptr<double>(ctx->acc));
+    (void)p;
+    BH_HIP(hipGetLastError());
+}
+
+// Training samples from the last sample-mode walk, appended to the set.
+int64_t mlp_collect(bh_ctx *ctx, const bh_params &p)
+{
+    Mlp &M = ctx->mlp;
+    const Tree &T = ctx->tree;
+    hipStream_t st = ctx->stream;
+    const int nreg = ctx->lnwave;
+    uint32_t *cnt = ensure<uint32_t>(M.soff, 2 * (size_t)nreg + 2);
+    uint32_t *off = cnt + nreg + 1;
+    k_region_cells<<<(nreg + 255) / 256, 256, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, nreg, cnt);
+    exclusive_scan_u32(ctx, cnt, off, nreg);
+    uint32_t total = 0;
+    BH_HIP(hipMemcpyAsync(&total, off + nreg, sizeof total, hipMemcpyDeviceToHost, st));
+    BH_HIP(hipStreamSynchronize(st));
+    if (total == 0) return 0;
+    float *nx = ensure<float>(M.snew_x, (size_t)total * kMlpIn), *ny = ensure<float>(M.snew_y, (size_t)total * 3);
+    k_samples<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, off, T.cA, T.cB, T.cF0, T.cF1, T.fnb, T.mo, T.bd,
+                                   p.eps, p.usequad, nx, ny);
+    BH_HIP(hipGetLastError());
+    // append to the training set
+    const int64_t old = M.nsamples, nn = old + total;
+    DevBuf gx, gy;
+    float *X = ensure<float>(gx, (size_t)nn * kMlpIn), *Y = ensure<float>(gy, (size_t)nn * 3);
+    if (old) {
+        BH_HIP(hipMemcpyAsync(X, M.sx.p, old * kMlpIn * sizeof(float), hipMemcpyDeviceToDevice, st));
+        BH_HIP(hipMemcpyAsync(Y, M.sy.p, old * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    BH_HIP(hipMemcpyAsync(X + old * kMlpIn, nx, (size_t)total * kMlpIn * sizeof(float), hipMemcpyDeviceToDevice, st));
+    BH_HIP(hipMemcpyAsync(Y + old * 3, ny, (size_t)total * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+    BH_HIP(hipStreamSynchronize(st));
+    if (M.sx.p) BH_HIP(hipFree(M.sx.p));
+    if (M.sy.p) BH_HIP(hipFree(M.sy.p));
+    M.sx = gx;
+    M.sy = gy;
+    M.nsamples = nn;
+    return total;
+}
+
+void mlp_forward(bh_ctx *ctx, const float *d_x, int64_t n, float *d_y)
+{
+    Mlp &M = ctx->mlp;
+    if (n <= 0) return;
+    k_mlp_forward<<<(int)((n + 63) / 64), 64, 0, ctx->stream>>>(d_x, n, ptr<bf16>(M.wb), ptr<float>(M.bias), d_y);
+    BH_HIP(hipGetLastError());
+}
+
+// Gradient of the mean squared error over the batch (x, y) at the current
+// parameters: grad[kMlpParams], loss.
+void mlp_grad(bh_ctx *ctx, const float *d_x, const float *d_y, int64_t n, float *d_grad, float *d_loss)
+{
+    Mlp &M = ctx->mlp;
+    const int nw = (int)((n + 63) / 64);
+    float *gpart = ensure<float>(M.gpart, (size_t)nw * kMlpParams);
+    float *lpart = ensure<float>(M.lpart, nw);
+    k_mlp_grad<<<nw, 64, 0, ctx->stream>>>(d_x, d_y, n, 1.f / (float)n, ptr<bf16>(M.wb), ptr<float>(M.bias), gpart,
+                                           lpart);
+    k_mlp_reduce<<<(kMlpParams + 255) / 256, 256, 0, ctx->stream>>>(gpart, nw, d_grad, lpart, d_loss);
+    BH_HIP(hipGetLastError());
+}
+
+void mlp_gather(bh_ctx *ctx, uint64_t a, uint64_t c, int64_t first, int64_t cnt, float *bx, float *by)
+{
+    Mlp &M = ctx->mlp;
+    k_gather_batch<<<(int)((cnt + 255) / 256), 256, 0, ctx->stream>>>(ptr<float>(M.sx), ptr<float>(M.sy), M.nsamples, a, c,
+                                                                    first, cnt, 1.f / M.norm[0], bx, by);
+    BH_HIP(hipGetLastError());
+}
+
+void mlp_adam(bh_ctx *ctx, const float *d_grad, float lr, int step)
+{
+    Mlp &M = ctx->mlp;
+    float *m = ensure<float>(M.adam_m, kMlpParams), *v = ensure<float>(M.adam_v, kMlpParams);
+    if (step == 1) {
+        BH_HIP(hipMemsetAsync(m, 0, kMlpParams * sizeof(float), ctx->stream));
+        BH_HIP(hipMemsetAsync(v, 0, kMlpParams * sizeof(float), ctx->stream));
+    }
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    k_adam<<<(kMlpParams + 255) / 256, 256, 0, ctx->stream>>>(ptr<float>(M.params), d_grad, m, v, lr, b1, b2, 1e-8f, c1,
+                                                              c2);
+    mlp_pack(ctx);
+    BH_HIP(hipGetLastError());
+}
+
+}  // namespace bh

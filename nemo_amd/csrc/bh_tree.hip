@@ -657,7 +657,8 @@ __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__re
 __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__restrict__ cfirst,
                              const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
                              const double4 *__restrict__ bd, double *mo,
-                             float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad)
+                             float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad,
+                             const uint64_t *__restrict__ keys, double4 box, float4 *cF0, float4 *cF1)
 {
     // rsize: side of the root box (the cell at level L has side rsize / 2^L)
     const int L = clevel[c];
@@ -771,6 +772,24 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     cB[c] = B;
     cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
     cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
+    if (cF0) {
+        // geometric centre of the cell from the key prefix of its first body
+        const uint64_t key = keys[cfirst[c]];
+        uint64_t ix = 0, iy = 0, iz = 0;
+        for (int l = 0; l < L; l++) {
+            const int sh = 60 - 3 * l;
+            ix = (ix << 1) | ((key >> (sh + 2)) & 1);
+            iy = (iy << 1) | ((key >> (sh + 1)) & 1);
+            iz = (iz << 1) | ((key >> sh) & 1);
+        }
+        const double size = rsize * ldexp(1.0, -L);
+        const double gx = box.x + ((double)ix + 0.5) * size, gy = box.y + ((double)iy + 0.5) * size,
+                     gz = box.z + ((double)iz + 0.5) * size;
+        const double qn = 1.0 / (M * size * size);
+        cF0[c] = make_float4((float)((px - gx) / size), (float)((py - gy) / size), (float)((pz - gz) / size),
+                             (float)(q[4] * qn));
+        cF1[c] = make_float4((float)(q[0] * qn), (float)(q[3] * qn), (float)(q[1] * qn), (float)(q[2] * qn));
+    }
 }
 
 // Cells bucketed by level (a one-digit radix sort of (level, cell id)) so the
@@ -794,6 +813,7 @@ __global__ void k_level_offsets(const uint32_t *__restrict__ hist_scan, int nblk
 }
 
 __global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__restrict__ loff, int L,
+                                const uint64_t *__restrict__ keys, float4 *cF0, float4 *cF1,
                                 const int *__restrict__ csub,
                                 const int *__restrict__ cfirst, const int *__restrict__ clast,
                                 const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *mo,
@@ -801,9 +821,11 @@ __global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__
                                 const double4 *__restrict__ box, int usequad)
 {
     const double rsize = box->w;
+    const double4 bx = *box;
     const uint32_t lo = loff[L], hi = loff[L + 1];
     for (uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x; k < hi; k += gridDim.x * blockDim.x)
-        cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad);
+        cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad, keys, bx, cF0,
+                     cF1);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -1073,6 +1095,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double *mo = ensure<double>(ctx->cmo, 10 * (size_t)cap);
     float4 *cA = ensure<float4>(ctx->cA, cap), *cB = ensure<float4>(ctx->cB, cap);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
+    float4 *cF0 = ensure<float4>(ctx->cF0, cap), *cF1 = ensure<float4>(ctx->cF1, cap);
     // cells bucketed by level, then the moments level by level, deepest first
     uint64_t *lk0 = ensure<uint64_t>(ctx->lkeys0, cap), *lk1 = ensure<uint64_t>(ctx->lkeys1, cap);
     uint32_t *lv0 = ensure<uint32_t>(ctx->lvals0, cap), *lv1 = ensure<uint32_t>(ctx->lvals1, cap);
@@ -1086,8 +1109,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int *llist = (const int *)lvs;
     const int mgrid = ctx->ncu * 4;
     for (int L = kMaxLevel; L >= 0; L--)
-        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1,
-                                                d_box, p.usequad);
+        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, cF0, cF1, csub, cfirst, clast, clevel, bd, mo, cA,
+                                                cB, cQ0, cQ1, d_box, p.usequad);
     BH_HIP(hipGetLastError());
     T.first = cfirst;
     T.last = clast;
@@ -1101,6 +1124,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cB = cB;
     T.cQ0 = cQ0;
     T.cQ1 = cQ1;
+    T.cF0 = cF0;
+    T.cF1 = cF1;
     if (p.mode == BH_MODE_FAST) build_groups(ctx, p, &tt, ks, tks_for_groups);
 }
 

@@ -200,6 +200,14 @@ struct WalkArgs {
     int *gctr;
     unsigned long long *counters;
     double *phi, *acc;
+    // learned correction: record, per group, the accepted cells whose
+    // size/distance lies in [lr_lo, lr_hi] (squared bounds) into a per-wave
+    // region: [used][g, n, cx, cy, cz, cell ids ...] ...
+    int lmode;            // 0 off, 1 every group, 2 every lstride-th group
+    int lstride;
+    float lr_lo2, lr_hi2;
+    int *lrec;
+    int lcap;
 };
 
 template <int NPL, bool QUAD, bool SOFT0>
@@ -216,6 +224,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
     int *fr0 = a.frontier + (int64_t)(blockIdx.x * kWaves + w) * 2 * a.fcap;
     int *fr1 = fr0 + a.fcap;
 
+    int lused = 1;                       // used ints of this wave's record region
     for (;;) {
         int g = 0;
         if (lane == 0) g = atomicAdd(a.gctr, 1);
@@ -262,6 +271,20 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
 
         int np = 0, nc = 0, nf = 0;
         long long tot_pp = 0, tot_pc = 0;
+        const bool lrecord = a.lmode == 1 || (a.lmode == 2 && g % a.lstride == 0);
+        int *lreg = a.lrec + (int64_t)(blockIdx.x * kWaves + w) * a.lcap;
+        int lhead = 0, lcnt = 0;
+        if (lrecord) {
+            lhead = lused;
+            if (lhead + 5 + 64 > a.lcap) {
+                if (lane == 0) atomicOr(&a.counters[S_FLAGS], (unsigned long long)F_LREC_OVF);
+            } else if (lane == 0) {
+                lreg[lhead] = g;
+                lreg[lhead + 2] = __float_as_int(cx);
+                lreg[lhead + 3] = __float_as_int(cy);
+                lreg[lhead + 4] = __float_as_int(cz);
+            }
+        }
         if (*a.ncellp == 0) {
             if (a.nb == 1 && lane == 0) fr0[0] = 0;
             nf = a.nb == 1 ? 1 : 0;
@@ -313,6 +336,20 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                             sSelf[w][rel] = 1;
                     }
                     np += __popcll(m);
+                }
+                // accepted cells in the learned range -> this group's record
+                if (lrecord) {
+                    const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+                    const float q = accept ? a.cB[c].w : 0.f;
+                    const bool take = accept && q >= a.lr_lo2 * r2 && q <= a.lr_hi2 * r2;
+                    const unsigned long long m = __ballot(take);
+                    const int at = lhead + 5 + lcnt;
+                    if (at + 64 <= a.lcap) {
+                        if (take) lreg[at + mbcnt(m)] = c;
+                        lcnt += __popcll(m);
+                    } else if (m && lane == 0) {
+                        atomicOr(&a.counters[S_FLAGS], (unsigned long long)F_LREC_OVF);
+                    }
                 }
                 // accepted cells -> P2C list
                 {
@@ -401,6 +438,13 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         tot_pc += nc;
         wave_lds_sync();
 
+        if (lrecord && lhead + 5 + 64 <= a.lcap) {
+            lused = lhead + 5 + lcnt;
+            if (lane == 0) {
+                lreg[lhead + 1] = lcnt;
+                lreg[0] = lused;
+            }
+        }
         int nself = 0;
 #pragma unroll
         for (int q = 0; q < NPL; q++) {
@@ -435,6 +479,11 @@ __global__ void k_fill_zero(double *phi, int64_t n, double *acc)
 
 __global__ void k_zero_i32(int *p) { *p = 0; }
 
+__global__ void k_clear_regions(int *lrec, int cap)
+{
+    if (threadIdx.x == 0) lrec[(int64_t)blockIdx.x * cap] = 1;
+}
+
 template <int NPL, bool QUAD, bool SOFT0>
 void launch(bh_ctx *ctx, WalkArgs &a, int grid)
 {
@@ -443,7 +492,7 @@ void launch(bh_ctx *ctx, WalkArgs &a, int grid)
 
 }  // namespace
 
-void walk_fast(bh_ctx *ctx, const bh_params &p)
+void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
 {
     const Tree &T = ctx->tree;
     const int64_t nt = ctx->ntest;
@@ -481,6 +530,10 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.counters = ptr<unsigned long long>(ctx->counters);
     a.phi = phi;
     a.acc = acc;
+    a.lmode = lmode;
+    a.lstride = std::max(1, lstride);
+    a.lr_lo2 = (float)(ctx->mlp.norm[1] * ctx->mlp.norm[1]);
+    a.lr_hi2 = (float)(ctx->mlp.norm[2] * ctx->mlp.norm[2]);
     // persistent grid sized for the device; each wave loops over groups
     const int64_t max_groups = (nt + 1) / 1;
     int grid = (int)std::min<int64_t>((max_groups + kWaves - 1) / kWaves, (int64_t)ctx->ncu * 8);
@@ -490,6 +543,17 @@ void walk_fast(bh_ctx *ctx, const bh_params &p)
     a.frontier = ensure<int>(ctx->frontier, (size_t)grid * kWaves * 2 * ctx->fcap);
     a.gctr = ensure<int>(ctx->gctr, 1);
     k_zero_i32<<<1, 1, 0, ctx->stream>>>(a.gctr);
+    const int nwave = grid * kWaves;
+    if (lmode) {
+        if (ctx->lcap == 0) ctx->lcap = 1 << 16;
+        a.lcap = (int)ctx->lcap;
+        a.lrec = ensure<int>(ctx->lrec, (size_t)nwave * ctx->lcap);
+        k_clear_regions<<<nwave, 64, 0, ctx->stream>>>(a.lrec, a.lcap);
+    } else {
+        a.lcap = 0;
+        a.lrec = ptr<int>(ctx->gctr);
+    }
+    ctx->lnwave = nwave;
     const bool soft0 = p.eps == 0.0;
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     if (npl == 2) {

@@ -52,6 +52,9 @@ _i64 = C.POINTER(C.c_int64)
 _u64 = C.POINTER(C.c_uint64)
 _i32 = C.POINTER(C.c_int)
 _vp = C.c_void_p
+_f = C.POINTER(C.c_float)
+MLP_PARAMS = 9347
+MLP_IN = 12
 
 # every symbol include/bh.h declares, with its ctypes signature
 SIGNATURES = {
@@ -77,6 +80,15 @@ SIGNATURES = {
     "bh_lf_drift": (C.c_int, [_vp]),
     "bh_lf_run": (C.c_int, [_vp, C.c_int64, C.c_int, _d, _i64, _i64]),
     "bh_lf_get": (C.c_int, [_vp, _d, _d, _d, _d, _i64]),
+    "bh_mlp_init": (C.c_int, [_vp, C.c_uint64]),
+    "bh_mlp_set": (C.c_int, [_vp, _f, _f]),
+    "bh_mlp_get": (C.c_int, [_vp, _f, _f]),
+    "bh_mlp_forward": (C.c_int, [_vp, C.c_int64, _f, _f]),
+    "bh_mlp_grad": (C.c_int, [_vp, C.c_int64, _f, _f, _f, _f]),
+    "bh_mlp_collect": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.c_double, C.c_double, _i64]),
+    "bh_mlp_samples": (C.c_int, [_vp, C.c_int64, _f, _f, _i64]),
+    "bh_mlp_clear_samples": (C.c_int, [_vp]),
+    "bh_mlp_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _f]),
 }
 
 
@@ -226,6 +238,57 @@ class Engine:
         c = np.zeros(n, np.int32)
         self._check(lib().bh_debug_groups(self._h, n, _p(f, _i32), _p(c, _i32), _p(ng, _i64)))
         return f, c
+
+    # -- learned correction -------------------------------------------------
+    def mlp_init(self, seed=1):
+        self._check(lib().bh_mlp_init(self._h, seed))
+
+    def mlp_set(self, params, norm=None):
+        p = np.ascontiguousarray(params, np.float32)
+        nrm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._check(lib().bh_mlp_set(self._h, _p(p, _f), None if nrm is None else _p(nrm, _f)))
+
+    def mlp_get(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        nrm = np.zeros(4, np.float32)
+        self._check(lib().bh_mlp_get(self._h, _p(p, _f), _p(nrm, _f)))
+        return p, nrm
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), 3), np.float32)
+        self._check(lib().bh_mlp_forward(self._h, len(x), _p(x, _f), _p(y, _f)))
+        return y
+
+    def mlp_grad(self, x, y):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(lib().bh_mlp_grad(self._h, len(x), _p(x, _f), _p(y, _f), _p(g, _f), _p(loss, _f)))
+        return g, float(loss[0])
+
+    def mlp_collect(self, params: Params, stride=1, rlo=0.25, rhi=1.6):
+        n = np.zeros(1, np.int64)
+        self._check(lib().bh_mlp_collect(self._h, C.byref(params), stride, rlo, rhi, _p(n, _i64)))
+        return int(n[0])
+
+    def mlp_samples(self):
+        n = np.zeros(1, np.int64)
+        self._check(lib().bh_mlp_samples(self._h, 0, None, None, _p(n, _i64)))
+        k = int(n[0])
+        x = np.zeros((k, MLP_IN), np.float32)
+        y = np.zeros((k, 3), np.float32)
+        self._check(lib().bh_mlp_samples(self._h, k, _p(x, _f), _p(y, _f), _p(n, _i64)))
+        return x, y
+
+    def mlp_clear_samples(self):
+        self._check(lib().bh_mlp_clear_samples(self._h))
+
+    def mlp_train(self, epochs=1, batch=8192, lr=1e-3, seed=1):
+        loss = np.zeros(epochs, np.float32)
+        self._check(lib().bh_mlp_train(self._h, epochs, batch, lr, seed, _p(loss, _f)))
+        return loss
 
     def inject(self, code):
         lib().bh_debug_inject(self._h, code)

@@ -1,0 +1,81 @@
+"""CPU oracle of the learned-correction MLP (numpy).
+
+TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's checks).
+
+Restates the device network 12 -> 64 -> 64 -> 64 -> 3 with SiLU: weights and
+activations rounded to bfloat16 where the device rounds them (inputs,
+weights, hidden activations, pre-activations kept for the backward pass,
+back-propagated dZ), products accumulated in float64 here (the device uses
+FP32 accumulation; the difference is far below the bf16 rounding).
+Parameter layout: W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+W4[3][64] b4[3].
+"""
+import numpy as np
+
+IN, HID, OUT = 12, 64, 3
+SIZES = [(HID, IN), (HID,), (HID, HID), (HID,), (HID, HID), (HID,), (OUT, HID), (OUT,)]
+NPARAMS = sum(int(np.prod(s)) for s in SIZES)
+
+
+EXACT = False   # tests of the oracle itself switch the rounding off
+
+
+def bf16(a):
+    """Round float32 values to bfloat16 (round to nearest even), as float64."""
+    if EXACT:
+        return np.asarray(a, np.float64)
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def unpack(p):
+    out, k = [], 0
+    for s in SIZES:
+        n = int(np.prod(s))
+        out.append(np.asarray(p[k:k + n], np.float64).reshape(s))
+        k += n
+    return out
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def silu_grad(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def forward(p, x, keep=False):
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
+    h = bf16(x)
+    zs, hs = [], [h]
+    for W, b in ((W1, b1), (W2, b2), (W3, b3)):
+        z = h @ bf16(W).T + b
+        h = bf16(silu(z if EXACT else z.astype(np.float32)))
+        zs.append(bf16(z))
+        hs.append(h)
+    y = h @ bf16(W4).T + b4
+    return (y, zs, hs) if keep else y
+
+
+def grad(p, x, t):
+    """Gradient of mean_s sum_k (y - t)^2 with the device's rounding points."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
+    n = len(x)
+    y, zs, hs = forward(p, x, keep=True)
+    d = y - t
+    loss = float((d * d).sum() / n)
+    dz = bf16(2.0 * d / n)
+    g = [None] * 8
+    g[6] = dz.T @ hs[3]
+    g[7] = dz.sum(0)
+    dh = dz @ bf16(W4)
+    for li, (W, zi) in ((2, (W3, 2)), (1, (W2, 1)), (0, (W1, 0))):
+        dz = bf16(dh * silu_grad(zs[zi]))
+        g[2 * li] = dz.T @ hs[li]
+        g[2 * li + 1] = dz.sum(0)
+        dh = dz @ bf16(W)
+    return np.concatenate([a.reshape(-1) for a in g]), loss

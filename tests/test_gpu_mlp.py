@@ -1,0 +1,81 @@
+"""Learned correction: BF16-MFMA forward/backward against the CPU oracle,
+sample generation, training, and the gain over the plain tree."""
+import numpy as np
+import pytest
+
+from nemo_amd import engine as E
+from oracle import mlp_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = E.Engine(0)
+    yield e
+    e.close()
+
+
+def test_forward_matches_oracle(eng):
+    eng.mlp_init(seed=3)
+    p, _ = eng.mlp_get()
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(300, 12)).astype(np.float32)
+    y = eng.mlp_forward(x)
+    ref = mlp_oracle.forward(p, x)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 2e-2, err
+
+
+def test_gradient_matches_oracle(eng):
+    eng.mlp_init(seed=4)
+    p, _ = eng.mlp_get()
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(256, 12)).astype(np.float32)
+    t = rng.normal(size=(256, 3)).astype(np.float32) * 0.3
+    g, loss = eng.mlp_grad(x, t)
+    gref, lref = mlp_oracle.grad(p, x, t)
+    assert abs(loss - lref) / lref < 1e-2
+    # per block of parameters, relative to the block norm
+    k = 0
+    for s in mlp_oracle.SIZES:
+        n = int(np.prod(s))
+        a, b = g[k:k + n], gref[k:k + n]
+        assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-7, (s, np.linalg.norm(a - b), np.linalg.norm(b))
+        k += n
+
+
+def test_training_reduces_loss_and_error(eng):
+    """Samples from one Plummer snapshot, training on the device, and the
+    trained model lowering the force error of the group walk at tol=1 on
+    another snapshot (the plain tree is the fallback outside the range)."""
+    m, x, _ = E.mkplummer(200000, seed=31)
+    eng.upload(m, x)
+    eng.mlp_init(seed=5)
+    eng.mlp_clear_samples()
+    p = E.default_params(tol=1.0, eps=0.01, usequad=1)
+    n = eng.mlp_collect(p, stride=2, rlo=0.3, rhi=1.0)
+    assert n > 100000
+    X, Y = eng.mlp_samples()
+    assert np.isfinite(X).all() and np.isfinite(Y).all()
+    # the residual grows with size/distance (higher multipoles)
+    sr = X[:, 3]
+    mag = np.linalg.norm(Y, axis=1)
+    assert np.median(mag[sr > 0.8]) > 2 * np.median(mag[sr < 0.45])
+    loss = eng.mlp_train(epochs=4, batch=8192, lr=2e-3)
+    assert loss[-1] < 0.8 * loss[0], loss
+    # evaluation on an independent snapshot
+    m2, x2, _ = E.mkplummer(200000, seed=32)
+    eng.upload(m2, x2)
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.permutation(len(m2))[:3000])
+    _, dacc = eng.direct(0.01, idx)
+    eng.eval(p)
+    _, a0 = eng.download()
+    pl = E.default_params(tol=1.0, eps=0.01, usequad=1, learned=1)
+    st = eng.eval(pl)
+    _, a1 = eng.download()
+    e0 = np.linalg.norm(a0[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    e1 = np.linalg.norm(a1[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    assert st.n_mlp > 0
+    assert np.median(e1) < np.median(e0), (np.median(e0), np.median(e1))

@@ -1,0 +1,34 @@
+"""The MLP oracle's backward pass against finite differences (rounding off),
+and the bf16 rounding helper against exact cases.  CPU only."""
+import numpy as np
+
+from oracle import mlp_oracle as M
+
+
+def test_bf16_rounding():
+    # bf16 keeps 7 fraction bits: ulp(1) = 2^-7; halfway cases go to even
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -9, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -8 + 2 ** -12], np.float32)
+    r = M.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0
+    assert r[3] == 1.015625 and r[4] == -2.5 and r[5] == 1.0078125
+
+
+def test_backward_matches_finite_differences():
+    M.EXACT = True
+    try:
+        rng = np.random.default_rng(0)
+        p = rng.normal(size=M.NPARAMS) * 0.3
+        x = rng.normal(size=(7, M.IN))
+        t = rng.normal(size=(7, M.OUT))
+        g, loss = M.grad(p, x, t)
+        h = 1e-6
+        for k in rng.choice(M.NPARAMS, 60, replace=False).tolist() + [M.NPARAMS - 1, 0, 768]:
+            e = np.zeros(M.NPARAMS)
+            e[k] = h
+            lp = ((M.forward(p + e, x) - t) ** 2).sum() / len(x)
+            lm = ((M.forward(p - e, x) - t) ** 2).sum() / len(x)
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
+        assert np.isclose(loss, ((M.forward(p, x) - t) ** 2).sum() / len(x))
+    finally:
+        M.EXACT = False
