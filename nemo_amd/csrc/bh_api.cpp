@@ -108,7 +108,8 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         if (q.mode == BH_MODE_STRICT) {
             bh::walk_strict(ctx, q);
         } else if (q.learned) {
-            if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "learned correction requested but no model is loaded");
+            if (!ctx->mlp.ready && q.learned != 2)
+                throw ApiFail(BH_ERR_STATE, "learned correction requested but no model is loaded");
             bh::walk_fast(ctx, q, 1, 1);
             bh::mlp_correct(ctx, q);
         } else {
@@ -122,8 +123,11 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         const unsigned long long f = stat[bh::S_FLAGS];
         if (f & bh::F_NONFINITE)
             throw ApiFail(BH_ERR_NONFINITE, "non-finite position of a massive body (expandbox cannot fit it)");
-        if (!(f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER | bh::F_LREC_OVF))) break;
+        if (!(f & (bh::F_CELL_OVF | bh::F_TCELL_OVF | bh::F_BOX_OPEN | bh::F_FRONTIER | bh::F_LREC_OVF |
+                   bh::F_ECAP_OVF)))
+            break;
         if (f & bh::F_LREC_OVF) ctx->lcap *= 4;
+        if (f & bh::F_ECAP_OVF) ctx->mlp.ecap *= 2;
         if (attempt >= 12) throw ApiFail(BH_ERR_CAPACITY, "evaluation did not converge (capacities)");
         if (f & bh::F_CELL_OVF)
             ctx->cell_cap = (int)std::min<uint64_t>(stat[bh::S_NCELL_RAW] + stat[bh::S_NCELL_RAW] / 4 + 64, INT32_MAX / 2);
@@ -264,7 +268,9 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->n2b_buf, &ctx->nbc_buf, &ctx->lrec,   &ctx->cF0,    &ctx->cF1,
                           &ctx->mlp.params, &ctx->mlp.wb, &ctx->mlp.bias, &ctx->mlp.gpart, &ctx->mlp.lpart,
                           &ctx->mlp.adam_m, &ctx->mlp.adam_v, &ctx->mlp.grad, &ctx->mlp.loss, &ctx->mlp.sx,
-                          &ctx->mlp.sy, &ctx->mlp.snew_x, &ctx->mlp.snew_y, &ctx->mlp.soff, &ctx->mlp.corr};
+                          &ctx->mlp.sy, &ctx->mlp.snew_x, &ctx->mlp.snew_y, &ctx->mlp.soff, &ctx->mlp.corr,
+                          &ctx->mlp.ecell, &ctx->mlp.egrp, &ctx->mlp.gstart, &ctx->mlp.gcnt, &ctx->mlp.gcentre,
+                          &ctx->mlp.res};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)

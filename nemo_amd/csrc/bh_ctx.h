@@ -11,8 +11,9 @@ namespace bh {
 // optimiser moments, training samples and the per-group corrections.
 struct Mlp {
     DevBuf params, wb, bias, gpart, lpart, adam_m, adam_v, grad, loss;
-    DevBuf sx, sy, snew_x, snew_y, soff, corr;
+    DevBuf sx, sy, snew_x, snew_y, soff, corr, ecell, egrp, gstart, gcnt, gcentre, res;
     int64_t nsamples = 0;
+    int64_t ecap = 0;           // capacity of the flattened correction entries
     float norm[kMlpNorm] = {1.f, 0.25f, 1.6f, 0.f};
     bool ready = false;
     int adam_step = 0;

@@ -79,7 +79,7 @@ enum StatSlot {
     S_NSLOTS = 16
 };
 enum StatFlag : unsigned long long {
-    F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16, F_LREC_OVF = 32
+    F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16, F_LREC_OVF = 32, F_ECAP_OVF = 64
 };
 
 }  // namespace bh

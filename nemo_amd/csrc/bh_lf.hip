@@ -139,6 +139,11 @@ __global__ void k_lf_record(unsigned long long *stat, const int *rec, long long 
 }
 
 __global__ void k_lf_next(int *rec) { *rec += 1; }
+__global__ void k_lf_begin(int *rec, unsigned long long *stat)
+{
+    *rec = 0;
+    stat[S_STICKY] = 0;
+}
 __global__ void k_set(int *p, int v) { *p = v; }
 
 inline int grid_for(int64_t n) { return (int)std::min<int64_t>(4096, std::max<int64_t>(1, (3 * n + kTpb - 1) / kTpb)); }
@@ -151,10 +156,12 @@ void lf_reset(bh_ctx *ctx)
     BH_HIP(hipGetLastError());
 }
 
-// Restart the per-call record index (the step counter is kept).
+// Restart the per-call record index and clear the flags or-ed over the
+// steps of the call (the step counter is kept).
 void lf_reset_record(bh_ctx *ctx)
 {
-    k_set<<<1, 1, 0, ctx->stream>>>(ptr<int>(ctx->lfstep) + 1, 0);
+    k_lf_begin<<<1, 1, 0, ctx->stream>>>(ptr<int>(ctx->lfstep) + 1,
+                                          ensure<unsigned long long>(ctx->counters, S_NSLOTS));
     BH_HIP(hipGetLastError());
 }
 

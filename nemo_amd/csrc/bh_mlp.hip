@@ -155,7 +155,6 @@ __device__ void stage_features(const Acts &a, const float *f, int nrow)
 
 // ---- batch inference ---------------------------------------------------------
 
-constexpr int kActsBytes = 64 * 32 * 2 + 64 * 64 * 2;
 
 __global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ X, int64_t n, const bf16 *__restrict__ Wb,
                                                     const float *__restrict__ bias, float *__restrict__ Y)
@@ -394,68 +393,115 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
-// One wave per record region of the walk: for every group record, the MLP
-// over its cells (64 at a time) and the sum of the predicted residual
-// accelerations, written to corr[g].
-__global__ __launch_bounds__(64) void k_learned(const int *__restrict__ lrec, int lcap, const float4 *__restrict__ cA,
-                                                const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
-                                                const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
-                                                const float *__restrict__ bias, float yscale, float4 *corr,
-                                                unsigned long long *stat)
+// Flatten the recorded (group, cells) of every region into one entry list:
+// entry e -> cell id and group; per group: first entry, count, centre.
+__global__ void k_flatten(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ off, int64_t ecap,
+                          int *ecell, int *egrp, int *gstart, int *gcnt, float4 *gcentre)
+{
+    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
+    const int used = reg[0];
+    int64_t e = off[blockIdx.x];
+    for (int p = 1; p < used;) {
+        const int g = reg[p];
+        const int n = e + reg[p + 1] <= ecap ? reg[p + 1] : 0;
+        if (threadIdx.x == 0) {
+            gstart[g] = (int)e;
+            gcnt[g] = n;
+            gcentre[g] = make_float4(__int_as_float(reg[p + 2]), __int_as_float(reg[p + 3]),
+                                     __int_as_float(reg[p + 4]), 0.f);
+        }
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            ecell[e + k] = reg[p + 5 + k];
+            egrp[e + k] = g;
+        }
+        e += n;
+        p += 5 + reg[p + 1];
+    }
+}
+
+// One wave per 64 entries: features, MLP, predicted residual acceleration.
+__device__ void k_learned_flat_body(int64_t tile, const int *__restrict__ ecell, const int *__restrict__ egrp,
+                                                     int64_t ne, const float4 *__restrict__ gcentre,
+                                                     const float4 *__restrict__ cA, const float4 *__restrict__ cB,
+                                                     const float4 *__restrict__ cF0, const float4 *__restrict__ cF1,
+                                                     const bf16 *__restrict__ Wb, const float *__restrict__ bias,
+                                                     float yscale, float4 *__restrict__ res)
 {
     __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
     __shared__ __attribute__((aligned(16))) bf16 sh[64 * 64];
     __shared__ float sy[64 * 4];
     Acts a{sx, {nullptr, nullptr, nullptr}, {sh, sh, sh}};
     const int l = lane_id();
-    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
-    const int used = reg[0];
-    unsigned long long napplied = 0;
-    for (int p = 1; p < used;) {
-        const int g = reg[p], n = reg[p + 1];
-        const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
-        float ax = 0.f, ay = 0.f, az = 0.f;
-        for (int base = 0; base < n; base += 64) {
-            const bool ok = base + l < n;
-            float f[kMlpIn];
-            float scale = 0.f;
-            if (ok) {
-                const int c = reg[p + 5 + base + l];
-                const float4 A = cA[c], B = cB[c];
-                scale = cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c], cF1[c], f);
-            } else {
-                for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
-            }
-            for (int k = 0; k < 32; k++) sx[l * 32 + k] = (bf16)(k < kMlpIn ? f[k] : 0.f);
-            wave_sync_lds();
-            float y[4][4];
-            mlp_forward_wave(a, Wb, bias, y);
-            const int col = l & 15;
-            if (col < 3)
-                for (int ti = 0; ti < 4; ti++)
-                    for (int e = 0; e < 4; e++) sy[(ti * 16 + 4 * (l >> 4) + e) * 4 + col] = y[ti][e];
-            wave_sync_lds();
-            const float w = yscale * scale;
-            ax += wave_sum(sy[l * 4 + 0] * w);
-            ay += wave_sum(sy[l * 4 + 1] * w);
-            az += wave_sum(sy[l * 4 + 2] * w);
-            wave_sync_lds();
-        }
-        if (l == 0) corr[g] = make_float4(ax, ay, az, 0.f);
-        napplied += (unsigned long long)n;
-        p += 5 + n;
+    const int64_t e = tile * 64 + l;
+    float f[kMlpIn];
+    float scale = 0.f;
+    if (e < ne) {
+        const int c = ecell[e];
+        const float4 ctr = gcentre[egrp[e]];
+        const float4 A = cA[c], B = cB[c];
+        scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c], cF1[c], f);
+    } else {
+        for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
     }
-    if (l == 0 && napplied) atomicAdd(&stat[S_NMLP], napplied);
+    bf16x8 v0, v1;
+    for (int k = 0; k < 8; k++) {
+        v0[k] = (bf16)f[k];
+        v1[k] = (bf16)(k < kMlpIn - 8 ? f[8 + k] : 0.f);
+    }
+    bf16x8 zero;
+    for (int k = 0; k < 8; k++) zero[k] = (bf16)0.f;
+    bf16x8 *row = reinterpret_cast<bf16x8 *>(sx + l * 32);
+    row[0] = v0;
+    row[1] = v1;
+    row[2] = zero;
+    row[3] = zero;
+    wave_sync_lds();
+    float y[4][4];
+    mlp_forward_wave(a, Wb, bias, y);
+    const int col = l & 15;
+    if (col < 3)
+        for (int ti = 0; ti < 4; ti++)
+            for (int q = 0; q < 4; q++) sy[(ti * 16 + 4 * (l >> 4) + q) * 4 + col] = y[ti][q];
+    wave_sync_lds();
+    if (e < ne) {
+        const float w = yscale * scale;
+        res[e] = make_float4(sy[l * 4] * w, sy[l * 4 + 1] * w, sy[l * 4 + 2] * w, 0.f);
+    }
+    wave_sync_lds();   // the next tile reuses sx, sh and sy
 }
 
-// Adds the group corrections to the accelerations of the group members.
-__global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
-                             const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
-                             const float4 *__restrict__ corr, double *acc)
+__global__ void k_zero_i32n(int *p, int n)
 {
-    const int g = blockIdx.x;
-    if (g >= (int)stat[S_NGROUPS]) return;
-    const float4 c = corr[g];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_learned_count(const uint32_t *total, int64_t ecap, unsigned long long *stat)
+{
+    stat[S_NMLP] += *total;
+    if ((int64_t)*total > ecap) stat[S_FLAGS] |= (unsigned long long)F_ECAP_OVF;
+}
+
+// Entry count read on the device; each wave strides over the 64-entry tiles.
+__global__ __launch_bounds__(64) void k_learned_flat_guard(const int *__restrict__ ecell, const int *__restrict__ egrp,
+                                                           const uint32_t *__restrict__ total,
+                                                           const float4 *__restrict__ gcentre,
+                                                           const float4 *__restrict__ cA, const float4 *__restrict__ cB,
+                                                           const float4 *__restrict__ cF0,
+                                                           const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
+                                                           const float *__restrict__ bias, float yscale,
+                                                           float4 *__restrict__ res, int64_t ecap)
+{
+    const int64_t ne = min((int64_t)*total, ecap);
+    for (int64_t t = blockIdx.x; t * 64 < ne; t += gridDim.x)
+        k_learned_flat_body(t, ecell, egrp, ne, gcentre, cA, cB, cF0, cF1, Wb, bias, yscale, res);
+}
+
+// Adds the correction c to the accelerations of group g's members.
+__device__ __forceinline__ void apply_group(int g, float4 c, const int *__restrict__ gfirst,
+                                            const int *__restrict__ gcount, const uint32_t *__restrict__ tperm,
+                                            double *acc)
+{
     const int f = gfirst[g], n = gcount[g];
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const int64_t o = tperm[f + k];
@@ -463,6 +509,41 @@ __global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restri
         acc[3 * o + 1] += (double)c.y;
         acc[3 * o + 2] += (double)c.z;
     }
+}
+
+// Per group (one wave, striding over the groups): sum of its entries'
+// residuals in a fixed order, added to the members' accelerations.
+__global__ __launch_bounds__(64) void k_group_apply(const int *__restrict__ gstart, const int *__restrict__ gcnt,
+                                                    const unsigned long long *__restrict__ stat,
+                                                    const float4 *__restrict__ res, const int *__restrict__ gfirst,
+                                                    const int *__restrict__ gcount,
+                                                    const uint32_t *__restrict__ tperm, double *acc)
+{
+    const int ng = (int)stat[S_NGROUPS];
+    const int l = lane_id();
+    for (int g = blockIdx.x; g < ng; g += gridDim.x) {
+        const int s = gstart[g], n = gcnt[g];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        for (int k = l; k < n; k += 64) {
+            const float4 r = res[s + k];
+            ax += r.x;
+            ay += r.y;
+            az += r.z;
+        }
+        ax = wave_sum(ax);
+        ay = wave_sum(ay);
+        az = wave_sum(az);
+        apply_group(g, make_float4(ax, ay, az, 0.f), gfirst, gcount, tperm, acc);
+    }
+}
+
+// Adds the per-group corrections corr[] to the group members' accelerations.
+__global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
+                             const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
+                             const float4 *__restrict__ corr, double *acc)
+{
+    const int ng = (int)stat[S_NGROUPS];
+    for (int g = blockIdx.x; g < ng; g += gridDim.x) apply_group(g, corr[g], gfirst, gcount, tperm, acc);
 }
 
 __global__ void k_zero_f4(float4 *p, int n)
@@ -480,6 +561,76 @@ __global__ void k_region_cells(const int *__restrict__ lrec, int lcap, int nreg,
     uint32_t c = 0;
     for (int p = 1; p < reg[0]; p += 5 + reg[p + 1]) c += reg[p + 1];
     cnt[r] = c;
+}
+
+// Exact residual (a_exact - a_multipole) of cell c at point t, FP64; the wave
+// shares the direct sum over the cell's bodies; valid in lane 0.
+__device__ void exact_residual(int c, double tx, double ty, double tz, const int2 *__restrict__ fnb,
+                               const double *__restrict__ mo, const double4 *__restrict__ bd, double e2, int usequad,
+                               double *res)
+{
+    const int l = lane_id();
+    const int2 fn = fnb[c];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    for (int b = l; b < fn.y; b += 64) {
+        const double4 q = bd[fn.x + b];
+        const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        const double w = q.w / (r2 * sqrt(r2));
+        ex += dx * w;
+        ey += dy * w;
+        ez += dz * w;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        ex += __shfl_xor(ex, d, 64);
+        ey += __shfl_xor(ey, d, 64);
+        ez += __shfl_xor(ez, d, 64);
+    }
+    const double *m = mo + 10 * (int64_t)c;
+    const double dx = m[1] - tx, dy = m[2] - ty, dz = m[3] - tz;
+    const double r2 = dx * dx + dy * dy + dz * dz + e2;
+    const double ri = 1.0 / sqrt(r2), ri2 = ri * ri;
+    const double mr3 = m[0] * ri * ri2;
+    double mx = dx * mr3, my = dy * mr3, mz = dz * mr3;
+    if (usequad) {
+        const double *Q = m + 4;   // xx xy xz yy yz zz
+        const double qx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+        const double qy = Q[1] * dx + Q[3] * dy + Q[4] * dz;
+        const double qz = Q[2] * dx + Q[4] * dy + Q[5] * dz;
+        const double r5 = ri2 * ri2 * ri;
+        const double phq = -0.5 * r5 * (dx * qx + dy * qy + dz * qz);
+        const double cf = 5.0 * phq * ri2;
+        mx -= dx * cf + qx * r5;
+        my -= dy * cf + qy * r5;
+        mz -= dz * cf + qz * r5;
+    }
+    res[0] = ex - mx;
+    res[1] = ey - my;
+    res[2] = ez - mz;
+}
+
+// Diagnostic: the correction a perfect model would give (exact residual at the
+// group centre for every recorded cell).
+__global__ __launch_bounds__(64) void k_exact_corr(const int *__restrict__ lrec, int lcap, const int2 *__restrict__ fnb,
+                                                   const double *__restrict__ mo, const double4 *__restrict__ bd,
+                                                   double eps, int usequad, float4 *corr)
+{
+    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
+    const int used = reg[0];
+    for (int p = 1; p < used;) {
+        const int g = reg[p], n = reg[p + 1];
+        const double tx = __int_as_float(reg[p + 2]), ty = __int_as_float(reg[p + 3]), tz = __int_as_float(reg[p + 4]);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int k = 0; k < n; k++) {
+            double r[3];
+            exact_residual(reg[p + 5 + k], tx, ty, tz, fnb, mo, bd, eps * eps, usequad, r);
+            ax += r[0];
+            ay += r[1];
+            az += r[2];
+        }
+        if (lane_id() == 0) corr[g] = make_float4((float)ax, (float)ay, (float)az, 0.f);
+        p += 5 + n;
+    }
 }
 
 // Training samples: for every recorded (group centre, cell), the features and
@@ -500,53 +651,20 @@ __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, in
     for (int p = 1; p < used;) {
         const int n = reg[p + 1];
         const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
-        const double tx = cx, ty = cy, tz = cz;
         for (int k = 0; k < n; k++, j++) {
             const int c = reg[p + 5 + k];
-            const int2 fn = fnb[c];
-            double ex = 0.0, ey = 0.0, ez = 0.0;
-            for (int b = l; b < fn.y; b += 64) {
-                const double4 q = bd[fn.x + b];
-                const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
-                const double r2 = dx * dx + dy * dy + dz * dz + e2;
-                const double w = q.w / (r2 * sqrt(r2));
-                ex += dx * w;
-                ey += dy * w;
-                ez += dz * w;
-            }
-            for (int d = 32; d >= 1; d >>= 1) {
-                ex += __shfl_xor(ex, d, 64);
-                ey += __shfl_xor(ey, d, 64);
-                ez += __shfl_xor(ez, d, 64);
-            }
+            double r[3];
+            exact_residual(c, cx, cy, cz, fnb, mo, bd, e2, usequad, r);
             if (l == 0) {
-                const double *m = mo + 10 * (int64_t)c;
-                const double dx = m[1] - tx, dy = m[2] - ty, dz = m[3] - tz;
-                const double r2 = dx * dx + dy * dy + dz * dz + e2;
-                const double ri = 1.0 / sqrt(r2), ri2 = ri * ri;
-                const double mr3 = m[0] * ri * ri2;
-                double mx = dx * mr3, my = dy * mr3, mz = dz * mr3;
-                if (usequad) {
-                    const double *Q = m + 4;   // xx xy xz yy yz zz
-                    const double qx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
-                    const double qy = Q[1] * dx + Q[3] * dy + Q[4] * dz;
-                    const double qz = Q[2] * dx + Q[4] * dy + Q[5] * dz;
-                    const double r5 = ri2 * ri2 * ri;
-                    const double phq = -0.5 * r5 * (dx * qx + dy * qy + dz * qz);
-                    const double cf = 5.0 * phq * ri2;
-                    mx -= dx * cf + qx * r5;
-                    my -= dy * cf + qy * r5;
-                    mz -= dz * cf + qz * r5;
-                }
                 float f[kMlpIn];
                 const float4 A = cA[c], B = cB[c];
                 const float scale =
                     cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c], cF1[c], f);
                 for (int t = 0; t < kMlpIn; t++) X[j * kMlpIn + t] = f[t];
                 const double inv = scale > 0.f ? 1.0 / (double)scale : 0.0;
-                Y[j * 3] = (float)((ex - mx) * inv);
-                Y[j * 3 + 1] = (float)((ey - my) * inv);
-                Y[j * 3 + 2] = (float)((ez - mz) * inv);
+                Y[j * 3] = (float)(r[0] * inv);
+                Y[j * 3 + 1] = (float)(r[1] * inv);
+                Y[j * 3 + 2] = (float)(r[2] * inv);
             }
         }
         p += 5 + n;
@@ -624,19 +742,45 @@ void mlp_set(bh_ctx *ctx, const float *host_params, const float *norm)
 // the recorded cells, added to the members' accelerations.
 void mlp_correct(bh_ctx *ctx, const bh_params &p)
 {
+    const bool exact = p.learned == 2;   // diagnostic: exact residual at the group centre
     Mlp &M = ctx->mlp;
     const Tree &T = ctx->tree;
     hipStream_t st = ctx->stream;
     const int64_t ngcap = ctx->ntest;
-    float4 *corr = ensure<float4>(M.corr, ngcap);
-    k_zero_f4<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(corr, (int)ngcap);
-    k_learned<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.cA, T.cB, T.cF0, T.cF1,
-                                          ptr<bf16>(M.wb), ptr<float>(M.bias), M.norm[0], corr,
-                                          ptr<unsigned long long>(ctx->counters));
-    k_apply_corr<<<(int)ngcap, 64, 0, st>>>(ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
-                                            ptr<unsigned long long>(ctx->counters), T.tperm, corr,
-                                            ptr<double>(ctx->acc));
-    (void)p;
+    unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
+    const int gridg = (int)std::min<int64_t>(ngcap, (int64_t)ctx->ncu * 32);
+    if (exact) {
+        float4 *corr = ensure<float4>(M.corr, ngcap);
+        k_zero_f4<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(corr, (int)ngcap);
+        k_exact_corr<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.fnb, T.mo, T.bd, p.eps,
+                                                 p.usequad, corr);
+        k_apply_corr<<<gridg, 64, 0, st>>>(ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), stat, T.tperm, corr,
+                                           ptr<double>(ctx->acc));
+    } else {
+        // flatten the records, one MLP tile per wave, per-group sums
+        const int nreg = ctx->lnwave;
+        uint32_t *cnt = ensure<uint32_t>(M.soff, 2 * (size_t)nreg + 2);
+        uint32_t *off = cnt + nreg + 1;
+        k_region_cells<<<(nreg + 255) / 256, 256, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, nreg, cnt);
+        exclusive_scan_u32(ctx, cnt, off, nreg);
+        if (M.ecap == 0) M.ecap = std::max<int64_t>(1 << 20, 16 * ctx->ntest);
+        const int64_t ecap = M.ecap;
+        int *ecell = ensure<int>(M.ecell, ecap), *egrp = ensure<int>(M.egrp, ecap);
+        int *gstart = ensure<int>(M.gstart, ngcap), *gcnt = ensure<int>(M.gcnt, ngcap);
+        float4 *gctr = ensure<float4>(M.gcentre, ngcap);
+        float4 *res = ensure<float4>(M.res, ecap);
+        k_zero_i32n<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(gcnt, (int)ngcap);
+        k_flatten<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, off, ecap, ecell, egrp, gstart, gcnt, gctr);
+        // the number of entries is only known on the device: launch over the
+        // bound and let waves past the end exit
+        k_learned_count<<<1, 1, 0, st>>>(off + nreg, ecap, stat);
+        const int64_t tiles = (ecap + 63) / 64;
+        k_learned_flat_guard<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 32), 64, 0, st>>>(
+            ecell, egrp, off + nreg, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias), M.norm[0], res,
+            ecap);
+        k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
+                                            T.tperm, ptr<double>(ctx->acc));
+    }
     BH_HIP(hipGetLastError());
 }
 

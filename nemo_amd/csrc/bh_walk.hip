@@ -210,7 +210,7 @@ struct WalkArgs {
     int lcap;
 };
 
-template <int NPL, bool QUAD, bool SOFT0>
+template <int NPL, bool QUAD, bool SOFT0, bool LREC>
 __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
 {
     constexpr int G = 64 * NPL;
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
 
         int np = 0, nc = 0, nf = 0;
         long long tot_pp = 0, tot_pc = 0;
-        const bool lrecord = a.lmode == 1 || (a.lmode == 2 && g % a.lstride == 0);
+        const bool lrecord = LREC && (a.lmode == 1 || (a.lmode == 2 && g % a.lstride == 0));
         int *lreg = a.lrec + (int64_t)(blockIdx.x * kWaves + w) * a.lcap;
         int lhead = 0, lcnt = 0;
         if (lrecord) {
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     np += __popcll(m);
                 }
                 // accepted cells in the learned range -> this group's record
-                if (lrecord) {
+                if (LREC && lrecord) {
                     const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
                     const float q = accept ? a.cB[c].w : 0.f;
                     const bool take = accept && q >= a.lr_lo2 * r2 && q <= a.lr_hi2 * r2;
@@ -487,7 +487,8 @@ __global__ void k_clear_regions(int *lrec, int cap)
 template <int NPL, bool QUAD, bool SOFT0>
 void launch(bh_ctx *ctx, WalkArgs &a, int grid)
 {
-    k_walk<NPL, QUAD, SOFT0><<<grid, kTpb, 0, ctx->stream>>>(a);
+    if (a.lmode) k_walk<NPL, QUAD, SOFT0, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else k_walk<NPL, QUAD, SOFT0, false><<<grid, kTpb, 0, ctx->stream>>>(a);
 }
 
 }  // namespace

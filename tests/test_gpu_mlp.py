@@ -46,30 +46,33 @@ def test_gradient_matches_oracle(eng):
 
 
 def test_training_reduces_loss_and_error(eng):
-    """Samples from one Plummer snapshot, training on the device, and the
-    trained model lowering the force error of the group walk at tol=1 on
-    another snapshot (the plain tree is the fallback outside the range)."""
-    m, x, _ = E.mkplummer(200000, seed=31)
-    eng.upload(m, x)
+    """Samples from two Plummer snapshots, training on the device, and the
+    trained model lowering the force error of the group walk at tol=1 on a
+    third snapshot (the plain tree is the fallback outside the range)."""
     eng.mlp_init(seed=5)
     eng.mlp_clear_samples()
-    p = E.default_params(tol=1.0, eps=0.01, usequad=1)
-    n = eng.mlp_collect(p, stride=2, rlo=0.3, rhi=1.0)
-    assert n > 100000
+    for seed in (31, 33):
+        m, x, _ = E.mkplummer(300000, seed=seed)
+        eng.upload(m, x)
+        for tol in (0.8, 1.0, 1.2):
+            eng.mlp_collect(E.default_params(tol=tol, eps=0.01, usequad=1), stride=1, rlo=0.25, rhi=1.3)
     X, Y = eng.mlp_samples()
+    assert len(X) > 1000000
     assert np.isfinite(X).all() and np.isfinite(Y).all()
     # the residual grows with size/distance (higher multipoles)
     sr = X[:, 3]
     mag = np.linalg.norm(Y, axis=1)
     assert np.median(mag[sr > 0.8]) > 2 * np.median(mag[sr < 0.45])
-    loss = eng.mlp_train(epochs=4, batch=8192, lr=2e-3)
+    del X, Y
+    loss = eng.mlp_train(epochs=8, batch=8192, lr=1e-3)
     assert loss[-1] < 0.8 * loss[0], loss
     # evaluation on an independent snapshot
-    m2, x2, _ = E.mkplummer(200000, seed=32)
+    m2, x2, _ = E.mkplummer(300000, seed=32)
     eng.upload(m2, x2)
     rng = np.random.default_rng(0)
     idx = np.sort(rng.permutation(len(m2))[:3000])
     _, dacc = eng.direct(0.01, idx)
+    p = E.default_params(tol=1.0, eps=0.01, usequad=1)
     eng.eval(p)
     _, a0 = eng.download()
     pl = E.default_params(tol=1.0, eps=0.01, usequad=1, learned=1)
@@ -77,5 +80,11 @@ def test_training_reduces_loss_and_error(eng):
     _, a1 = eng.download()
     e0 = np.linalg.norm(a0[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
     e1 = np.linalg.norm(a1[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    print("median |da|/|a| plain %.3e learned %.3e; p99 %.3e %.3e" % (
+        np.median(e0), np.median(e1), np.percentile(e0, 99), np.percentile(e1, 99)))
     assert st.n_mlp > 0
-    assert np.median(e1) < np.median(e0), (np.median(e0), np.median(e1))
+    assert np.median(e1) < 0.75 * np.median(e0), (np.median(e0), np.median(e1))
+    # the correction is deterministic: a second evaluation is bit-identical
+    eng.eval(pl)
+    _, a2 = eng.download()
+    assert np.array_equal(a1, a2)
