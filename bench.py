@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: particle-force evaluations per second at N = 2^22 Plummer,
-quadrupole Barnes-Hut on one MI355X, at the opening angle where the median
-|da|/|a| against an FP64 direct sum is 1e-3.
+"""Benchmark: particle-force evaluations per second at N = 2^22 Plummer on
+one MI355X, at the opening angle where the median |da|/|a| against an FP64
+direct sum (8192-body seeded subset) is 1e-3: the learned-correction walk
+against the plain tree.
 
 A step is one full force evaluation of the resident snapshot: root box,
 Morton keys, radix sort, topology, FP64 moments, group walk with the
-interactions (and the learned correction when enabled).  Inputs are resident
-in HBM before timing starts; results stay on the device.
+interactions and, for the learned variant, the MLP correction.  Inputs are
+resident in HBM before timing starts; results stay on the device.
 
-Prints one JSON line (see the keys below).  The roofline entry is for the
-walk kernel, the dominant kernel: 20 flop per interaction (P2P and P2C,
-counted by the kernel) over the walk kernel's own event-timed duration.
-The cpu_baseline entry times the CPU restatement of hackforce (FP64,
-single thread) on a bounded sample of the same snapshot: one full tree
-build plus the walk of a subset of bodies, extrapolated to N.
+Plain tree: monopole and quadrupole cells are both calibrated to the target
+median error and the faster is the plain figure.  Learned: the MLP is
+trained on the device from residual samples of two other Plummer snapshots
+(seeds 41 and 43, tol 0.8/1.0/1.2), then the learned walk (quadrupole cells
++ correction) is calibrated to the same target on the seed-123 snapshot.
+`value` is the learned variant's evals/s (the plain figure and the ratio are
+in `config`); with --no-learned it is the plain figure.
+
+Prints one JSON line.  `roofline` is for the dominant kernel, the walk: 20
+flop per interaction (P2P and P2C, counted by the kernel) over the walk
+kernel's own event-timed duration, against MI355X's FP32 peak (the walk is
+FP32 vector-ALU bound; "mfma" is the compute-bound label of the schema).
+`roofline_mlp` is the MLP inference kernel: 18,304 flop per evaluation
+against the dense BF16 MFMA peak.  `cpu_baseline` times the CPU restatement
+of hackforce (FP64, single thread) on a bounded sample of the same snapshot:
+one full tree build plus the walk of a subset of bodies, extrapolated to N.
 """
 import argparse
 import json
@@ -27,8 +38,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-force evals/s, N=4M, median |da|/|a|=1e-3: learned vs plain tree"
-FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32 (packed); unpacked FMA rate is 78.6
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector packed / dense MFMA figure)
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 FLOP_PER_INTERACTION = 20
+FLOP_PER_MLP = 2 * (12 * 64 + 2 * 64 * 64 + 64 * 3)   # 18,304
 
 
 def rel_err(a, ref):
@@ -42,7 +55,7 @@ def measure_err(eng, E, p, idx, dacc):
     return float(np.median(e)), float(np.percentile(e, 99))
 
 
-def calibrate_tol(eng, E, base, idx, dacc, target, lo=0.2, hi=1.3, iters=9):
+def calibrate_tol(eng, E, base, idx, dacc, target, lo=0.2, hi=1.4, iters=9):
     """Largest tol whose median error is <= target (bisection; error grows with tol)."""
     best = None
     for _ in range(iters):
@@ -79,13 +92,17 @@ def cpu_baseline(m, x, idx_err, dacc, tol, eps, usequad, budget_s):
     pilot = idx_err[:64]
     _, _, _, _, t_tree, t_walk = oracle.force_sample(m, x, pilot, tol, eps, usequad)
     per = t_walk / len(pilot)
-    ns = int(max(64, min(len(idx_err), (budget_s - t_tree) / max(per, 1e-9))))
-    sample = idx_err[:ns]
+    n = len(m)
+    ns = int(max(64, min(n, (budget_s - 2 * t_tree) / max(per, 1e-9))))
+    # the error subset first, then further seeded random bodies up to the budget
+    rest = np.setdiff1d(np.random.default_rng(7).permutation(n)[: ns + len(idx_err)], idx_err)
+    sample = np.concatenate([idx_err, rest])[:ns]
+    sample = np.sort(sample)
     _, acc, n2b, nbc, t_tree, t_walk = oracle.force_sample(m, x, sample, tol, eps, usequad)
     per = t_walk / ns
-    n = len(m)
     t_full = t_tree + per * n
-    e = rel_err(acc, dacc[:ns])
+    have = np.isin(idx_err, sample)
+    e = rel_err(acc[np.searchsorted(sample, idx_err[have])], dacc[have])
     return {
         "value": n / t_full,
         "unit": "evals/s",
@@ -93,9 +110,30 @@ def cpu_baseline(m, x, idx_err, dacc, tol, eps, usequad, budget_s):
         "kind": "port",
         "sample": "oracle hackforce restatement (FP64, 1 thread): full tree build of N=%d (%.2f s) + walk of %d "
                   "bodies (%.2f s, %.0f interactions/body), extrapolated to N; same tol=%.4f eps=%g quad=%d; "
-                  "sample median |da|/|a|=%.3g" % (n, t_tree, ns, t_walk, (n2b + nbc) / ns, tol, eps, usequad,
-                                                   float(np.median(e))),
+                  "median |da|/|a| on the error subset %.3g" % (n, t_tree, ns, t_walk, (n2b + nbc) / ns, tol, eps,
+                                                                usequad, float(np.median(e))),
     }
+
+
+def train_learned(eng, E, n, eps, seeds=(41, 43), tols=(0.8, 1.0, 1.2), stride=3, epochs=10):
+    """Residual samples of other snapshots, then training on the device."""
+    t0 = time.perf_counter()
+    eng.mlp_init(5)
+    eng.mlp_clear_samples()
+    for s in seeds:
+        m, x, _ = E.mkplummer(n, seed=s, with_vel=False)
+        eng.upload(m, x)
+        for tol in tols:
+            eng.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=1), stride=stride, rlo=0.25, rhi=1.3)
+    nsamp = eng.mlp_nsamples()
+    t1 = time.perf_counter()
+    loss = eng.mlp_train(epochs=epochs, batch=8192, lr=1e-3)
+    t2 = time.perf_counter()
+    eng.mlp_clear_samples()
+    return {"samples": nsamp, "snapshots": "Plummer N=%d seeds %s, tol %s, every %d-th group" % (
+                n, list(seeds), list(tols), stride),
+            "epochs": epochs, "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
+            "t_collect_s": t1 - t0, "t_train_s": t2 - t1}
 
 
 def main():
@@ -107,20 +145,25 @@ def main():
     ap.add_argument("--eps", type=float, default=0.01)
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--nsub", type=int, default=8192)
-    ap.add_argument("--tol", type=float, default=None, help="skip calibration and use this tol")
+    ap.add_argument("--tol", type=float, default=None, help="skip the plain calibration and use this tol")
     ap.add_argument("--group", type=int, default=128)
-    ap.add_argument("--quad", type=int, default=-1, help="-1: calibrate both and keep the faster")
+    ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (multi-GPU sharding is "
-              "described in DESIGN.md, not built)" % args.gpus, file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (how the walk would shard "
+              "across GPUs is described, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from nemo_amd import engine as E
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
 
     if args.copy_probe:
         # a plain copy kernel of known size, so that FETCH_SIZE/WRITE_SIZE
@@ -133,77 +176,99 @@ def main():
         torch.cuda.synchronize()
         del a, b
     n = args.n
-    m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
     eng = E.Engine(0)
+    learned_train = None
+    if not args.no_learned:
+        learned_train = train_learned(eng, E, n, args.eps, epochs=args.epochs)
+        log("learned: %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
+            learned_train["samples"], learned_train["t_collect_s"], learned_train["t_train_s"],
+            learned_train["loss_first"], learned_train["loss_last"]))
+    m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
     eng.upload(m, x)
     rng = np.random.default_rng(123)
     idx = np.sort(rng.permutation(n)[: args.nsub])
     t0 = time.perf_counter()
     _, dacc = eng.direct(args.eps, idx)
-    t_direct = time.perf_counter() - t0
-    print("direct-sum reference: %d targets x %d sources in %.2f s" % (len(idx), n, t_direct), file=sys.stderr,
-          flush=True)
+    log("direct-sum reference: %d targets x %d sources in %.2f s" % (len(idx), n, time.perf_counter() - t0))
 
-    # plain tree: monopole and quadrupole cells are both calibrated to the
-    # target median error; the faster of the two is the plain baseline
-    variants = []
-    for usequad in ((0, 1) if args.quad < 0 else (args.quad,)):
-        base = dict(eps=args.eps, usequad=usequad, group_size=args.group)
-        if args.tol is None:
+    def variant(base, fixed_tol=None):
+        if fixed_tol is None:
             tol, med, p99 = calibrate_tol(eng, E, base, idx, dacc, args.target)
         else:
-            tol = args.tol
+            tol = fixed_tol
             med, p99 = measure_err(eng, E, E.default_params(**dict(base, tol=tol)), idx, dacc)
         p = E.default_params(**dict(base, tol=tol))
         sec, stats = time_steps(eng, p, args.steps, args.warmup)
-        print("plain tree usequad=%d: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (usequad, tol, med, p99, sec * 1e3),
-              file=sys.stderr, flush=True)
-        variants.append((sec, usequad, tol, med, p99, stats))
-    sec, usequad, tol, med, p99, stats = min(variants, key=lambda v: v[0])
-    value = n / sec
-    n_int = stats[-1].n2btot + stats[-1].nbctot
-    t_walk = float(np.mean([s.t_walk_kernel_ms for s in stats])) * 1e-3
-    achieved = FLOP_PER_INTERACTION * n_int / t_walk / 1e12
+        last = stats[-1]
+        return {"usequad": base["usequad"], "learned": base.get("learned", 0), "tol": tol,
+                "median_rel_err": med, "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
+                "interactions_per_step": int(last.n2btot + last.nbctot), "n2b": int(last.n2btot),
+                "nbc": int(last.nbctot), "n_mlp": int(last.n_mlp), "ngroups": int(last.ngroups),
+                "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])),
+                "t_sort_ms": float(np.mean([s.t_sort_ms for s in stats])),
+                "t_walk_kernel_ms": float(np.mean([s.t_walk_kernel_ms for s in stats])),
+                "t_mlp_kernel_ms": float(np.mean([s.t_mlp_ms for s in stats])),
+                "t_walk_phase_ms": float(np.mean([s.t_walk_ms for s in stats]))}
+
+    plain = []
+    for usequad in ((0, 1) if args.quad < 0 else (args.quad,)):
+        v = variant(dict(eps=args.eps, usequad=usequad, group_size=args.group), args.tol)
+        log("plain tree usequad=%d: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
+            usequad, v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
+        plain.append(v)
+    best_plain = min(plain, key=lambda v: v["ms_per_step"])
+    learned = None
+    if not args.no_learned:
+        learned = variant(dict(eps=args.eps, usequad=1, group_size=args.group, learned=1))
+        log("learned walk: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, %d evaluations)" % (
+            learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
+            learned["t_mlp_kernel_ms"], learned["n_mlp"]))
+    head_v = learned if learned is not None else best_plain
+    t_walk = head_v["t_walk_kernel_ms"] * 1e-3
+    achieved = FLOP_PER_INTERACTION * head_v["interactions_per_step"] / t_walk / 1e12
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head_v["evals_per_s"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
+        "ms_per_step": head_v["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic Plummer (mkplummer recipe, own counter RNG, seed 123)",
         "config": {
-            "workload": "plummer_n%d_plain_tree_matched_median_err" % n,
-            "n": n, "eps": args.eps, "tol": tol, "usequad": usequad, "group_size": args.group,
-            "plain_variants": [{"usequad": v[1], "tol": v[2], "median_rel_err": v[3], "p99_rel_err": v[4],
-                                "ms_per_step": v[0] * 1e3} for v in variants],
-            "median_rel_err": med, "p99_rel_err": p99, "target_median_err": args.target,
-            "err_subset": len(idx), "learned": None,
-            "interactions_per_step": int(n_int), "n2b": int(stats[-1].n2btot), "nbc": int(stats[-1].nbctot),
-            "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])),
-            "t_sort_ms": float(np.mean([s.t_sort_ms for s in stats])),
-            "t_walk_kernel_ms": t_walk * 1e3,
-            "interactions_per_s": n_int / sec,
-            "ngroups": int(stats[-1].ngroups), "mean_group_fill": n / max(1, stats[-1].ngroups),
+            "workload": "plummer_n%d_%s_matched_median_err" % (n, "learned_vs_plain" if learned else "plain_tree"),
+            "n": n, "eps": args.eps, "group_size": args.group, "target_median_err": args.target,
+            "err_subset": len(idx),
+            "value_is": "learned" if learned else "plain",
+            "learned": learned, "learned_training": learned_train,
+            "plain": best_plain, "plain_variants": plain,
+            "learned_over_plain": (learned["evals_per_s"] / best_plain["evals_per_s"]) if learned else None,
         },
         "roofline": {
-            "bound": "valu_fp32",
-            "kernel": "k_walk<%d,%s,false>" % (2 if args.group == 128 else 1, "true" if usequad else "false"),
+            "bound": "mfma",
+            "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
+            "kernel": "k_walk<%d,%s,false,%s>" % (2 if args.group == 128 else 1, "true" if head_v["usequad"] else "false",
+                                                 "true" if head_v["learned"] else "false"),
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "flop_convention": "20 flop per interaction (P2P and quadrupole P2C)",
+            "flop_convention": "20 flop per interaction (P2P and P2C), interactions counted by the kernel",
         },
     }
+    if learned and learned["t_mlp_kernel_ms"] > 0:
+        a_mlp = FLOP_PER_MLP * learned["n_mlp"] / (learned["t_mlp_kernel_ms"] * 1e-3) / 1e12
+        out["roofline_mlp"] = {"bound": "mfma", "kernel": "k_learned_flat_guard", "achieved": a_mlp,
+                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a_mlp / BF16_PEAK_TFLOPS,
+                               "traffic": None, "flop_convention": "18,304 flop per MLP evaluation"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, x, idx, dacc, tol, args.eps, usequad, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(m, x, idx, dacc, best_plain["tol"], args.eps, best_plain["usequad"],
+                                           args.cpu_budget)
     eng.close()
     print(json.dumps(out))
 

@@ -75,6 +75,7 @@ typedef struct {
     double t_walk_kernel_ms; /* device time of the walk kernel launch alone */
     double t_sort_ms;    /* device time of the mass-set key sort */
     int64_t ngroups;     /* fast mode: groups of test points walked */
+    double t_mlp_ms;     /* learned mode: device time of the MLP inference kernel alone */
 } bh_stats;
 
 typedef struct bh_ctx bh_ctx;

@@ -165,6 +165,11 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     s.t_walk_kernel_ms = tw;
     s.t_sort_ms = ts;
     s.ngroups = q.mode == BH_MODE_FAST ? ctx->ngroups : 0;
+    if (q.mode == BH_MODE_FAST && q.learned == 1) {
+        float tm = 0;
+        BH_HIP(hipEventElapsedTime(&tm, ctx->ev[8], ctx->ev[9]));
+        s.t_mlp_ms = tm;
+    }
     ctx->last = s;
     ctx->last_mode = q.mode;
     if (st) *st = s;

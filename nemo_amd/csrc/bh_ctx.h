@@ -86,7 +86,8 @@ struct bh_ctx {
     int box_rounds_keep = 2;    // expansion rounds from the previous box
     bool capturing = false;     // a graph is being captured: no events, no host syncs
     int last_mode = 0;
-    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // 0-2 evaluation phases, 4-5 walk kernel, 6-7 sort, 8-9 MLP inference
+    hipEvent_t ev[10] = {};
     float t_walk_kernel_ms = 0.f, t_sort_ms = 0.f;
     bh_stats last{};
 };

@@ -775,9 +775,11 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         // bound and let waves past the end exit
         k_learned_count<<<1, 1, 0, st>>>(off + nreg, ecap, stat);
         const int64_t tiles = (ecap + 63) / 64;
+        if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[8], st));
         k_learned_flat_guard<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 32), 64, 0, st>>>(
             ecell, egrp, off + nreg, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias), M.norm[0], res,
             ecap);
+        if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
         k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
                                             T.tperm, ptr<double>(ctx->acc));
     }

@@ -40,7 +40,7 @@ class Stats(C.Structure):
     _fields_ = [("n2btot", C.c_int64), ("nbctot", C.c_int64), ("ncell", C.c_int64), ("deep_buckets", C.c_int64),
                 ("rmin", C.c_double * 3), ("rsize", C.c_double), ("t_tree_ms", C.c_double),
                 ("t_walk_ms", C.c_double), ("n_mlp", C.c_int64), ("t_walk_kernel_ms", C.c_double),
-                ("t_sort_ms", C.c_double), ("ngroups", C.c_int64)]
+                ("t_sort_ms", C.c_double), ("ngroups", C.c_int64), ("t_mlp_ms", C.c_double)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k == "rmin" else getattr(self, k)) for k, _ in self._fields_}
@@ -281,6 +281,11 @@ class Engine:
         y = np.zeros((k, 3), np.float32)
         self._check(lib().bh_mlp_samples(self._h, k, _p(x, _f), _p(y, _f), _p(n, _i64)))
         return x, y
+
+    def mlp_nsamples(self):
+        n = np.zeros(1, np.int64)
+        self._check(lib().bh_mlp_samples(self._h, 0, None, None, _p(n, _i64)))
+        return int(n[0])
 
     def mlp_clear_samples(self):
         self._check(lib().bh_mlp_clear_samples(self._h))
