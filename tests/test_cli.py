@@ -1,0 +1,200 @@
+"""The hackforce command (C, over the C ABI) and its snapshot codec.
+
+CPU tests: the codec through `csf` (byte-swapped input, truncation, idempotent
+copies) and hackforce's keyword and file checks, which fail before any GPU
+call.  GPU tests: hackforce output against the oracle (bit-exact in strict
+mode), the reference's known interaction total, test= and multi-snapshot
+streams, options=, and the fast walk's error against the direct sum.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nemo_amd import snapshot
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "nemo_amd", "bin")
+
+
+def run(prog, *args, cwd=None):
+    return subprocess.run([os.path.join(BIN, prog)] + list(args), capture_output=True, cwd=cwd, timeout=300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(os.path.join(BIN, "csf")):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "nemo_amd"), "bin/csf"], check=True)
+
+
+def write_stream(path, snaps, history="test stream"):
+    out = bytearray()
+    snapshot.put_history(out, history)
+    for m, x, v, t in snaps:
+        snapshot.put_snapshot(out, m, x, v, t)
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+# ---- CPU: codec and argument handling ------------------------------------------------
+
+@pytest.mark.parametrize("name", ["hack128.sun4", "plum10"])
+def test_csf_copies_and_unswaps(golden, tmp_path, name):
+    src = os.path.join(golden, name)
+    a, b = str(tmp_path / "a"), str(tmp_path / "b")
+    assert run("csf", src, a).returncode == 0
+    assert run("csf", "in=" + a, "out=" + b).returncode == 0
+    raw, cp = open(src, "rb").read(), open(a, "rb").read()
+    assert len(raw) == len(cp)
+    # big-endian input is written in host (little-endian) order, then copies are exact
+    assert cp[:2] in (b"\x92\x09", b"\x92\x0b")
+    assert open(b, "rb").read() == cp
+    s0, s1 = snapshot.read(src)[0], snapshot.read(a)[0]
+    assert np.array_equal(s0.mass, s1.mass) and np.array_equal(s0.pos, s1.pos) and np.array_equal(s0.vel, s1.vel)
+    assert snapshot.bsf(raw) == snapshot.bsf(cp)
+
+
+def test_csf_rejects_truncated_and_garbage(golden, tmp_path):
+    raw = open(os.path.join(golden, "hack128.sun4"), "rb").read()
+    for k, blob in enumerate((raw[:3000], raw[:-1], b"\x00\x01garbage")):
+        p = tmp_path / ("t%d" % k)
+        p.write_bytes(blob)
+        r = run("csf", str(p), str(tmp_path / ("o%d" % k)))
+        assert r.returncode == 255
+        assert b"### Fatal error [csf]" in r.stderr
+
+
+def test_hackforce_refuses_existing_output(golden, tmp_path):
+    out = tmp_path / "exists"
+    out.write_bytes(b"x")
+    r = run("hackforce", os.path.join(golden, "hack128.sun4"), str(out))
+    assert r.returncode == 255
+    assert b"already exists" in r.stderr
+    assert out.read_bytes() == b"x"
+
+
+@pytest.mark.parametrize("args, msg", [
+    (["out=/nonexistent/x"], b"must be given"),
+    (["in=a", "out=b", "bogus=1"], b"unknown"),
+    (["in=a", "out=b", "mode=fastest"], b"mode"),
+    (["in=a", "out=b", "tol=abc"], b"not a number"),
+    (["in=/nonexistent/a", "out=b"], b"cannot open"),
+])
+def test_hackforce_argument_errors(tmp_path, args, msg):
+    r = run("hackforce", *args, cwd=str(tmp_path))
+    assert r.returncode == 255, r
+    assert b"### Fatal error [hackforce]" in r.stderr and msg in r.stderr, r.stderr
+
+
+def test_hackforce_help_lists_reference_keywords():
+    r = run("hackforce", "help")
+    assert r.returncode == 0
+    for k in (b"in=", b"out=", b"test=", b"tol=1.0", b"eps=0.05", b"rsize=4.0", b"rmin=", b"options=mass,phase",
+              b"fcells=0.75"):
+        assert k in r.stdout
+
+
+# ---- GPU: results ---------------------------------------------------------------------
+
+gpu = pytest.mark.gpu
+
+
+def _items(path):
+    return snapshot.parse(open(path, "rb").read())
+
+
+@gpu
+def test_hackforce_strict_matches_oracle_and_kat(golden, tmp_path):
+    """Default keywords (tol=1, eps=0.05, monopole) on the reference's
+    hack128 snapshot, strict mode: bit-identical to the oracle, and the
+    reference's published interaction total 3864 at t=0."""
+    from oracle import oracle
+
+    src = os.path.join(golden, "hack128.sun4")
+    out = str(tmp_path / "res")
+    r = run("hackforce", "in=" + src, "out=" + out, "mode=strict")
+    assert r.returncode == 0, r.stderr
+    items = _items(out)
+    hist = [it.data for it in items if it.tag == "History"]
+    assert len(hist) == 2 and hist[-1].startswith(b"hackforce in=")
+    snap = [it for it in items if it.tag == "SnapShot"]
+    assert len(snap) == 1
+    par, prt, dg = snap[0].find("Parameters"), snap[0].find("Particles"), snap[0].find("Diagnostics")
+    assert [c.tag for c in par.children] == ["Nobj", "Time", "tol", "eps"]
+    assert [c.tag for c in prt.children] == ["CoordSystem", "Mass", "PhaseSpace", "Potential", "Acceleration"]
+    assert [c.tag for c in dg.children] == ["n2btot", "nbctot", "cputree", "cpufcal"]
+    s = snapshot.read(src)[0]
+    phi, acc, n2b, nbc, _, _ = oracle.force(s.mass, s.pos, tol=1.0, eps=0.05)
+    assert np.array_equal(prt.find("Potential").data, phi)
+    assert np.array_equal(prt.find("Acceleration").data, acc)
+    assert int(dg.find("n2btot").data.reshape(-1)[0]) == n2b and int(dg.find("nbctot").data.reshape(-1)[0]) == nbc
+    assert n2b + nbc == 3864
+    assert np.array_equal(prt.find("PhaseSpace").data[:, 0, :], s.pos)
+
+
+@gpu
+def test_hackforce_qp_rmin_options(golden, tmp_path):
+    """hackforce_qp (quadrupole cells), a given rmin= and options=phase."""
+    from oracle import oracle
+
+    src = os.path.join(golden, "hack128.sun4")
+    out = str(tmp_path / "res")
+    r = run("hackforce_qp", src, out, "tol=0.7", "eps=0.02", "rmin=-3,-3,-3", "rsize=6", "options=phase",
+            "mode=strict")
+    assert r.returncode == 0, r.stderr
+    prt = [it for it in _items(out) if it.tag == "SnapShot"][0].find("Particles")
+    assert prt.find("Mass") is None and prt.find("PhaseSpace") is not None
+    s = snapshot.read(src)[0]
+    phi, acc, _, _, _, _ = oracle.force(s.mass, s.pos, tol=0.7, eps=0.02, usequad=True, rsize=6.0,
+                                        rmin=[-3.0, -3.0, -3.0])
+    assert np.array_equal(prt.find("Potential").data, phi)
+    assert np.array_equal(prt.find("Acceleration").data, acc)
+
+
+@gpu
+def test_hackforce_test_file_and_multisnapshot(tmp_path):
+    """test=: masses from the first in= snapshot, one output per test
+    snapshot; without test=, one output per in= snapshot."""
+    from nemo_amd import engine as E
+    from oracle import oracle
+
+    m, x, v = E.mkplummer(3000, seed=5)
+    tp1 = np.random.default_rng(1).normal(size=(500, 3))
+    tp2 = np.random.default_rng(2).normal(size=(700, 3))
+    write_stream(str(tmp_path / "mass"), [(m, x, v, 0.0), (m, x * 1.1, v, 1.0)])
+    write_stream(str(tmp_path / "test"), [(np.zeros(500), tp1, tp1, 0.5), (np.zeros(700), tp2, tp2, 0.75)])
+    r = run("hackforce", "in=mass", "out=o1", "test=test", "mode=strict", "tol=0.6", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    outs = snapshot.read(str(tmp_path / "o1"))
+    assert [len(o.mass) for o in outs] == [500, 700] and [o.time for o in outs] == [0.5, 0.75]
+    for o, tp in zip(outs, (tp1, tp2)):
+        phi, acc, _, _, _, _ = oracle.force(m, x, tol=0.6, eps=0.05, tpos=tp)
+        assert np.array_equal(o.extra["Potential"], phi) and np.array_equal(o.extra["Acceleration"], acc)
+    r = run("hackforce", "in=mass", "out=o2", "mode=strict", "tol=0.6", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    outs = snapshot.read(str(tmp_path / "o2"))
+    assert [o.time for o in outs] == [0.0, 1.0]
+    phi, acc, _, _, _, _ = oracle.force(m, x * 1.1, tol=0.6, eps=0.05)
+    assert np.array_equal(outs[1].extra["Acceleration"], acc)
+
+
+@gpu
+def test_hackforce_fast_and_direct_modes(tmp_path):
+    """The default fast walk is within the tolerance band of the FP64
+    direct sum (mode=direct), itself equal to the oracle's direct sum."""
+    from nemo_amd import engine as E
+    from oracle import oracle
+
+    m, x, v = E.mkplummer(20000, seed=8)
+    write_stream(str(tmp_path / "in"), [(m, x, v, 0.0)])
+    assert run("hackforce_qp", "in", "fast", "tol=0.5", "eps=0.01", cwd=str(tmp_path)).returncode == 0
+    r = run("hackforce", "in", "direct", "eps=0.01", "mode=direct", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    fa = snapshot.read(str(tmp_path / "fast"))[0].extra["Acceleration"]
+    da = snapshot.read(str(tmp_path / "direct"))[0].extra["Acceleration"]
+    idx = np.arange(0, 20000, 97)
+    _, oacc = oracle.direct(m, x, 0.01, idx)
+    np.testing.assert_allclose(da[idx], oacc, rtol=1e-12, atol=1e-12)
+    e = np.linalg.norm(fa - da, axis=1) / np.linalg.norm(da, axis=1)
+    assert np.median(e) < 1e-3 and np.percentile(e, 99) < 1e-2
