@@ -11,8 +11,9 @@
  *                   the same force_calc() split so the snapshot stays resident on the device
  *   bh_direct       directcode hackgrav()              src/nbody/evolve/directcode/grav.c:23-65
  *   bh_lf_*         hackcode1 stepsystem()/diagnostics()  code.c:159-191, code_io.c:351-388
- *   bh_batch_force  many independent hackforce calls in one launch (no reference counterpart:
+ *   bh_batch_*      many independent hackforce calls in one launch (no reference counterpart:
  *                   the reference runs independent processes, src/scripts/nemo.bench:69-90)
+ *   bh_autotune     group size / leaf size chosen by timing (no reference counterpart)
  *   bh_mlp_*        learned far-field correction (no reference counterpart)
  *   bh_mkplummer    mkplummer's sampling recipe     src/nbody/init/mkplummer.c:257-422 (own RNG)
  *
@@ -135,6 +136,36 @@ bh_status bh_lf_drift(bh_ctx *ctx);
 bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, int64_t *n2b, int64_t *nbc);
 /* Copy the current state to the host; any pointer may be NULL. */
 bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *phi, int64_t *nstep);
+
+/* Batched ensembles: nsys independent systems in one tree build and one
+ * persistent walk launch (the reference runs one hackforce process per
+ * system, src/scripts/nemo.bench:69-90).  System s owns bodies
+ * [off[s], off[s+1]) of mass/pos (off[0] = 0, every system non-empty, every
+ * body massive, nsys <= 2^18) and has its own eps[s] and tol[s]; each system
+ * gets its own box (expandbox from p's rsize/rmin) and its own tree, and its
+ * bodies see only its own bodies (self skipped).  p: usequad, group_size,
+ * nleaf, rsize/rmin; p->eps/tol are not used.  Fast mode only.  After
+ * bh_batch_upload, bh_eval evaluates every system and bh_download returns
+ * phi/acc in input order. */
+bh_status bh_batch_upload(bh_ctx *ctx, int nsys, const int64_t *off, const double *mass, const double *pos,
+                          const double *eps, const double *tol);
+bh_status bh_batch_force(bh_ctx *ctx, const bh_params *p, int nsys, const int64_t *off, const double *mass,
+                         const double *pos, const double *eps, const double *tol, double *phi, double *acc,
+                         bh_stats *st);
+/* Leapfrog of a batch: as bh_lf_init, then bh_lf_force / bh_lf_drift /
+ * bh_lf_run / bh_lf_get work on every system at once; every system keeps its
+ * own box; diag holds 36 doubles per system per step ([step][system][36]). */
+bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int64_t *off, const double *mass,
+                           const double *pos, const double *vel, const double *eps, const double *tol, double dt);
+
+/* Autotuner (no reference counterpart): times the fast walk on the uploaded
+ * problem (single or batch) for group_size in {64, 128} x nleaf in
+ * {0, 1, 2, 4, 8}, best of reps evaluations each (device time of tree +
+ * walk), and makes the fastest the default used when group_size = 0 and
+ * nleaf = -1.  ms (may be NULL) receives BH_AUTOTUNE_NCAND times in that
+ * order; best (may be NULL) the chosen parameters. */
+#define BH_AUTOTUNE_NCAND 10
+bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms);
 
 /* Learned far-field correction (no reference counterpart).  A 12-64-64-64-3
  * SiLU MLP on BF16 MFMA predicts, for a cell accepted by the group walk whose

@@ -66,6 +66,8 @@ void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, i
     if (!mass || !pos) throw ApiFail(BH_ERR_ARG, "mass/pos is NULL");
     const bool self = (tpos == nullptr || ntest == 0);
     if (!self && (ntest < 0 || ntest > (int64_t)INT32_MAX / 2)) throw ApiFail(BH_ERR_ARG, "bad ntest");
+    ctx->nsys = 0;
+    ctx->sysbits = 0;
     ctx->nmass = nmass;
     int64_t nb = 0;
     for (int64_t i = 0; i < nmass; i++) nb += mass[i] != 0.0;
@@ -95,9 +97,11 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     check_params(p);
     if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_eval before bh_upload");
     bh_params q = *p;
-    if (q.group_size == 0) q.group_size = 128;
-    if (q.nleaf < 0) q.nleaf = 2;
+    if (q.group_size == 0) q.group_size = ctx->tuned_group;
+    if (q.nleaf < 0) q.nleaf = ctx->tuned_nleaf;
     if (q.mode == BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
+    if (ctx->nsys && (q.mode != BH_MODE_FAST || q.learned))
+        throw ApiFail(BH_ERR_ARG, "batches are evaluated by the fast walk without the learned correction");
     if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
     unsigned long long stat[bh::S_NSLOTS];
     double box[4];
@@ -175,6 +179,44 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     if (st) *st = s;
 }
 
+// Batch upload: per-system offsets, eps and tol; every body massive.
+void batch_upload(bh_ctx *ctx, int nsys, const int64_t *off, const double *mass, const double *pos, const double *eps,
+                  const double *tol)
+{
+    if (nsys < 1 || nsys > (1 << 18)) throw ApiFail(BH_ERR_ARG, "nsys must be in [1, 2^18]");
+    if (!off || !mass || !pos || !eps || !tol) throw ApiFail(BH_ERR_ARG, "NULL batch argument");
+    if (off[0] != 0) throw ApiFail(BH_ERR_ARG, "off[0] must be 0");
+    for (int s = 0; s < nsys; s++) {
+        if (off[s + 1] <= off[s]) throw ApiFail(BH_ERR_ARG, "every system needs at least one body");
+        if (!(eps[s] >= 0.0) || !std::isfinite(eps[s]) || !(tol[s] >= 0.0) || !std::isfinite(tol[s]))
+            throw ApiFail(BH_ERR_ARG, "per-system eps and tol must be finite and >= 0");
+    }
+    const int64_t n = off[nsys];
+    for (int64_t i = 0; i < n; i++)
+        if (!(mass[i] != 0.0)) throw ApiFail(BH_ERR_ARG, "batch bodies must all be massive");
+    upload(ctx, n, mass, pos, 0, nullptr);
+    int bits = 0;
+    while ((1 << bits) < nsys) bits += 3;
+    ctx->sysbits = bits;
+    std::vector<float> sp(2 * (size_t)nsys);
+    bool eps0 = false;
+    for (int s = 0; s < nsys; s++) {
+        sp[2 * s] = (float)(tol[s] * tol[s]);
+        sp[2 * s + 1] = (float)(eps[s] * eps[s]);
+        eps0 = eps0 || eps[s] == 0.0;
+    }
+    int64_t *doff = bh::ensure<int64_t>(ctx->boff, nsys + 1);
+    float *dsp = bh::ensure<float>(ctx->sysp, 2 * (size_t)nsys);
+    BH_HIP(hipMemcpyAsync(doff, off, (nsys + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipMemcpyAsync(dsp, sp.data(), sp.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    bh::ensure<double4>(ctx->sysbox, nsys);
+    bh::ensure<int>(ctx->sysroot, nsys);
+    BH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->nsys = nsys;
+    ctx->sys_eps0 = eps0;
+    ctx->box_valid = false;
+}
+
 void lf_upload_state(bh_ctx *ctx, int64_t n, const double *vel)
 {
     double *dv = bh::ensure<double>(ctx->vel, 3 * n);
@@ -199,8 +241,8 @@ void lf_drop_graph(bh_ctx *ctx)
 bh_params lf_params(const bh_ctx *ctx)
 {
     bh_params q = ctx->lfp;
-    if (q.group_size == 0) q.group_size = 128;
-    if (q.nleaf < 0) q.nleaf = 2;
+    if (q.group_size == 0) q.group_size = ctx->tuned_group;
+    if (q.nleaf < 0) q.nleaf = ctx->tuned_nleaf;
     q.keep_box = ctx->box_valid ? 1 : 0;
     return q;
 }
@@ -449,16 +491,17 @@ bh_status bh_lf_force(bh_ctx *ctx, double *diag, int64_t *n2b, int64_t *nbc)
         bh_params q = lf_params(ctx);
         bh::lf_reset_record(ctx);
         eval(ctx, &q, nullptr);
-        double *dd = bh::ensure<double>(ctx->diag_buf, 36);
+        const int nd = 36 * std::max(1, ctx->nsys);
+        double *dd = bh::ensure<double>(ctx->diag_buf, nd);
         long long *d2 = bh::ensure<long long>(ctx->n2b_buf, 1), *dc = bh::ensure<long long>(ctx->nbc_buf, 1);
         bh::lf_after_force(ctx, dd, d2, dc);
-        double h[36];
+        std::vector<double> h(nd);
         long long c[2];
-        BH_HIP(hipMemcpyAsync(h, dd, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(h.data(), dd, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipMemcpyAsync(&c[0], d2, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipMemcpyAsync(&c[1], dc, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
-        if (diag) memcpy(diag, h, sizeof h);
+        if (diag) memcpy(diag, h.data(), nd * sizeof(double));
         if (n2b) *n2b = c[0];
         if (nbc) *nbc = c[1];
     });
@@ -482,7 +525,8 @@ bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, in
         if (nsteps == 0) return;
         hipStream_t st = ctx->stream;
         const int64_t cap = std::max<int64_t>(nsteps, ctx->graph_cap);
-        double *dd = bh::ensure<double>(ctx->diag_buf, 36 * (size_t)cap);
+        const int64_t nd = 36 * (int64_t)std::max(1, ctx->nsys);
+        double *dd = bh::ensure<double>(ctx->diag_buf, nd * (size_t)cap);
         long long *d2 = bh::ensure<long long>(ctx->n2b_buf, cap), *dc = bh::ensure<long long>(ctx->nbc_buf, cap);
         bh::lf_reset_record(ctx);
         unsigned long long *stat = bh::ptr<unsigned long long>(ctx->counters);
@@ -538,7 +582,7 @@ bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, in
         double box[4];
         BH_HIP(hipMemcpyAsync(h, stat, sizeof h, hipMemcpyDeviceToHost, st));
         BH_HIP(hipMemcpyAsync(box, ctx->dbox.p, sizeof box, hipMemcpyDeviceToHost, st));
-        if (diag) BH_HIP(hipMemcpyAsync(diag, dd, 36 * nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (diag) BH_HIP(hipMemcpyAsync(diag, dd, nd * nsteps * sizeof(double), hipMemcpyDeviceToHost, st));
         if (n2b) BH_HIP(hipMemcpyAsync(n2b, d2, nsteps * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         if (nbc) BH_HIP(hipMemcpyAsync(nbc, dc, nsteps * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         BH_HIP(hipStreamSynchronize(st));
@@ -564,6 +608,83 @@ bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *
         if (acc) BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
         if (phi) BH_HIP(hipMemcpy(phi, ctx->phi.p, n * sizeof(double), hipMemcpyDeviceToHost));
         if (nstep) *nstep = ctx->lf_nstep;
+    });
+}
+
+// ---- batched ensembles and the autotuner ------------------------------------------
+
+bh_status bh_batch_upload(bh_ctx *ctx, int nsys, const int64_t *off, const double *mass, const double *pos,
+                          const double *eps, const double *tol)
+{
+    return guarded(ctx, [&] { batch_upload(ctx, nsys, off, mass, pos, eps, tol); });
+}
+
+bh_status bh_batch_force(bh_ctx *ctx, const bh_params *p, int nsys, const int64_t *off, const double *mass,
+                         const double *pos, const double *eps, const double *tol, double *phi, double *acc,
+                         bh_stats *st)
+{
+    return guarded(ctx, [&] {
+        if (!phi || !acc) throw ApiFail(BH_ERR_ARG, "phi/acc is NULL");
+        batch_upload(ctx, nsys, off, mass, pos, eps, tol);
+        bh_params q = *p;
+        q.keep_box = 0;
+        eval(ctx, &q, st);
+        BH_HIP(hipMemcpy(phi, ctx->phi.p, ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+        BH_HIP(hipMemcpy(acc, ctx->acc.p, 3 * ctx->ntest * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int64_t *off, const double *mass,
+                           const double *pos, const double *vel, const double *eps, const double *tol, double dt)
+{
+    return guarded(ctx, [&] {
+        check_params(p);
+        if (!vel) throw ApiFail(BH_ERR_ARG, "vel is NULL");
+        if (!(dt > 0.0) || !std::isfinite(dt)) throw ApiFail(BH_ERR_ARG, "dt must be finite and > 0");
+        if (p->mode != BH_MODE_FAST || p->learned) throw ApiFail(BH_ERR_ARG, "batches use the fast walk");
+        lf_drop_graph(ctx);
+        batch_upload(ctx, nsys, off, mass, pos, eps, tol);
+        lf_upload_state(ctx, off[nsys], vel);
+        ctx->lfp = *p;
+        ctx->lf_dt = dt;
+        ctx->lf_nstep = 0;
+        ctx->lf_ready = true;
+    });
+}
+
+bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms)
+{
+    return guarded(ctx, [&] {
+        check_params(p);
+        if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_autotune before an upload");
+        if (p->mode != BH_MODE_FAST) throw ApiFail(BH_ERR_ARG, "the autotuner tunes the fast walk");
+        static const int groups[2] = {64, 128}, leaves[5] = {0, 1, 2, 4, 8};
+        double tbest = 1e300;
+        bh_params qbest = *p;
+        int k = 0;
+        for (int g : groups)
+            for (int nl : leaves) {
+                bh_params q = *p;
+                q.group_size = g;
+                q.nleaf = nl;
+                q.keep_box = 0;
+                bh_stats st{};
+                eval(ctx, &q, &st);      // warm-up; capacities settle here
+                double t = 1e300;
+                for (int r = 0; r < std::max(1, reps); r++) {
+                    eval(ctx, &q, &st);
+                    t = std::min(t, st.t_tree_ms + st.t_walk_ms);
+                }
+                if (ms) ms[k] = t;
+                k++;
+                if (t < tbest) {
+                    tbest = t;
+                    qbest = q;
+                }
+            }
+        ctx->tuned_group = qbest.group_size;
+        ctx->tuned_nleaf = qbest.nleaf;
+        if (best) *best = qbest;
     });
 }
 
@@ -662,8 +783,8 @@ bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo
         ctx->mlp.norm[1] = (float)rlo;
         ctx->mlp.norm[2] = (float)rhi;
         bh_params q = *p;
-        if (q.group_size == 0) q.group_size = 128;
-        if (q.nleaf < 0) q.nleaf = 2;
+        if (q.group_size == 0) q.group_size = ctx->tuned_group;
+        if (q.nleaf < 0) q.nleaf = ctx->tuned_nleaf;
         q.mode = BH_MODE_FAST;
         q.learned = 0;
         if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);

@@ -78,9 +78,19 @@ struct bh_ctx {
     int64_t lcap = 0;            // ints per wave region
     int lnwave = 0;              // waves (regions) of the last recording walk
     bh::DevBuf cF0, cF1;         // per-cell learned features
+    // batched ensemble (nsys > 0): system s owns bodies [boff[s], boff[s+1])
+    // of the mass set, in input order and in sorted order (its id is the top
+    // sysbits bits of the Morton key); per-system box, (tol^2, eps^2) and the
+    // node the walk of its groups starts from
+    int nsys = 0;
+    int sysbits = 0;
+    bool sys_eps0 = false;      // some system has eps = 0
+    bh::DevBuf boff, sysbox, sysp, sysroot, sysdiag;
     bh::Tree tree;
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays
+    int tuned_group = 128;      // defaults for group_size = 0 / nleaf = -1 (set by bh_autotune)
+    int tuned_nleaf = 2;
     int tcell_cap = 0;          // cell capacity of a separate test-set topology
     int box_rounds = 6;         // expansion rounds from a fresh box
     int box_rounds_keep = 2;    // expansion rounds from the previous box

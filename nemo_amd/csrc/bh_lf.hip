@@ -129,6 +129,49 @@ __global__ void k_diag_final(const double *partial, int nblk, double *out, int *
     }
 }
 
+// Batches: the 36 diagnostics of every system (one block per system, the
+// same per-thread order and fixed tree reduction as above), written to
+// out[(*rec * nsys + s) * 36].
+__global__ __launch_bounds__(kTpb) void k_diag_sys(const int64_t *__restrict__ off, int nsys, const double *mass,
+                                                   const double *pos, const double *vel, const double *acc,
+                                                   const double *phi, double *out, const int *rec)
+{
+    __shared__ double red[kTpb];
+    __shared__ double s[36];
+    const int sy = blockIdx.x;
+    double acc36[36];
+    for (int k = 0; k < 36; k++) acc36[k] = 0.0;
+    for (int64_t i = off[sy] + threadIdx.x; i < off[sy + 1]; i += kTpb) {
+        double t[36];
+        body_terms(pos + 3 * i, vel + 3 * i, acc + 3 * i, mass[i], phi[i], t);
+        for (int k = 0; k < 36; k++) acc36[k] += t[k];
+    }
+    for (int k = 0; k < 36; k++) {
+        red[threadIdx.x] = acc36[k];
+        __syncthreads();
+        for (int h = kTpb / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) s[k] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *o = out + 36 * ((int64_t)(*rec) * nsys + sy);
+        const double mtot = s[0];
+        o[0] = s[1] + s[2];
+        o[1] = s[1];
+        o[2] = s[2];
+        for (int j = 0; j < 9; j++) {
+            o[3 + j] = s[3 + j];
+            o[12 + j] = s[12 + j];
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) o[21 + 3 * i + j] = s[21 + 3 * i + j] - s[21 + 3 * j + i];
+        for (int c = 0; c < 6; c++) o[30 + c] = s[30 + c] / mtot;
+    }
+}
+
 // Per-step interaction counts (the walk's counters of this step).
 __global__ void k_lf_record(unsigned long long *stat, const int *rec, long long *n2b, long long *nbc)
 {
@@ -174,12 +217,19 @@ void lf_after_force(bh_ctx *ctx, double *diag_out, long long *n2b_out, long long
     int *nstep = ptr<int>(ctx->lfstep), *rec = nstep + 1;
     k_lf_kick<<<grid_for(n), kTpb, 0, st>>>(n, ptr<double>(ctx->vel), ptr<double>(ctx->acc), ptr<double>(ctx->acc_old),
                                             nstep, 0.5 * ctx->lf_dt);
-    const int nblk = (int)std::min<int64_t>(kDiagBlocks, std::max<int64_t>(1, (n + kTpb - 1) / kTpb));
-    const int64_t chunk = (n + nblk - 1) / nblk;
-    double *partial = ensure<double>(ctx->diag_partial, 36 * (size_t)kDiagBlocks);
-    k_diag_partial<<<nblk, kTpb, 0, st>>>(n, chunk, ptr<double>(ctx->mass), ptr<double>(ctx->pos), ptr<double>(ctx->vel),
-                                          ptr<double>(ctx->acc), ptr<double>(ctx->phi), partial);
-    k_diag_final<<<1, 64, 0, st>>>(partial, nblk, diag_out, rec);
+    if (ctx->nsys) {
+        k_diag_sys<<<ctx->nsys, kTpb, 0, st>>>(ptr<int64_t>(ctx->boff), ctx->nsys, ptr<double>(ctx->mass),
+                                               ptr<double>(ctx->pos), ptr<double>(ctx->vel), ptr<double>(ctx->acc),
+                                               ptr<double>(ctx->phi), diag_out, rec);
+    } else {
+        const int nblk = (int)std::min<int64_t>(kDiagBlocks, std::max<int64_t>(1, (n + kTpb - 1) / kTpb));
+        const int64_t chunk = (n + nblk - 1) / nblk;
+        double *partial = ensure<double>(ctx->diag_partial, 36 * (size_t)kDiagBlocks);
+        k_diag_partial<<<nblk, kTpb, 0, st>>>(n, chunk, ptr<double>(ctx->mass), ptr<double>(ctx->pos),
+                                              ptr<double>(ctx->vel), ptr<double>(ctx->acc), ptr<double>(ctx->phi),
+                                              partial);
+        k_diag_final<<<1, 64, 0, st>>>(partial, nblk, diag_out, rec);
+    }
     k_lf_record<<<1, 1, 0, st>>>(ptr<unsigned long long>(ctx->counters), rec, n2b_out, nbc_out);
     BH_HIP(hipGetLastError());
 }

@@ -168,6 +168,65 @@ __global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *
     boxi[1] = first + 1;
 }
 
+// ---- batched ensembles: one box per system -----------------------------------
+
+// expandbox for every system of a batch (one block each): from the caller's
+// box, or the system's previous box, take the first body in input order that
+// lies outside, double the box towards it (load.c:68-95), and go on from the
+// next body; every body of a batch is massive.  All threads compute the same
+// box.  At most `rounds` expansions; a box still open is flagged.
+__global__ __launch_bounds__(kTpb) void k_sys_box(const double *__restrict__ pos, const int64_t *__restrict__ off,
+                                                  double4 *sysbox, double4 *box0, double rx, double ry, double rz,
+                                                  double rsize, int keep, int rounds, unsigned long long *stat)
+{
+    __shared__ int red[kTpb / 64];
+    const int s = blockIdx.x;
+    const int lo = (int)off[s], hi = (int)off[s + 1];
+    double4 b = keep ? sysbox[s] : make_double4(rx, ry, rz, rsize);
+    int start = lo;
+    bool closed = false;
+    for (int round = 0; round <= rounds; round++) {
+        int first = hi;
+        for (int i = start + (int)threadIdx.x; i < hi; i += blockDim.x)
+            if (!fits(pos + 3 * (int64_t)i, b)) {
+                first = i;
+                break;
+            }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) first = min(first, __shfl_xor(first, d, 64));
+        if (lane_id() == 0) red[threadIdx.x >> 6] = first;
+        __syncthreads();
+        int f = red[0];
+#pragma unroll
+        for (int k = 1; k < kTpb / 64; k++) f = min(f, red[k]);
+        __syncthreads();
+        if (f == hi) {
+            closed = true;
+            break;
+        }
+        if (round == rounds) break;
+        const double *x = pos + 3 * (int64_t)f;
+        if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) {
+            if (threadIdx.x == 0) atomicOr(&stat[S_FLAGS], (unsigned long long)F_NONFINITE);
+            closed = true;
+            break;
+        }
+        for (int guard = 0; guard < 2000 && !fits(x, b); guard++) {
+            double rmid0 = b.x + 0.5 * b.w, rmid1 = b.y + 0.5 * b.w, rmid2 = b.z + 0.5 * b.w;
+            if (x[0] < rmid0) b.x -= b.w;
+            if (x[1] < rmid1) b.y -= b.w;
+            if (x[2] < rmid2) b.z -= b.w;
+            b.w = 2.0 * b.w;
+        }
+        start = f + 1;
+    }
+    if (threadIdx.x == 0) {
+        sysbox[s] = b;
+        if (s == 0) *box0 = b;
+        if (!closed) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
+    }
+}
+
 // ---- keys ------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v)
@@ -181,7 +240,8 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v)
     return v;
 }
 
-__device__ __forceinline__ uint64_t morton_of(const double *x, const double4 b)
+// Interleaved top `levels` bits of the three 30-bit integer coordinates.
+__device__ __forceinline__ uint64_t morton_of(const double *x, const double4 b, int levels = kMaxLevel)
 {
     uint64_t c[3];
     const double rm[3] = {b.x, b.y, b.z};
@@ -192,9 +252,28 @@ __device__ __forceinline__ uint64_t morton_of(const double *x, const double4 b)
         if (!(xsc >= 0.0)) xi = 0;
         else if (!(xsc < 1.0)) xi = (1 << 30) - 1;
         else xi = (int64_t)floor(1073741824.0 * xsc);
-        c[k] = (uint64_t)(xi >> 9);
+        c[k] = (uint64_t)(xi >> (30 - levels));
     }
     return (spread3(c[0]) << 2) | (spread3(c[1]) << 1) | spread3(c[2]);
+}
+
+// Batch keys: [system id : sysbits][Morton code of kMaxLevel - sysbits/3
+// levels in the system's own box]; the system digits form the top levels.
+__global__ void k_keys_batch(const double *__restrict__ pos, int64_t n, const int64_t *__restrict__ off, int nsys,
+                             const double4 *__restrict__ sysbox, int sysbits, uint64_t *keys, uint32_t *vals)
+{
+    const int levels = kMaxLevel - sysbits / 3;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nsys;          // last system with off[s] <= i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t m = morton_of(pos + 3 * i, sysbox[lo], levels);
+        keys[i] = ((uint64_t)lo << (3 * levels)) | m;
+        vals[i] = (uint32_t)i;
+    }
 }
 
 // Mass set: massless bodies get bit 63 so they sort after every massive body.
@@ -525,11 +604,10 @@ __device__ __forceinline__ unsigned octant_box(unsigned m)
 template <bool WRITE>
 __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                                 const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
-                                const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int G, int *gfirst,
-                                int *gcount, uint32_t off)
+                                const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int G, int lsys,
+                                int *gfirst, int *gcount, uint32_t off)
 {
     const int s = cfirst[c], e = clast[c], L = clevel[c];
-    if (e - s + 1 <= G) return 0;
     uint32_t k = 0;
     auto emit = [&](int start, int count) {
         if (WRITE) {
@@ -538,6 +616,20 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
         }
         k++;
     };
+    if (L < lsys) {
+        // a level of system digits (batches): every child that is a whole
+        // system (a system-root cell or a one-body system) with at most G
+        // points is a group of its own, never merged with another system
+        for (int t = s; t <= e;) {
+            const bool is_cell = cpl[t] >= L + 1;
+            const int cc = (t == s) ? c + 1 : (int)cbase[t];
+            const int tn = is_cell ? clast[cc] + 1 : t + 1;
+            if (!(is_cell && L + 1 < lsys) && tn - t <= G) emit(t, tn - t);
+            t = tn;
+        }
+        return k;
+    }
+    if (e - s + 1 <= G) return 0;
     if (L == kMaxLevel) {
         for (int t = s; t <= e; t += G) emit(t, min(G, e - t + 1));
         return k;
@@ -586,7 +678,7 @@ __global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *_
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
                               const uint8_t *__restrict__ clevel, int cap, const unsigned long long *ncellp, int G,
-                              uint32_t *gcnt)
+                              int lsys, uint32_t *gcnt)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cap) return;
@@ -594,18 +686,35 @@ __global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *_
         gcnt[c] = 0;
         return;
     }
-    gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, nullptr, nullptr, 0);
+    gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, nullptr, nullptr, 0);
 }
 
 __global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
-                              const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G,
+                              const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G, int lsys,
                               const uint32_t *__restrict__ goff, int *gfirst, int *gcount)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (int)*ncellp) return;
-    pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, gfirst, gcount, goff[c]);
+    pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, gfirst, gcount, goff[c]);
+}
+
+// Walk roots of a batch: the system's root cell (the cell at the first
+// spatial level, lsys) or, for a one-body system, that body.
+__global__ void k_sysroot_init(const int64_t *__restrict__ off, int nsys, int *root)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) root[s] = (int)off[s];
+}
+__global__ void k_sysroot_cells(const uint8_t *__restrict__ clevel, const int *__restrict__ cfirst,
+                                const uint64_t *__restrict__ keys, int lsys, int sysbits,
+                                const unsigned long long *ncellp, int *root)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (int)*ncellp || clevel[c] != lsys) return;
+    const int s = sysbits ? (int)(keys[cfirst[c]] >> (63 - sysbits)) : 0;
+    root[s] = ~c;
 }
 
 // Children of each cell in octant order; parent links; child-cell counts.
@@ -658,10 +767,20 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
                              const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
                              const double4 *__restrict__ bd, double *mo,
                              float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad,
-                             const uint64_t *__restrict__ keys, double4 box, float4 *cF0, float4 *cF1)
+                             const uint64_t *__restrict__ keys, double4 box, float4 *cF0, float4 *cF1, int lsys,
+                             int sysbits, const double4 *__restrict__ sysbox)
 {
-    // rsize: side of the root box (the cell at level L has side rsize / 2^L)
+    // rsize: side of the root box (the cell at level L has side rsize / 2^L).
+    // Batches: the box of the cell's system, levels counted from its root
+    // (lsys); the cells above the system roots get an infinite size.
     const int L = clevel[c];
+    int Lg = L;
+    if (sysbox) {
+        const int sys = sysbits ? (int)(keys[cfirst[c]] >> (63 - sysbits)) : 0;
+        box = sysbox[sys];
+        rsize = box.w;
+        Lg = L - lsys;
+    }
     double M = 0.0, px = 0.0, py = 0.0, pz = 0.0;
     double q[6] = {0, 0, 0, 0, 0, 0};
     if (L == kMaxLevel) {
@@ -767,22 +886,22 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     split2(py, A.y, B.y);
     split2(pz, A.z, B.z);
     A.w = (float)M;
-    B.w = (float)(rsize * rsize * ldexp(1.0, -2 * L));
+    B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
     cA[c] = A;
     cB[c] = B;
     cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
     cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
-    if (cF0) {
+    if (cF0 && Lg >= 0) {
         // geometric centre of the cell from the key prefix of its first body
         const uint64_t key = keys[cfirst[c]];
         uint64_t ix = 0, iy = 0, iz = 0;
-        for (int l = 0; l < L; l++) {
+        for (int l = L - Lg; l < L; l++) {
             const int sh = 60 - 3 * l;
             ix = (ix << 1) | ((key >> (sh + 2)) & 1);
             iy = (iy << 1) | ((key >> (sh + 1)) & 1);
             iz = (iz << 1) | ((key >> sh) & 1);
         }
-        const double size = rsize * ldexp(1.0, -L);
+        const double size = rsize * ldexp(1.0, -Lg);
         const double gx = box.x + ((double)ix + 0.5) * size, gy = box.y + ((double)iy + 0.5) * size,
                      gz = box.z + ((double)iz + 0.5) * size;
         const double qn = 1.0 / (M * size * size);
@@ -818,14 +937,15 @@ __global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__
                                 const int *__restrict__ cfirst, const int *__restrict__ clast,
                                 const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *mo,
                                 float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
-                                const double4 *__restrict__ box, int usequad)
+                                const double4 *__restrict__ box, int usequad, int lsys, int sysbits,
+                                const double4 *__restrict__ sysbox)
 {
     const double rsize = box->w;
     const double4 bx = *box;
     const uint32_t lo = loff[L], hi = loff[L + 1];
     for (uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x; k < hi; k += gridDim.x * blockDim.x)
         cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad, keys, bx, cF0,
-                     cF1);
+                     cF1, lsys, sysbits, sysbox);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -912,7 +1032,7 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
 
 // Groups of the sorted test set for the fast walk (ctx->gfirst/gcount, count
 // in stat[S_NGROUPS]).
-static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G)
+static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G, int lsys = 0)
 {
     hipStream_t st = ctx->stream;
     unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
@@ -925,10 +1045,10 @@ static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_
     uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->cap + 1);
     uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->cap + 1);
     k_group_count<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->cap,
-                                                    t->ncellp, G, gcnt);
+                                                    t->ncellp, G, lsys, gcnt);
     exclusive_scan_u32(ctx, gcnt, goff, t->cap);
     k_group_write<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncellp, G,
-                                                    goff, gf, gc);
+                                                    lsys, goff, gf, gc);
     k_copy_count<<<1, 1, 0, st>>>(goff + t->cap, stat + S_NGROUPS);
     BH_HIP(hipGetLastError());
 }
@@ -954,9 +1074,10 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo,
     const int64_t nt = ctx->ntest;
     const Tree &T = ctx->tree;
     if (ctx->self_test && nt == T.nb && tree_topo) {
-        make_groups(ctx, tree_topo, ks, nt, G);
+        make_groups(ctx, tree_topo, ks, nt, G, ctx->sysbits / 3);
         return;
     }
+    if (ctx->nsys > 1) throw ApiFail(BH_ERR_STATE, "batch groups need the system-aware topology");
     if (nt <= G || nt < 2) {
         make_groups(ctx, nullptr, nullptr, nt, G);
         return;
@@ -988,6 +1109,13 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const double r0 = p.rmin_given ? p.rmin[0] : -p.rsize / 2.0;
     const double r1 = p.rmin_given ? p.rmin[1] : -p.rsize / 2.0;
     const double r2 = p.rmin_given ? p.rmin[2] : -p.rsize / 2.0;
+    const int nsys = ctx->nsys;
+    const int64_t *sysoff = nsys ? ptr<int64_t>(ctx->boff) : nullptr;
+    double4 *sysbox = nsys ? ensure<double4>(ctx->sysbox, nsys) : nullptr;
+    if (nsys) {
+        k_sys_box<<<nsys, kTpb, 0, st>>>(d_pos, sysoff, sysbox, d_box, r0, r1, r2, p.rsize, keep ? 1 : 0,
+                                         16 * ctx->box_rounds, stat);
+    } else {
     k_box_init<<<1, 1, 0, st>>>(d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
     const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
     int *cand = ensure<int>(ctx->bcand, n);
@@ -1005,12 +1133,18 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     }
     k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
     k_box_check<<<1, 1, 0, st>>>(d_boxi, stat);
+    }
 
     // -- keys and sort of the mass set (massless bodies sort last)
     uint64_t *k0 = ensure<uint64_t>(ctx->keys0, n), *k1 = ensure<uint64_t>(ctx->keys1, n);
     uint32_t *v0 = ensure<uint32_t>(ctx->vals0, n), *v1 = ensure<uint32_t>(ctx->vals1, n);
     const int grid_n = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 16);
-    k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0);
+    if (nsys) {
+        k_keys_batch<<<grid_n, kTpb, 0, st>>>(d_pos, n, sysoff, nsys, sysbox, ctx->sysbits, k0, v0);
+        k_sysroot_init<<<nblocks(nsys), kTpb, 0, st>>>(sysoff, nsys, ensure<int>(ctx->sysroot, nsys));
+    } else {
+        k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0);
+    }
     uint64_t *ks;
     uint32_t *vs;
     if (timing) BH_HIP(hipEventRecord(ctx->ev[6], st));
@@ -1108,9 +1242,13 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     k_level_offsets<<<1, 64, 0, st>>>(ptr<uint32_t>(ctx->hist), lnblk, loffd);
     const int *llist = (const int *)lvs;
     const int mgrid = ctx->ncu * 4;
+    const int lsys = ctx->sysbits / 3;
     for (int L = kMaxLevel; L >= 0; L--)
         k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, cF0, cF1, csub, cfirst, clast, clevel, bd, mo, cA,
-                                                cB, cQ0, cQ1, d_box, p.usequad);
+                                                cB, cQ0, cQ1, d_box, p.usequad, lsys, ctx->sysbits, sysbox);
+    if (nsys)
+        k_sysroot_cells<<<nblocks(cap), kTpb, 0, st>>>(clevel, cfirst, ks, lsys, ctx->sysbits, tt.ncellp,
+                                                       ptr<int>(ctx->sysroot));
     BH_HIP(hipGetLastError());
     T.first = cfirst;
     T.last = clast;

@@ -208,6 +208,11 @@ struct WalkArgs {
     float lr_lo2, lr_hi2;
     int *lrec;
     int lcap;
+    // batches: system offsets, per-system (tol^2, eps^2) and walk roots
+    int nsys;
+    const int64_t *boff;
+    const float2 *sysp;
+    const int *sysroot;
 };
 
 template <int NPL, bool QUAD, bool SOFT0, bool LREC>
@@ -232,6 +237,21 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         if (g >= (int)a.stat[S_NGROUPS]) break;
         const int64_t gbase = a.gfirst[g];
         const int gcount = a.gcount[g];
+        // opening and softening of this group's system
+        float tolsq = a.tolsq, eps2 = a.eps2;
+        int sys = -1;
+        if (a.nsys) {
+            int lo = 0, hi = a.nsys;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (a.boff[mid] <= gbase) lo = mid;
+                else hi = mid;
+            }
+            sys = lo;
+            const float2 pp = a.sysp[sys];
+            tolsq = pp.x;
+            eps2 = pp.y;
+        }
 
         // particles of this lane; inactive slots copy the first particle
         float4 h[NPL], l[NPL];
@@ -285,7 +305,10 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 lreg[lhead + 4] = __float_as_int(cz);
             }
         }
-        if (*a.ncellp == 0) {
+        if (sys >= 0) {
+            if (lane == 0) fr0[0] = a.sysroot[sys];
+            nf = 1;
+        } else if (*a.ncellp == 0) {
             if (a.nb == 1 && lane == 0) fr0[0] = 0;
             nf = a.nb == 1 ? 1 : 0;
         } else {
@@ -319,7 +342,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     const int2 fn = a.fnb[c];
                     const bool small = fn.y <= a.nleaf || a.level[c] == kMaxLevel;
-                    open = a.tolsq * d2 < B.w || fn.y <= a.nleaf;
+                    open = tolsq * d2 < B.w || fn.y <= a.nleaf;
                     accept = !open;
                     if (open) nk = small ? fn.y : a.nkids[c];
                     bfirst = small ? fn.x : -1;
@@ -390,7 +413,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 }
                 wave_lds_sync();
                 while (np >= 64) {
-                    eval_p2p<NPL, SOFT0>(L, P, 64, a.eps2);
+                    eval_p2p<NPL, SOFT0>(L, P, 64, eps2);
                     tot_pp += 64;
                     np -= 64;
                     wave_lds_sync();
@@ -400,7 +423,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     wave_lds_sync();
                 }
                 while (nc >= 64) {
-                    eval_p2c<NPL, QUAD, SOFT0>(L, Cl, 64, a.eps2);
+                    eval_p2c<NPL, QUAD, SOFT0>(L, Cl, 64, eps2);
                     tot_pc += 64;
                     nc -= 64;
                     wave_lds_sync();
@@ -432,8 +455,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             continue;
         }
         wave_lds_sync();
-        if (np > 0) eval_p2p<NPL, SOFT0>(L, P, np, a.eps2);
-        if (nc > 0) eval_p2c<NPL, QUAD, SOFT0>(L, Cl, nc, a.eps2);
+        if (np > 0) eval_p2p<NPL, SOFT0>(L, P, np, eps2);
+        if (nc > 0) eval_p2c<NPL, QUAD, SOFT0>(L, Cl, nc, eps2);
         tot_pp += np;
         tot_pc += nc;
         wave_lds_sync();
@@ -453,7 +476,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             double ph = (double)L.ph[q] + (double)L.pl[q];
             if (sSelf[w][k]) {
                 nself++;
-                if (!SOFT0) ph -= (double)(a.bhi[gbase + k].w * rsq(a.eps2));
+                if (!SOFT0) ph -= (double)(a.bhi[gbase + k].w * rsq(eps2));
             }
             const int64_t o = a.tperm[gbase + k];
             a.phi[o] = -ph;
@@ -535,6 +558,12 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     a.lstride = std::max(1, lstride);
     a.lr_lo2 = (float)(ctx->mlp.norm[1] * ctx->mlp.norm[1]);
     a.lr_hi2 = (float)(ctx->mlp.norm[2] * ctx->mlp.norm[2]);
+    a.nsys = ctx->nsys;
+    if (ctx->nsys) {
+        a.boff = ptr<int64_t>(ctx->boff);
+        a.sysp = ptr<float2>(ctx->sysp);
+        a.sysroot = ptr<int>(ctx->sysroot);
+    }
     // persistent grid sized for the device; each wave loops over groups
     const int64_t max_groups = (nt + 1) / 1;
     int grid = (int)std::min<int64_t>((max_groups + kWaves - 1) / kWaves, (int64_t)ctx->ncu * 8);
@@ -555,7 +584,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
         a.lrec = ptr<int>(ctx->gctr);
     }
     ctx->lnwave = nwave;
-    const bool soft0 = p.eps == 0.0;
+    const bool soft0 = ctx->nsys ? ctx->sys_eps0 : p.eps == 0.0;
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     if (npl == 2) {
         if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);

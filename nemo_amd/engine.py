@@ -89,7 +89,13 @@ SIGNATURES = {
     "bh_mlp_samples": (C.c_int, [_vp, C.c_int64, _f, _f, _i64]),
     "bh_mlp_clear_samples": (C.c_int, [_vp]),
     "bh_mlp_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _f]),
+    "bh_batch_upload": (C.c_int, [_vp, C.c_int, _i64, _d, _d, _d, _d]),
+    "bh_batch_force": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _i64, _d, _d, _d, _d, _d, _d, C.POINTER(Stats)]),
+    "bh_batch_lf_init": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _i64, _d, _d, _d, _d, _d, C.c_double]),
+    "bh_autotune": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.POINTER(Params), _d]),
 }
+
+AUTOTUNE_CANDIDATES = [(g, nl) for g in (64, 128) for nl in (0, 1, 2, 4, 8)]
 
 
 def lib():
@@ -240,6 +246,27 @@ class Engine:
         return f, c
 
     # -- learned correction -------------------------------------------------
+    # ---- batched ensembles and the autotuner ----
+    def batch_upload(self, off, mass, pos, eps, tol):
+        """Systems s = 0..len(off)-2 own bodies [off[s], off[s+1])."""
+        off = np.ascontiguousarray(off, np.int64)
+        mass = np.ascontiguousarray(mass, np.float64)
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        eps = np.ascontiguousarray(eps, np.float64)
+        tol = np.ascontiguousarray(tol, np.float64)
+        nsys = len(off) - 1
+        if len(eps) != nsys or len(tol) != nsys or off[-1] != len(mass):
+            raise ValueError("inconsistent batch arrays")
+        self._check(lib().bh_batch_upload(self._h, nsys, _p(off, _i64), _p(mass), _p(pos), _p(eps), _p(tol)))
+        self.ntest = len(mass)
+
+    def autotune(self, params: Params, reps=3):
+        """Returns (best Params, ms per candidate in AUTOTUNE_CANDIDATES order)."""
+        best = Params()
+        ms = np.zeros(len(AUTOTUNE_CANDIDATES))
+        self._check(lib().bh_autotune(self._h, C.byref(params), reps, C.byref(best), _p(ms)))
+        return best, ms
+
     def mlp_init(self, seed=1):
         self._check(lib().bh_mlp_init(self._h, seed))
 
@@ -304,7 +331,9 @@ class Leapfrog:
     stepper interface of nemo_amd.hackcode1.run: force(), diagnostics(),
     drift(), and mass/pos/vel (downloaded on access)."""
 
-    def __init__(self, engine: Engine, mass, pos, vel, dt, **params):
+    def __init__(self, engine: Engine, mass, pos, vel, dt, batch=None, **params):
+        """batch = (off, eps, tol) integrates an ensemble of systems
+        (bh_batch_lf_init); diagnostics then have one row of 36 per system."""
         params.setdefault("rsize", 4.0)          # hackcode1 box: rmin=-2, rsize=4 (code.c:119-120)
         self.engine = engine
         self.mass = np.ascontiguousarray(mass, np.float64)
@@ -313,11 +342,23 @@ class Leapfrog:
         self.n = len(self.mass)
         self.dt = dt
         self.params = default_params(**params)
-        engine._check(lib().bh_lf_init(engine._h, C.byref(self.params), self.n, _p(self.mass), _p(pos), _p(vel), dt))
+        if batch is None:
+            self.nsys = 0
+            engine._check(lib().bh_lf_init(engine._h, C.byref(self.params), self.n, _p(self.mass), _p(pos), _p(vel),
+                                           dt))
+        else:
+            off, eps, tol = (np.ascontiguousarray(a, t) for a, t in zip(batch, (np.int64, np.float64, np.float64)))
+            self.nsys = len(off) - 1
+            engine._check(lib().bh_batch_lf_init(engine._h, C.byref(self.params), self.nsys, _p(off, _i64),
+                                                 _p(self.mass), _p(pos), _p(vel), _p(eps), _p(tol), dt))
         self._diag = None
 
+    def _dshape(self, nsteps=None):
+        s = (36,) if self.nsys == 0 else (self.nsys, 36)
+        return s if nsteps is None else (nsteps,) + s
+
     def force(self):
-        d = np.zeros(36)
+        d = np.zeros(self._dshape())
         a = np.zeros(1, np.int64)
         b = np.zeros(1, np.int64)
         self.engine._check(lib().bh_lf_force(self.engine._h, _p(d), _p(a, _i64), _p(b, _i64)))
@@ -331,8 +372,9 @@ class Leapfrog:
         self.engine._check(lib().bh_lf_drift(self.engine._h))
 
     def run(self, nsteps, graph=True):
-        """nsteps full steps; returns (diag[nsteps,36], n2b[nsteps], nbc[nsteps])."""
-        d = np.zeros((nsteps, 36))
+        """nsteps full steps; returns (diag[nsteps,36] (batches:
+        [nsteps,nsys,36]), n2b[nsteps], nbc[nsteps])."""
+        d = np.zeros(self._dshape(nsteps))
         a = np.zeros(nsteps, np.int64)
         b = np.zeros(nsteps, np.int64)
         self.engine._check(lib().bh_lf_run(self.engine._h, nsteps, int(bool(graph)), _p(d), _p(a, _i64), _p(b, _i64)))
