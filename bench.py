@@ -136,8 +136,141 @@ def train_learned(eng, E, n, eps, seeds=(41, 43), tols=(0.8, 1.0, 1.2), stride=3
             "t_collect_s": t1 - t0, "t_train_s": t2 - t1}
 
 
+def walk_roofline(n_int, t_walk_s, kernel):
+    achieved = FLOP_PER_INTERACTION * n_int / t_walk_s / 1e12
+    return {"bound": "mfma", "compute_unit": "FP32 vector ALU (the walk issues no MFMA)", "kernel": kernel,
+            "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "flop_convention": "20 flop per interaction (P2P and P2C), interactions counted by the kernel"}
+
+
+def run_ensemble(args, E, log):
+    """512 independent Plummer systems, N log-uniform in [256, 65536]
+    (seeded), eps_i = 1/(4 N_i^(1/3)), tol 0.7, quadrupole cells: one tree
+    build + one walk launch for all systems (group/leaf size from the
+    autotuner), then 32 graph-captured leapfrog steps at dt = 1/32."""
+    rng = np.random.default_rng(args.seed)
+    nsys = args.nsys
+    ns = np.round(np.exp(rng.uniform(np.log(256), np.log(65536), nsys))).astype(np.int64)
+    ms, xs, vs = [], [], []
+    for s, n in enumerate(ns):
+        m, x, v = E.mkplummer(int(n), seed=args.seed * 100003 + s)
+        ms.append(m)
+        xs.append(x)
+        vs.append(v)
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    eps = 1.0 / (4.0 * ns ** (1.0 / 3.0))
+    tol = np.full(nsys, 0.7)
+    m, x, v = np.concatenate(ms), np.concatenate(xs), np.concatenate(vs)
+    ntot = int(off[-1])
+    eng = E.Engine(0)
+    # FP64 direct sums per system on the device (256 seeded bodies each), before the batch upload
+    ref = []
+    for s in range(nsys):
+        a, b = int(off[s]), int(off[s + 1])
+        idx = np.sort(np.random.default_rng(s).permutation(b - a)[: min(256, b - a)])
+        eng.upload(m[a:b], x[a:b])
+        ref.append((idx, eng.direct(float(eps[s]), idx)[1]))
+    eng.batch_upload(off, m, x, eps, tol)
+    base = E.default_params(usequad=1)
+    t0 = time.perf_counter()
+    best, ms_c = eng.autotune(base, reps=2)
+    t_tune = time.perf_counter() - t0
+    log("autotune: group %d nleaf %d (%.1f s; candidates %s ms)" % (best.group_size, best.nleaf, t_tune,
+                                                                   np.round(ms_c, 3).tolist()))
+    p = E.default_params(usequad=1)          # tuned defaults
+    sec, stats = time_steps(eng, p, args.steps, args.warmup)
+    _, acc = eng.download()
+    med = []
+    p99 = []
+    for s in range(nsys):
+        a = int(off[s])
+        idx, da = ref[s]
+        e = np.linalg.norm(acc[a + idx] - da, axis=1) / np.linalg.norm(da, axis=1)
+        med.append(np.median(e))
+        p99.append(np.percentile(e, 99))
+    last = stats[-1]
+    n_int = int(last.n2btot + last.nbctot)
+    t_walk = float(np.mean([st.t_walk_kernel_ms for st in stats])) * 1e-3
+    # leapfrog: one eager step, then graph replays
+    lf = E.Leapfrog(eng, m, x, v, 1.0 / 32.0, batch=(off, eps, tol), usequad=1)
+    lf.run(1, graph=True)
+    t0 = time.perf_counter()
+    d, n2b, nbc = lf.run(args.lf_steps, graph=True)
+    t_lf = time.perf_counter() - t0
+    e0 = d[0, :, 0]
+    drift = np.abs(d[-1, :, 0] - e0) / np.abs(e0)
+    eng.close()
+    return {
+        "metric": "aggregate particle-force evals/s, 512-system Plummer ensemble in one launch",
+        "value": ntot / sec, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic Plummer ensemble (own counter RNG, seed %d)" % args.seed,
+        "config": {"workload": "ensemble_%d_plummer_logU_256_65536" % nsys, "nsys": nsys, "bodies": ntot,
+                   "n_min": int(ns.min()), "n_max": int(ns.max()), "tol": 0.7, "usequad": 1,
+                   "eps": "1/(4 N^(1/3)) per system", "tuned_group": best.group_size, "tuned_nleaf": best.nleaf,
+                   "autotune_ms": ms_c.tolist(), "interactions_per_s": n_int / sec,
+                   "interactions_per_step": n_int, "t_tree_ms": float(np.mean([st.t_tree_ms for st in stats])),
+                   "t_walk_kernel_ms": t_walk * 1e3,
+                   "median_rel_err": {"median_over_systems": float(np.median(med)), "max": float(np.max(med))},
+                   "p99_rel_err": {"median_over_systems": float(np.median(p99)), "max": float(np.max(p99))},
+                   "leapfrog": {"steps": args.lf_steps, "dt": 1.0 / 32.0, "graph": True,
+                                "ms_per_step": t_lf / args.lf_steps * 1e3,
+                                "interactions_per_s": float((n2b + nbc).sum() / t_lf),
+                                "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}},
+        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false>" % (2 if best.group_size == 128 else 1)),
+    }
+
+
+def run_diskhalo(args, E, log):
+    """Exponential disk (N = 2^21, M = 0.1, scale length 1, sech^2 height
+    0.1, Q = 1.5) in a Hernquist halo (N = 2^22, M = 1, a = 4), eps = 0.02:
+    force error of the first evaluation, then 256 graph-captured leapfrog
+    steps at dt = 1/64 (tol 0.7, quadrupole cells)."""
+    nd, nh = args.ndisk, args.nhalo
+    m, x, v = E.mkdiskhalo(nd, nh, seed=args.seed)
+    n = len(m)
+    eng = E.Engine(0)
+    eng.upload(m, x)
+    idx = np.sort(np.random.default_rng(args.seed).permutation(n)[: args.nsub])
+    _, dacc = eng.direct(0.02, idx)
+    p = E.default_params(tol=args.dh_tol, eps=0.02, usequad=1, rsize=4.0)
+    st = eng.eval(p)
+    _, acc = eng.download()
+    e = rel_err(acc[idx], dacc)
+    lf = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, tol=args.dh_tol, eps=0.02, usequad=1)
+    lf.run(1, graph=True)
+    t0 = time.perf_counter()
+    d, n2b, nbc = lf.run(args.lf_steps, graph=True)
+    t_lf = time.perf_counter() - t0
+    drift = abs(d[-1, 0] - d[0, 0]) / abs(d[0, 0])
+    eng.close()
+    return {
+        "metric": "leapfrog steps/s, disk N=2^21 + halo N=2^22, hipGraph-captured",
+        "value": args.lf_steps / t_lf, "unit": "steps/s", "n_gpus": 1, "steps": args.lf_steps, "warmup": 1,
+        "ms_per_step": t_lf / args.lf_steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic disk + Hernquist halo (own recipe and RNG, seed %d)" % args.seed,
+        "config": {"workload": "diskhalo_%d_%d_leapfrog" % (nd, nh), "n": n, "eps": 0.02, "tol": args.dh_tol,
+                   "usequad": 1, "dt": 1.0 / 64.0, "evals_per_s": n * args.lf_steps / t_lf,
+                   "interactions_per_s": float((n2b + nbc).sum() / t_lf),
+                   "relative_energy_drift": float(drift), "energy_first_last": [float(d[0, 0]), float(d[-1, 0])],
+                   "first_eval": {"median_rel_err": float(np.median(e)), "p99_rel_err": float(np.percentile(e, 99)),
+                                  "t_tree_ms": st.t_tree_ms, "t_walk_kernel_ms": st.t_walk_kernel_ms}},
+        "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3, "k_walk<2,true,false,false>"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="plummer4m", choices=["plummer4m", "ensemble", "diskhalo"],
+                    help="plummer4m: the headline metric (default); ensemble / diskhalo: the batch and "
+                         "time-integration workloads")
+    ap.add_argument("--seed", type=int, default=123)
+    ap.add_argument("--nsys", type=int, default=512)
+    ap.add_argument("--lf-steps", type=int, default=0, help="leapfrog steps (default 32 ensemble, 256 disk+halo)")
+    ap.add_argument("--ndisk", type=int, default=1 << 21)
+    ap.add_argument("--nhalo", type=int, default=1 << 22)
+    ap.add_argument("--dh-tol", type=float, default=0.7)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -164,6 +297,13 @@ def main():
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
+
+    if args.workload != "plummer4m":
+        if args.lf_steps <= 0:
+            args.lf_steps = 32 if args.workload == "ensemble" else 256
+        out = (run_ensemble if args.workload == "ensemble" else run_diskhalo)(args, E, log)
+        print(json.dumps(out))
+        return
 
     if args.copy_probe:
         # a plain copy kernel of known size, so that FETCH_SIZE/WRITE_SIZE

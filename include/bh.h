@@ -194,6 +194,14 @@ bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t se
  * zero centre of mass; counter-based RNG keyed by seed, so the result does
  * not depend on thread count.  vel may be NULL. */
 void bh_mkplummer(int64_t n, uint64_t seed, double *mass, double *pos, double *vel);
+/* Synthetic galaxy (no reference counterpart; the time-integration
+ * workload): exponential disk (mass mdisk, scale length rdisk, sech^2 scale
+ * height zdisk, Toomre q) in a Hernquist halo (mass mhalo, scale radius
+ * ahalo, isotropic, truncated at 95 % of its mass), G = 1, approximate
+ * equilibrium; bodies 0..ndisk-1 are the disk.  Counter-based RNG keyed by
+ * seed. */
+void bh_mkdiskhalo(int64_t ndisk, int64_t nhalo, uint64_t seed, double mdisk, double rdisk, double zdisk, double q,
+                   double mhalo, double ahalo, double *mass, double *pos, double *vel);
 
 #ifdef __cplusplus
 }

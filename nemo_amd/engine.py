@@ -75,6 +75,8 @@ SIGNATURES = {
     "bh_debug_perm": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
     "bh_debug_groups": (C.c_int, [_vp, C.c_int64, _i32, _i32, _i64]),
     "bh_mkplummer": (None, [C.c_int64, C.c_uint64, _d, _d, _d]),
+    "bh_mkdiskhalo": (None, [C.c_int64, C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double,
+                             C.c_double, C.c_double, _d, _d, _d]),
     "bh_lf_init": (C.c_int, [_vp, C.POINTER(Params), C.c_int64, _d, _d, _d, C.c_double]),
     "bh_lf_force": (C.c_int, [_vp, _d, _i64, _i64]),
     "bh_lf_drift": (C.c_int, [_vp]),
@@ -138,6 +140,16 @@ def mkplummer(n, seed=123, with_vel=True):
     pos = np.zeros((n, 3))
     vel = np.zeros((n, 3)) if with_vel else None
     lib().bh_mkplummer(n, seed, _p(mass), _p(pos), _p(vel) if with_vel else None)
+    return mass, pos, vel
+
+
+def mkdiskhalo(ndisk=1 << 21, nhalo=1 << 22, seed=123, mdisk=0.1, rdisk=1.0, zdisk=0.1, q=1.5, mhalo=1.0, ahalo=4.0):
+    """Exponential disk in a Hernquist halo (bh_mkdiskhalo); disk first."""
+    n = ndisk + nhalo
+    mass = np.zeros(n)
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    lib().bh_mkdiskhalo(ndisk, nhalo, seed, mdisk, rdisk, zdisk, q, mhalo, ahalo, _p(mass), _p(pos), _p(vel))
     return mass, pos, vel
 
 
