@@ -153,28 +153,139 @@ __device__ void stage_features(const Acts &a, const float *f, int nrow)
     wave_sync_lds();
 }
 
+// ---- register-resident inference -------------------------------------------------
+// The transposed product D' = W H^T (A = weight rows, B = activations) leaves
+// in lane l the outputs j = 16t + 4(l/16) + e of sample l%16, which is the B
+// operand layout of the next layer (k = 8(l/16) + e') once K is permuted
+// inside each 32-wide step: slot e' of lane group q holds k = 32s + 4q + e'
+// (e' < 4) or 32s + 16 + 4q + e' - 4 (e' >= 4).  The weight fragments are
+// loaded once with that permutation and stay in VGPRs; activations never
+// leave registers.  Rounding points are those of the LDS path: bf16 inputs,
+// weights and hidden activations, FP32 accumulation and biases.
+__device__ __forceinline__ int kperm(int s, int q, int e) { return e < 4 ? 32 * s + 4 * q + e : 32 * s + 12 + 4 * q + e; }
+
+struct RegW {
+    bf16x8 w1[4], w2[4][2], w3[4][2], w4[2];
+};
+
+__device__ __forceinline__ void load_regw(RegW &R, const bf16 *Wb)
+{
+    const int l = lane_id(), r = l & 15, q = l >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        R.w1[t] = *reinterpret_cast<const bf16x8 *>(Wb + kW1b + (16 * t + r) * 32 + 8 * q);
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                R.w2[t][s][e] = Wb[kW2b + (16 * t + r) * 64 + kperm(s, q, e)];
+                R.w3[t][s][e] = Wb[kW3b + (16 * t + r) * 64 + kperm(s, q, e)];
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) R.w4[s][e] = Wb[kW4b + r * 64 + kperm(s, q, e)];   // rows >= 3 are zero
+}
+
+// SiLU with the hardware reciprocal (one rounding step more than a division,
+// far below the bf16 rounding that follows).
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+
+__device__ __forceinline__ void act_pack(const f32x4 h[4], bf16x8 hb[2])
+{
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            hb[s][e] = (bf16)silu_fast(h[2 * s][e]);
+            hb[s][4 + e] = (bf16)silu_fast(h[2 * s + 1][e]);
+        }
+}
+
+__device__ __forceinline__ f32x4 bias4(const float *sb, int off)
+{
+    const float4 b = *reinterpret_cast<const float4 *>(sb + off);
+    return f32x4{b.x, b.y, b.z, b.w};
+}
+
+// Forward of 16 samples (lane l: sample l%16) from the layer-1 B operand xb
+// (inputs 8(l/16)..+7 of the sample); lanes l < 16 return outputs 0..2.
+__device__ __forceinline__ f32x4 mlp_fwd_reg(const RegW &R, const float *sb, bf16x8 xb)
+{
+    const int q = lane_id() >> 4;
+    f32x4 h[4];
+    bf16x8 hb[2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = mfma(R.w1[t], xb, bias4(sb, 16 * t + 4 * q));
+    act_pack(h, hb);
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = mfma(R.w2[t][1], hb[1], mfma(R.w2[t][0], hb[0], bias4(sb, 64 + 16 * t + 4 * q)));
+    act_pack(h, hb);
+#pragma unroll
+    for (int t = 0; t < 4; t++) h[t] = mfma(R.w3[t][1], hb[1], mfma(R.w3[t][0], hb[0], bias4(sb, 128 + 16 * t + 4 * q)));
+    act_pack(h, hb);
+    f32x4 y = q == 0 ? bias4(sb, 192) : f32x4{0.f, 0.f, 0.f, 0.f};
+    return mfma(R.w4[1], hb[1], mfma(R.w4[0], hb[0], y));
+}
+
+// Layer-1 B operand of sub-tile sub from 64 staged rows of 16 bf16.
+__device__ __forceinline__ bf16x8 xb_of(const bf16 *sx, int sub)
+{
+    const int l = lane_id(), r = 16 * sub + (l & 15), q = l >> 4;
+    bf16x8 v;
+    if (q < 2) {
+        v = *reinterpret_cast<const bf16x8 *>(sx + r * 16 + 8 * q);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = (bf16)0.f;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void stage_row(bf16 *sx, const float *f)
+{
+    const int l = lane_id();
+    bf16x8 v0, v1;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        v0[k] = (bf16)f[k];
+        v1[k] = (bf16)(k < kMlpIn - 8 ? f[8 + k] : 0.f);
+    }
+    reinterpret_cast<bf16x8 *>(sx + l * 16)[0] = v0;
+    reinterpret_cast<bf16x8 *>(sx + l * 16)[1] = v1;
+}
+
 // ---- batch inference ---------------------------------------------------------
 
 
 __global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ X, int64_t n, const bf16 *__restrict__ Wb,
                                                     const float *__restrict__ bias, float *__restrict__ Y)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
-    __shared__ __attribute__((aligned(16))) bf16 sh[64 * 64];
-    Acts a{sx, {nullptr, nullptr, nullptr}, {sh, sh, sh}};
-    const int64_t r0 = (int64_t)blockIdx.x * 64;
-    stage_features(a, X + r0 * kMlpIn, (int)std::min<int64_t>(64, n - r0));
-    float y[4][4];
-    mlp_forward_wave(a, Wb, bias, y);
-    const int l = lane_id(), col = l & 15;
-    if (col < 3)
+    __shared__ __attribute__((aligned(16))) float sb[208];
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 16];
+    const int l = lane_id();
+    for (int k = l; k < 208; k += 64) sb[k] = bias[k];
+    RegW R;
+    load_regw(R, Wb);
+    for (int64_t r0 = (int64_t)blockIdx.x * 64; r0 < n; r0 += (int64_t)gridDim.x * 64) {
+        float f[kMlpIn];
 #pragma unroll
-        for (int ti = 0; ti < 4; ti++)
+        for (int k = 0; k < kMlpIn; k++) f[k] = r0 + l < n ? X[(r0 + l) * kMlpIn + k] : 0.f;
+        stage_row(sx, f);
+        wave_sync_lds();
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                int64_t row = r0 + ti * 16 + 4 * (l >> 4) + e;
-                if (row < n) Y[row * 3 + col] = y[ti][e];
+        for (int sub = 0; sub < 4; sub++) {
+            const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
+            const int64_t row = r0 + 16 * sub + l;
+            if (l < 16 && row < n) {
+                Y[row * 3] = y[0];
+                Y[row * 3 + 1] = y[1];
+                Y[row * 3 + 2] = y[2];
             }
+        }
+        wave_sync_lds();
+    }
 }
 
 // ---- training: forward + backward, per-wave partial gradients ----------------
@@ -419,57 +530,6 @@ __global__ void k_flatten(const int *__restrict__ lrec, int lcap, const uint32_t
     }
 }
 
-// One wave per 64 entries: features, MLP, predicted residual acceleration.
-__device__ void k_learned_flat_body(int64_t tile, const int *__restrict__ ecell, const int *__restrict__ egrp,
-                                                     int64_t ne, const float4 *__restrict__ gcentre,
-                                                     const float4 *__restrict__ cA, const float4 *__restrict__ cB,
-                                                     const float4 *__restrict__ cF0, const float4 *__restrict__ cF1,
-                                                     const bf16 *__restrict__ Wb, const float *__restrict__ bias,
-                                                     float yscale, float4 *__restrict__ res)
-{
-    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 32];
-    __shared__ __attribute__((aligned(16))) bf16 sh[64 * 64];
-    __shared__ float sy[64 * 4];
-    Acts a{sx, {nullptr, nullptr, nullptr}, {sh, sh, sh}};
-    const int l = lane_id();
-    const int64_t e = tile * 64 + l;
-    float f[kMlpIn];
-    float scale = 0.f;
-    if (e < ne) {
-        const int c = ecell[e];
-        const float4 ctr = gcentre[egrp[e]];
-        const float4 A = cA[c], B = cB[c];
-        scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c], cF1[c], f);
-    } else {
-        for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
-    }
-    bf16x8 v0, v1;
-    for (int k = 0; k < 8; k++) {
-        v0[k] = (bf16)f[k];
-        v1[k] = (bf16)(k < kMlpIn - 8 ? f[8 + k] : 0.f);
-    }
-    bf16x8 zero;
-    for (int k = 0; k < 8; k++) zero[k] = (bf16)0.f;
-    bf16x8 *row = reinterpret_cast<bf16x8 *>(sx + l * 32);
-    row[0] = v0;
-    row[1] = v1;
-    row[2] = zero;
-    row[3] = zero;
-    wave_sync_lds();
-    float y[4][4];
-    mlp_forward_wave(a, Wb, bias, y);
-    const int col = l & 15;
-    if (col < 3)
-        for (int ti = 0; ti < 4; ti++)
-            for (int q = 0; q < 4; q++) sy[(ti * 16 + 4 * (l >> 4) + q) * 4 + col] = y[ti][q];
-    wave_sync_lds();
-    if (e < ne) {
-        const float w = yscale * scale;
-        res[e] = make_float4(sy[l * 4] * w, sy[l * 4 + 1] * w, sy[l * 4 + 2] * w, 0.f);
-    }
-    wave_sync_lds();   // the next tile reuses sx, sh and sy
-}
-
 __global__ void k_zero_i32n(int *p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -482,19 +542,52 @@ __global__ void k_learned_count(const uint32_t *total, int64_t ecap, unsigned lo
     if ((int64_t)*total > ecap) stat[S_FLAGS] |= (unsigned long long)F_ECAP_OVF;
 }
 
-// Entry count read on the device; each wave strides over the 64-entry tiles.
-__global__ __launch_bounds__(64) void k_learned_flat_guard(const int *__restrict__ ecell, const int *__restrict__ egrp,
-                                                           const uint32_t *__restrict__ total,
-                                                           const float4 *__restrict__ gcentre,
-                                                           const float4 *__restrict__ cA, const float4 *__restrict__ cB,
-                                                           const float4 *__restrict__ cF0,
-                                                           const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
-                                                           const float *__restrict__ bias, float yscale,
-                                                           float4 *__restrict__ res, int64_t ecap)
+// Predicted residual acceleration of every flattened entry: each wave keeps
+// the weights in registers and strides over 64-entry tiles (the entry count
+// is read on the device); features of a tile go through LDS once.
+__global__ __launch_bounds__(64) void k_learned_reg(const int *__restrict__ ecell, const int *__restrict__ egrp,
+                                                    const uint32_t *__restrict__ total, int64_t ecap,
+                                                    const float4 *__restrict__ gcentre, const float4 *__restrict__ cA,
+                                                    const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
+                                                    const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
+                                                    const float *__restrict__ bias, float yscale,
+                                                    float4 *__restrict__ res)
 {
+    __shared__ __attribute__((aligned(16))) float sb[208];
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 16];
+    const int l = lane_id();
+    for (int k = l; k < 208; k += 64) sb[k] = bias[k];
+    RegW R;
+    load_regw(R, Wb);
     const int64_t ne = min((int64_t)*total, ecap);
-    for (int64_t t = blockIdx.x; t * 64 < ne; t += gridDim.x)
-        k_learned_flat_body(t, ecell, egrp, ne, gcentre, cA, cB, cF0, cF1, Wb, bias, yscale, res);
+    for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < ne; t0 += (int64_t)gridDim.x * 64) {
+        const int64_t e = t0 + l;
+        float f[kMlpIn];
+        float scale = 0.f;
+        if (e < ne) {
+            const int c = ecell[e];
+            const float4 ctr = gcentre[egrp[e]];
+            const float4 A = cA[c], B = cB[c];
+            scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c], cF1[c],
+                                  f);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+        }
+        stage_row(sx, f);
+        wave_sync_lds();
+#pragma unroll
+        for (int sub = 0; sub < 4; sub++) {
+            const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
+            const float sc = __shfl(scale, 16 * sub + (l & 15), 64);
+            const int64_t er = t0 + 16 * sub + l;
+            if (l < 16 && er < ne) {
+                const float w = yscale * sc;
+                res[er] = make_float4(y[0] * w, y[1] * w, y[2] * w, 0.f);
+            }
+        }
+        wave_sync_lds();
+    }
 }
 
 // Adds the correction c to the accelerations of group g's members.
@@ -776,9 +869,9 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         k_learned_count<<<1, 1, 0, st>>>(off + nreg, ecap, stat);
         const int64_t tiles = (ecap + 63) / 64;
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[8], st));
-        k_learned_flat_guard<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 32), 64, 0, st>>>(
-            ecell, egrp, off + nreg, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias), M.norm[0], res,
-            ecap);
+        k_learned_reg<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 8), 64, 0, st>>>(
+            ecell, egrp, off + nreg, ecap, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias),
+            M.norm[0], res);
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
         k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
                                             T.tperm, ptr<double>(ctx->acc));
@@ -828,7 +921,8 @@ void mlp_forward(bh_ctx *ctx, const float *d_x, int64_t n, float *d_y)
 {
     Mlp &M = ctx->mlp;
     if (n <= 0) return;
-    k_mlp_forward<<<(int)((n + 63) / 64), 64, 0, ctx->stream>>>(d_x, n, ptr<bf16>(M.wb), ptr<float>(M.bias), d_y);
+    k_mlp_forward<<<(int)std::min<int64_t>((n + 63) / 64, (int64_t)ctx->ncu * 8), 64, 0, ctx->stream>>>(
+        d_x, n, ptr<bf16>(M.wb), ptr<float>(M.bias), d_y);
     BH_HIP(hipGetLastError());
 }
 
