@@ -106,7 +106,8 @@ bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, d
 bh_status bh_sync(bh_ctx *ctx);
 
 /* Device tree inspection (tests): Morton keys of the uploaded mass set in
- * input order, and the cells in DFS pre-order. */
+ * input order, and the cells in DFS pre-order (quad6 only after an
+ * evaluation with usequad). */
 bh_status bh_debug_keys(bh_ctx *ctx, uint64_t *keys);
 bh_status bh_debug_cells(bh_ctx *ctx, int64_t cap, int *level, int *nbody, double *mass, double *com,
                          double *quad6, int64_t *ncell);

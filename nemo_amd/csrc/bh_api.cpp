@@ -310,7 +310,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
-                          &ctx->lvals0, &ctx->lvals1, &ctx->cfnb,
+                          &ctx->lvals0, &ctx->lvals1, &ctx->cfnb, &ctx->cowner, &ctx->gowner,
                           &ctx->vel,    &ctx->acc_old, &ctx->lfstep, &ctx->diag_partial, &ctx->diag_buf,
                           &ctx->n2b_buf, &ctx->nbc_buf, &ctx->lrec,   &ctx->cF0,    &ctx->cF1,
                           &ctx->mlp.params, &ctx->mlp.wb, &ctx->mlp.bias, &ctx->mlp.gpart, &ctx->mlp.lpart,
@@ -789,7 +789,9 @@ bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo
         q.learned = 0;
         if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
         for (int attempt = 0;; attempt++) {
+            ctx->mlp_sampling = true;
             bh::build_tree(ctx, q);
+            ctx->mlp_sampling = false;
             bh::walk_fast(ctx, q, 2, std::max(1, stride));
             unsigned long long stat[bh::S_NSLOTS];
             BH_HIP(hipMemcpyAsync(stat, ctx->counters.p, sizeof stat, hipMemcpyDeviceToHost, ctx->stream));

@@ -48,7 +48,7 @@ struct bh_ctx {
     bh::DevBuf thi, tlo, td, tperm, tself;
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
-    bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre;
+    bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre, cowner, gowner;
     bh::DevBuf cmo, cfnb, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
@@ -89,6 +89,7 @@ struct bh_ctx {
     bh::Tree tree;
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays
+    bool mlp_sampling = false;  // a sample-collection walk is running (the tree writes learned features)
     int tuned_group = 128;      // defaults for group_size = 0 / nleaf = -1 (set by bh_autotune)
     int tuned_nleaf = 2;
     int tcell_cap = 0;          // cell capacity of a separate test-set topology
@@ -129,7 +130,8 @@ T *ptr(DevBuf &b) { return static_cast<T *>(b.p); }
 
 // bh_tree.hip
 void build_tree(bh_ctx *ctx, const bh_params &p);
-void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n);
+void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n,
+                        const unsigned long long *dn = nullptr);
 void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uint32_t *v1, int64_t n,
                       int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout);
 // bh_exact.hip

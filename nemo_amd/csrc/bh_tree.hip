@@ -294,8 +294,17 @@ __global__ void k_keys(const double *__restrict__ pos, const double *__restrict_
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = kTpb * kScanItems;
 
-__global__ void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, uint32_t *partial)
+// n: entries known to the host; dn (optional): count on the device, which
+// limits the scan to min(n, *dn) entries (the rest of out is not written).
+__device__ __forceinline__ int64_t scan_len(int64_t n, const unsigned long long *dn)
 {
+    return dn ? min(n, (int64_t)*dn) : n;
+}
+
+__global__ void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, const unsigned long long *dn,
+                              uint32_t *partial)
+{
+    n = scan_len(n, dn);
     const int64_t base = (int64_t)blockIdx.x * kScanChunk;
     uint32_t s = 0;
     for (int r = 0; r < kScanItems; r++) {
@@ -307,7 +316,8 @@ __global__ void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, uint32
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_top(uint32_t *partial, int64_t nparts)
+// Scans the block sums; the total goes to partial[nparts] and out[n].
+__global__ void k_scan_top(uint32_t *partial, int64_t nparts, uint32_t *out, int64_t n)
 {
     uint32_t carry = 0;
     for (int64_t base = 0; base < nparts; base += kTpb) {
@@ -318,13 +328,18 @@ __global__ void k_scan_top(uint32_t *partial, int64_t nparts)
         if (i < nparts) partial[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) partial[nparts] = carry;
+    if (threadIdx.x == 0) {
+        partial[nparts] = carry;
+        out[n] = carry;
+    }
 }
 
-__global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n,
+__global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n, const unsigned long long *dn,
                             const uint32_t *__restrict__ partial)
 {
+    n = scan_len(n, dn);
     const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    if (base >= n) return;
     const int64_t mine = base + (int64_t)threadIdx.x * kScanItems;
     uint32_t v[kScanItems];
     uint32_t s = 0;
@@ -342,7 +357,6 @@ __global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n,
         if (i < n) out[i] = ex;
         ex += v[r];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kTpb - 1) out[n] = partial[gridDim.x];
 }
 
 // ---- radix sort (LSD, 8-bit digits, stable) -----------------------------------
@@ -365,34 +379,48 @@ __global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int s
     hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Scatter one tile.  Within a round the 256 threads hold 256 consecutive
-// keys; ranks among equal digits come from a wave-level match (8 ballots)
-// and a per-digit prefix over the block's 4 waves, so the order of equal
-// digits is preserved (round, wave, lane) = input order.
+// Scatter one tile of kSortTile keys.  The tile is first ranked into LDS in
+// digit order: within a round of 256 consecutive keys, ranks among equal
+// digits come from a wave-level match (8 ballots) and a per-digit prefix over
+// the block's 4 waves, so equal digits keep their input order (round, wave,
+// lane).  Then consecutive threads copy consecutive LDS slots to the output,
+// so the writes of one digit's run are contiguous.
 __global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                         uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
-                                                        int64_t n, int shift, const uint32_t *__restrict__ hist_scan,
-                                                        int nblk)
+                                                        int64_t n, int shift, const uint32_t *__restrict__ hist,
+                                                        const uint32_t *__restrict__ hist_scan, int nblk)
 {
-    __shared__ uint32_t run[256];          // next free output slot per digit
+    __shared__ uint64_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    __shared__ uint32_t toff[256];         // first LDS slot of each digit in this tile
+    __shared__ uint32_t gofs[256];         // output position of that slot
+    __shared__ uint32_t run[256];          // next free LDS slot per digit
     __shared__ uint32_t wcnt[kTpb / 64][256];
     const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-    run[t] = hist_scan[(int64_t)t * nblk + blockIdx.x];
+    {
+        uint32_t tot;
+        const uint32_t c = hist[(int64_t)t * nblk + blockIdx.x];
+        const uint32_t o = block_excl_scan(c, &tot);
+        toff[t] = o;
+        run[t] = o;
+        gofs[t] = hist_scan[(int64_t)t * nblk + blockIdx.x] - o;
+    }
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const int cnt = (int)min<int64_t>(kSortTile, n - base);
     const uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
     for (int r = 0; r < kSortItems; r++) {
 #pragma unroll
         for (int k = 0; k < kTpb / 64; k++) wcnt[k][t] = 0;
         __syncthreads();
-        const int64_t i = base + (int64_t)r * kTpb + t;
-        const bool ok = i < n;
-        uint64_t key = ok ? kin[i] : 0;
-        uint32_t val = ok ? vin[i] : 0;
-        uint32_t d = ok ? (uint32_t)((key >> shift) & 255u) : 0u;
+        const int li = r * kTpb + t;
+        const bool ok = li < cnt;
+        const uint64_t key = ok ? kin[base + li] : 0;
+        const uint32_t val = ok ? vin[base + li] : 0;
+        const uint32_t d = ok ? (uint32_t)((key >> shift) & 255u) : 0u;
         uint64_t peers = __ballot(ok);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            uint64_t bal = __ballot((d >> b) & 1u);
+            const uint64_t bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
         const uint32_t rank = __popcll(peers & lt);
@@ -402,7 +430,7 @@ __global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restri
             uint32_t acc = run[t];
 #pragma unroll
             for (int k = 0; k < kTpb / 64; k++) {
-                uint32_t c = wcnt[k][t];
+                const uint32_t c = wcnt[k][t];
                 wcnt[k][t] = acc;
                 acc += c;
             }
@@ -410,11 +438,17 @@ __global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restri
         }
         __syncthreads();
         if (ok) {
-            uint32_t dst = wcnt[w][d] + rank;
-            kout[dst] = key;
-            vout[dst] = val;
+            const uint32_t slot = wcnt[w][d] + rank;
+            skey[slot] = key;
+            sval[slot] = val;
         }
-        __syncthreads();
+        __syncthreads();   // wcnt is cleared at the top of the next round
+    }
+    for (int j = t; j < cnt; j += kTpb) {
+        const uint64_t key = skey[j];
+        const uint32_t dst = gofs[(key >> shift) & 255u] + (uint32_t)j;
+        kout[dst] = key;
+        vout[dst] = sval[j];
     }
 }
 
@@ -503,33 +537,45 @@ __device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restric
     }
 }
 
-// One thread per cell.  The cells that start at body s are the levels
-// cpl[s-1]+1 .. cpl[s] with ids cbase[s] ..; the owner of cell id is the last
-// s with cbase[s] <= id.
-__global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase,
-                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
-                        int nb, const unsigned long long *ncellp, int *cfirst, int *clast, uint8_t *clevel,
-                        int *cskip, int *pre, int2 *fnb, unsigned long long *deep)
+// Owner body of every cell: body s starts the cells cbase[s] .. cbase[s+1]-1.
+__global__ void k_owner(const uint32_t *__restrict__ cbase, int nb, const unsigned long long *ncellp, int *owner)
 {
-    int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (int)*ncellp) return;
-    int lo = 0, hi = nb;           // upper_bound over cbase[0..nb)
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (cbase[mid] <= (uint32_t)id) lo = mid + 1;
-        else hi = mid;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nb) return;
+    const int id1 = min((int)cbase[s + 1], (int)*ncellp);
+    for (int id = (int)cbase[s]; id < id1; id++) owner[id] = s;
+}
+
+// One thread per cell (grid-stride over the device cell count).  The cells
+// that start at body s are the levels cpl[s-1]+1 .. cpl[s] with ids from
+// cbase[s]; the range of a cell of level L ends before the first j >= s with
+// cpl[j] < L.
+__global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase, const int *__restrict__ owner,
+                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
+                        const unsigned long long *ncellp, int *cfirst, int *clast, uint8_t *clevel, int *cskip,
+                        int *pre, int2 *fnb, unsigned long long *deep)
+{
+    const int ncell = (int)*ncellp;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < ncell; base += stride) {
+        const int id = base + threadIdx.x;
+        bool isdeep = false;
+        if (id < ncell) {
+            const int s = owner[id];
+            const int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
+            const int L = l0 + (id - (int)cbase[s]);
+            const int e = range_end(cpl, m1, m2, m3, s, L);
+            cfirst[id] = s;
+            clast[id] = e;
+            clevel[id] = (uint8_t)L;
+            if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
+            if (fnb) fnb[id] = make_int2(s, e - s + 1);
+            if (pre) pre[s + id] = ~id;
+            isdeep = L == kMaxLevel;
+        }
+        const unsigned long long m = __ballot(isdeep);
+        if (deep && m && lane_id() == 0) atomicAdd(deep, (unsigned long long)__popcll(m));
     }
-    const int s = lo - 1;
-    const int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
-    const int L = l0 + (id - (int)cbase[s]);
-    const int e = range_end(cpl, m1, m2, m3, s, L);
-    cfirst[id] = s;
-    clast[id] = e;
-    clevel[id] = (uint8_t)L;
-    if (cskip) cskip[id] = (e + 1) + (int)cbase[e + 1];
-    if (fnb) fnb[id] = make_int2(s, e - s + 1);
-    if (pre) pre[s + id] = ~id;
-    if (deep && L == kMaxLevel) atomicAdd(deep, 1ULL);
 }
 
 // Pre-order position of body s: after the cells that start at s.
@@ -601,11 +647,29 @@ __device__ __forceinline__ unsigned octant_box(unsigned m)
     return box;
 }
 
+// End (exclusive) of the child of cell c (bodies s..e, level L) that starts
+// at body t: a child cell's last body + 1, or t + 1 for a body; *cc gets the
+// child cell or -1.  A child cell past the built cells (a capacity overflow,
+// which is evaluated again) or an inconsistent range ends the run at e + 1,
+// so every loop over children advances and stays in bounds.
+__device__ __forceinline__ int child_end(int c, int s, int e, int t, int L, const int8_t *__restrict__ cpl,
+                                         const uint32_t *__restrict__ cbase, const int *__restrict__ clast, int ncell,
+                                         int *cc)
+{
+    *cc = -1;
+    if (cpl[t] < L + 1) return t + 1;
+    const int k = (t == s) ? c + 1 : (int)cbase[t];
+    if (k >= ncell) return e + 1;
+    *cc = k;
+    const int tn = clast[k] + 1;
+    return (tn > t && tn <= e + 1) ? tn : e + 1;
+}
+
 template <bool WRITE>
 __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                                 const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                                 const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int G, int lsys,
-                                int *gfirst, int *gcount, uint32_t off)
+                                int ncell, int *gfirst, int *gcount, uint32_t off)
 {
     const int s = cfirst[c], e = clast[c], L = clevel[c];
     uint32_t k = 0;
@@ -621,10 +685,9 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
         // system (a system-root cell or a one-body system) with at most G
         // points is a group of its own, never merged with another system
         for (int t = s; t <= e;) {
-            const bool is_cell = cpl[t] >= L + 1;
-            const int cc = (t == s) ? c + 1 : (int)cbase[t];
-            const int tn = is_cell ? clast[cc] + 1 : t + 1;
-            if (!(is_cell && L + 1 < lsys) && tn - t <= G) emit(t, tn - t);
+            int cc;
+            const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
+            if (!(cc >= 0 && L + 1 < lsys) && tn - t <= G) emit(t, tn - t);
             t = tn;
         }
         return k;
@@ -638,9 +701,8 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
     // octants of the children holding more than G points
     unsigned big = 0;
     for (int t = s; t <= e;) {
-        const bool is_cell = cpl[t] >= L + 1;
-        const int cc = (t == s) ? c + 1 : (int)cbase[t];
-        const int tn = is_cell ? clast[cc] + 1 : t + 1;
+        int cc;
+        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
         if (tn - t > G) big |= 1u << ((keys[t] >> shift) & 7);
         t = tn;
     }
@@ -649,9 +711,8 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
     int t = s, run = 0, rs = s;
     unsigned rmask = 0;
     while (t <= e) {
-        const bool is_cell = cpl[t] >= L + 1;
-        const int cc = (t == s) ? c + 1 : (int)cbase[t];
-        const int tn = is_cell ? clast[cc] + 1 : t + 1;
+        int cc;
+        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
         const int size = tn - t;
         const unsigned o = 1u << ((keys[t] >> shift) & 7);
         if (size > G) {
@@ -677,16 +738,12 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
 __global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
-                              const uint8_t *__restrict__ clevel, int cap, const unsigned long long *ncellp, int G,
+                              const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G,
                               int lsys, uint32_t *gcnt)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap) return;
-    if (c >= (int)*ncellp) {
-        gcnt[c] = 0;
-        return;
-    }
-    gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, nullptr, nullptr, 0);
+    const int ncell = (int)*ncellp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+        gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, nullptr, nullptr, 0);
 }
 
 __global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
@@ -695,9 +752,9 @@ __global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *_
                               const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G, int lsys,
                               const uint32_t *__restrict__ goff, int *gfirst, int *gcount)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int)*ncellp) return;
-    pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, gfirst, gcount, goff[c]);
+    const int ncell = (int)*ncellp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+        pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, gfirst, gcount, goff[c]);
 }
 
 // Walk roots of a batch: the system's root cell (the cell at the first
@@ -718,14 +775,27 @@ __global__ void k_sysroot_cells(const uint8_t *__restrict__ clevel, const int *_
 }
 
 // Children of each cell in octant order; parent links; child-cell counts.
+__device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                            const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
+                            const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell, int *csub,
+                            int *cnkids, int *cnkc, int *cparent, int *cctr);
+
 __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                            const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                            const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
                            const unsigned long long *ncellp, int *csub, int *cnkids, int *cnkc, int *cparent,
                            int *cctr)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= (int)*ncellp) return;
+    const int ncell = (int)*ncellp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
+        children_of(c, keys, cpl, cbase, cfirst, clast, clevel, ncell, csub, cnkids, cnkc, cparent, cctr);
+}
+
+__device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
+                            const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
+                            const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell, int *csub,
+                            int *cnkids, int *cnkc, int *cparent, int *cctr)
+{
     int s = cfirst[c], e = clast[c], L = clevel[c];
     int *sub = csub + 8 * (int64_t)c;
 #pragma unroll
@@ -741,18 +811,17 @@ __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__re
     int nk = 0, nkc = 0;
     int t = s;
     while (t <= e) {
-        int o = (int)((keys[t] >> shift) & 7);
-        int child_is_cell = cpl[t] >= L + 1;
-        if (child_is_cell) {
-            int cc = (t == s) ? c + 1 : (int)cbase[t];
+        const int o = (int)((keys[t] >> shift) & 7);
+        int cc;
+        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
+        if (cc >= 0) {
             sub[o] = ~cc;
             cparent[cc] = c;
             nkc++;
-            t = clast[cc] + 1;
         } else {
             sub[o] = t;
-            t++;
         }
+        t = tn;
         nk++;
     }
     cnkids[c] = nk;
@@ -879,8 +948,10 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     me[1] = px;
     me[2] = py;
     me[3] = pz;
+    if (usequad) {
 #pragma unroll
-    for (int u = 0; u < 6; u++) me[4 + u] = q[u];
+        for (int u = 0; u < 6; u++) me[4 + u] = q[u];
+    }
     float4 A, B;
     split2(px, A.x, B.x);
     split2(py, A.y, B.y);
@@ -889,8 +960,10 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
     cA[c] = A;
     cB[c] = B;
-    cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
-    cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
+    if (usequad) {
+        cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+        cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
+    }
     if (cF0 && Lg >= 0) {
         // geometric centre of the cell from the key prefix of its first body
         const uint64_t key = keys[cfirst[c]];
@@ -911,24 +984,98 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     }
 }
 
-// Cells bucketed by level (a one-digit radix sort of (level, cell id)) so the
-// moments can be computed level by level, deepest first, with a kernel
-// boundary between levels.
-__global__ void k_level_keys(const uint8_t *__restrict__ clevel, int cap, const unsigned long long *ncellp,
-                             uint64_t *keys, uint32_t *vals)
+// Cells bucketed by level so the moments can be computed level by level,
+// deepest first, with a kernel boundary between levels: a counting sort with
+// one global atomic per (block, level).  The order of cells inside a level
+// is arbitrary; each cell's moments depend only on its children.
+constexpr int kLevels = kMaxLevel + 1;
+
+__global__ void k_level_zero(uint32_t *lcount)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cap) return;
-    keys[c] = c < (int)*ncellp ? clevel[c] : 255;
-    vals[c] = (uint32_t)c;
+    if (threadIdx.x < kLevels) lcount[threadIdx.x] = 0;
 }
 
-// Start of each digit 0..kMaxLevel+1 in the sorted order, from the scanned
-// digit-major histogram of the pass (entry d*nblk is digit d in block 0).
-__global__ void k_level_offsets(const uint32_t *__restrict__ hist_scan, int nblk, uint32_t *out)
+// Lanes of the wave holding the same level (levels < 32): a 5-ballot match.
+__device__ __forceinline__ uint64_t level_peers(int L, bool ok)
 {
-    int d = threadIdx.x;
-    if (d <= kMaxLevel + 1) out[d] = hist_scan[(int64_t)d * nblk];
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 5; b++) {
+        const uint64_t bal = __ballot((L >> b) & 1);
+        peers &= ((L >> b) & 1) ? bal : ~bal;
+    }
+    return peers;
+}
+
+__global__ __launch_bounds__(kTpb) void k_level_count(const uint8_t *__restrict__ clevel,
+                                                      const unsigned long long *ncellp, uint32_t *lcount)
+{
+    __shared__ uint32_t h[kLevels];
+    if (threadIdx.x < kLevels) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int ncell = (int)*ncellp;
+    const int l = lane_id();
+    const uint64_t lt = l ? (~0ULL >> (64 - l)) : 0ULL;
+    for (int base = blockIdx.x * blockDim.x; base < ncell; base += gridDim.x * blockDim.x) {
+        const int c = base + threadIdx.x;
+        const bool ok = c < ncell;
+        const int L = ok ? clevel[c] : 0;
+        const uint64_t peers = level_peers(L, ok);
+        if (ok && __popcll(peers & lt) == 0) atomicAdd(&h[L], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    if (threadIdx.x < kLevels && h[threadIdx.x]) atomicAdd(&lcount[threadIdx.x], h[threadIdx.x]);
+}
+
+// Offsets of the levels (loff[0..kLevels]) and the running cursors.
+__global__ void k_level_scan(const uint32_t *__restrict__ lcount, uint32_t *loff, uint32_t *lcursor)
+{
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (int L = 0; L < kLevels; L++) {
+        loff[L] = acc;
+        lcursor[L] = acc;
+        acc += lcount[L];
+    }
+    loff[kLevels] = acc;
+}
+
+// Each block counts its cells per level, reserves ranges in the level lists,
+// then places its cells (same grid-stride order in both passes).
+__global__ __launch_bounds__(kTpb) void k_level_place(const uint8_t *__restrict__ clevel,
+                                                      const unsigned long long *ncellp, uint32_t *lcursor, int *list)
+{
+    __shared__ uint32_t h[kLevels], base[kLevels];
+    if (threadIdx.x < kLevels) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int ncell = (int)*ncellp;
+    const int l = lane_id();
+    const uint64_t lt = l ? (~0ULL >> (64 - l)) : 0ULL;
+    for (int b0 = blockIdx.x * blockDim.x; b0 < ncell; b0 += gridDim.x * blockDim.x) {
+        const int c = b0 + threadIdx.x;
+        const bool ok = c < ncell;
+        const int L = ok ? clevel[c] : 0;
+        const uint64_t peers = level_peers(L, ok);
+        if (ok && __popcll(peers & lt) == 0) atomicAdd(&h[L], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    if (threadIdx.x < kLevels) {
+        base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&lcursor[threadIdx.x], h[threadIdx.x]) : 0u;
+        h[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int b0 = blockIdx.x * blockDim.x; b0 < ncell; b0 += gridDim.x * blockDim.x) {
+        const int c = b0 + threadIdx.x;
+        const bool ok = c < ncell;
+        const int L = ok ? clevel[c] : 0;
+        const uint64_t peers = level_peers(L, ok);
+        const uint32_t rank = __popcll(peers & lt);
+        const int leader = ok ? __ffsll((unsigned long long)peers) - 1 : 0;
+        uint32_t wb = 0;
+        if (ok && rank == 0) wb = atomicAdd(&h[L], (uint32_t)__popcll(peers));
+        wb = __shfl(wb, leader, 64);
+        if (ok) list[base[L] + wb + rank] = c;
+    }
 }
 
 __global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__restrict__ loff, int L,
@@ -949,19 +1096,21 @@ __global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
+// Grid of the kernels that stride over a device-side count bounded by cap.
+inline int capgrid(const bh_ctx *ctx, int64_t cap) { return (int)std::min<int64_t>(nblocks(cap), (int64_t)ctx->ncu * 8); }
 
 }  // namespace
 
 // ---- host launchers ------------------------------------------------------------
 
-void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n)
+void exclusive_scan_u32(bh_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n, const unsigned long long *dn)
 {
-    // out must hold n + 1 entries; out[n] = total
+    // out must hold n + 1 entries; out[n] = total (in != out)
     int64_t nparts = std::max<int64_t>(1, (n + kScanChunk - 1) / kScanChunk);
     uint32_t *partial = ensure<uint32_t>(ctx->scan_tmp, nparts + 1);
-    k_scan_reduce<<<nparts, kTpb, 0, ctx->stream>>>(in, n, partial);
-    k_scan_top<<<1, kTpb, 0, ctx->stream>>>(partial, nparts);
-    k_scan_down<<<nparts, kTpb, 0, ctx->stream>>>(in, out, n, partial);
+    k_scan_reduce<<<nparts, kTpb, 0, ctx->stream>>>(in, n, dn, partial);
+    k_scan_top<<<1, kTpb, 0, ctx->stream>>>(partial, nparts, out, n);
+    k_scan_down<<<nparts, kTpb, 0, ctx->stream>>>(in, out, n, dn, partial);
     BH_HIP(hipGetLastError());
 }
 
@@ -969,13 +1118,14 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
                       int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout)
 {
     int nblk = (int)std::max<int64_t>(1, (n + kSortTile - 1) / kSortTile);
-    uint32_t *hist = ensure<uint32_t>(ctx->hist, 256 * (size_t)nblk + 1);
+    uint32_t *hist = ensure<uint32_t>(ctx->hist, 2 * (256 * (size_t)nblk + 1));
+    uint32_t *hscan = hist + 256 * (size_t)nblk + 1;
     uint64_t *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         k_radix_hist<<<nblk, kTpb, 0, ctx->stream>>>(ka, n, shift, hist, nblk);
-        exclusive_scan_u32(ctx, hist, hist, 256 * (int64_t)nblk);
-        k_radix_scatter<<<nblk, kTpb, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, hist, nblk);
+        exclusive_scan_u32(ctx, hist, hscan, 256 * (int64_t)nblk);
+        k_radix_scatter<<<nblk, kTpb, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, hist, hscan, nblk);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -998,7 +1148,7 @@ struct Topo {
 
 static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &bcpl, DevBuf &bcnt, DevBuf &bbase,
                      DevBuf &bm1, DevBuf &bfirst, DevBuf &blast, DevBuf &blevel, DevBuf *bskip, DevBuf *bpre,
-                     DevBuf *bfnb, int slot, int slot_raw, unsigned long long flag)
+                     DevBuf *bfnb, DevBuf &bowner, int slot, int slot_raw, unsigned long long flag)
 {
     hipStream_t st = ctx->stream;
     unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
@@ -1023,8 +1173,10 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     int *cskip = bskip ? ensure<int>(*bskip, cap) : nullptr;
     int *pre = bpre ? ensure<int>(*bpre, (size_t)n + cap) : nullptr;
     int2 *fnb = bfnb ? ensure<int2>(*bfnb, cap) : nullptr;
-    k_cells<<<nblocks(cap), kTpb, 0, st>>>(t.cpl, t.cbase, m1, m2, m3, n, t.ncellp, t.first, t.last, t.level, cskip, pre,
-                                           fnb, bskip ? stat + S_DEEP : nullptr);
+    int *owner = ensure<int>(bowner, cap);
+    k_owner<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, owner);
+    k_cells<<<capgrid(ctx, cap), kTpb, 0, st>>>(t.cpl, t.cbase, owner, m1, m2, m3, t.ncellp, t.first, t.last, t.level,
+                                                cskip, pre, fnb, bskip ? stat + S_DEEP : nullptr);
     if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, pre);
     BH_HIP(hipGetLastError());
     return t;
@@ -1044,11 +1196,11 @@ static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_
     }
     uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->cap + 1);
     uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->cap + 1);
-    k_group_count<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->cap,
-                                                    t->ncellp, G, lsys, gcnt);
-    exclusive_scan_u32(ctx, gcnt, goff, t->cap);
-    k_group_write<<<nblocks(t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level, t->ncellp, G,
-                                                    lsys, goff, gf, gc);
+    k_group_count<<<capgrid(ctx, t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level,
+                                                         t->ncellp, G, lsys, gcnt);
+    exclusive_scan_u32(ctx, gcnt, goff, t->cap, t->ncellp);
+    k_group_write<<<capgrid(ctx, t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level,
+                                                         t->ncellp, G, lsys, goff, gf, gc);
     k_copy_count<<<1, 1, 0, st>>>(goff + t->cap, stat + S_NGROUPS);
     BH_HIP(hipGetLastError());
 }
@@ -1084,7 +1236,7 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo,
     }
     if (ctx->tcell_cap == 0) ctx->tcell_cap = (int)std::min<int64_t>(2 * nt + 64, INT32_MAX / 2);
     Topo t = topology(ctx, tks, (int)nt, ctx->tcell_cap, ctx->gcpl, ctx->gccnt, ctx->gcbase, ctx->gm1, ctx->gcf,
-                      ctx->gcl, ctx->gclv, nullptr, nullptr, nullptr, S_TNCELL, S_TNCELL_RAW, F_TCELL_OVF);
+                      ctx->gcl, ctx->gclv, nullptr, nullptr, nullptr, ctx->gowner, S_TNCELL, S_TNCELL_RAW, F_TCELL_OVF);
     make_groups(ctx, &t, tks, nt, G);
 }
 
@@ -1214,7 +1366,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     }
     const int cap = ctx->cell_cap;
     Topo tt = topology(ctx, ks, nb, cap, ctx->cpl, ctx->ccnt, ctx->cbase, ctx->cmin1, ctx->cfirst, ctx->clast,
-                       ctx->clevel, &ctx->cskip, &ctx->pre, &ctx->cfnb, S_NCELL, S_NCELL_RAW, F_CELL_OVF);
+                       ctx->clevel, &ctx->cskip, &ctx->pre, &ctx->cfnb, ctx->cowner, S_NCELL, S_NCELL_RAW, F_CELL_OVF);
     T.cap = cap;
     int8_t *cpl = tt.cpl;
     uint32_t *cbase = tt.cbase;
@@ -1224,27 +1376,27 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *csub = ensure<int>(ctx->csub, 8 * (size_t)cap);
     int *cnkids = ensure<int>(ctx->cnkids, cap), *cnkc = ensure<int>(ctx->cnkc, cap);
     int *cparent = ensure<int>(ctx->cparent, cap), *cctr = ensure<int>(ctx->cctr, cap);
-    k_children<<<nblocks(cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids, cnkc,
+    k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids, cnkc,
                                               cparent, cctr);
     double *mo = ensure<double>(ctx->cmo, 10 * (size_t)cap);
     float4 *cA = ensure<float4>(ctx->cA, cap), *cB = ensure<float4>(ctx->cB, cap);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
     float4 *cF0 = ensure<float4>(ctx->cF0, cap), *cF1 = ensure<float4>(ctx->cF1, cap);
     // cells bucketed by level, then the moments level by level, deepest first
-    uint64_t *lk0 = ensure<uint64_t>(ctx->lkeys0, cap), *lk1 = ensure<uint64_t>(ctx->lkeys1, cap);
-    uint32_t *lv0 = ensure<uint32_t>(ctx->lvals0, cap), *lv1 = ensure<uint32_t>(ctx->lvals1, cap);
-    k_level_keys<<<nblocks(cap), kTpb, 0, st>>>(clevel, cap, tt.ncellp, lk0, lv0);
-    uint64_t *lks;
-    uint32_t *lvs;
-    radix_sort_pairs(ctx, lk0, lk1, lv0, lv1, cap, 0, 8, &lks, &lvs);
-    const int lnblk = (int)((cap + kSortTile - 1) / kSortTile);
-    uint32_t *loffd = ensure<uint32_t>(ctx->lhist, kMaxLevel + 2);
-    k_level_offsets<<<1, 64, 0, st>>>(ptr<uint32_t>(ctx->hist), lnblk, loffd);
-    const int *llist = (const int *)lvs;
+    uint32_t *lwork = ensure<uint32_t>(ctx->lhist, 3 * (kLevels + 1));
+    uint32_t *lcount = lwork, *loffd = lwork + (kLevels + 1), *lcursor = lwork + 2 * (kLevels + 1);
+    int *llist = ensure<int>(ctx->llist, cap);
+    k_level_zero<<<1, 64, 0, st>>>(lcount);
+    k_level_count<<<capgrid(ctx, cap), kTpb, 0, st>>>(clevel, tt.ncellp, lcount);
+    k_level_scan<<<1, 64, 0, st>>>(lcount, loffd, lcursor);
+    k_level_place<<<capgrid(ctx, cap), kTpb, 0, st>>>(clevel, tt.ncellp, lcursor, llist);
     const int mgrid = ctx->ncu * 4;
     const int lsys = ctx->sysbits / 3;
+    // the learned features are written only for the learned walk and sampling
+    const bool feat = p.learned != 0 || ctx->mlp_sampling;
     for (int L = kMaxLevel; L >= 0; L--)
-        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, cF0, cF1, csub, cfirst, clast, clevel, bd, mo, cA,
+        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst, clast,
+                                                clevel, bd, mo, cA,
                                                 cB, cQ0, cQ1, d_box, p.usequad, lsys, ctx->sysbits, sysbox);
     if (nsys)
         k_sysroot_cells<<<nblocks(cap), kTpb, 0, st>>>(clevel, cfirst, ks, lsys, ctx->sysbits, tt.ncellp,
