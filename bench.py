@@ -286,7 +286,11 @@ def main():
     ap.add_argument("--group", type=int, default=128)
     ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-tol", type=float, default=None,
+                    help="skip the learned calibration and use this tol (profiling runs)")
     ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
+    ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -322,11 +326,18 @@ def main():
     n = args.n
     eng = E.Engine(0)
     learned_train = None
-    if not args.no_learned:
+    if not args.no_learned and args.model:
+        with np.load(args.model) as z:
+            eng.mlp_set(z["params"], z["norm"])
+        learned_train = {"loaded_from": args.model}
+    elif not args.no_learned:
         learned_train = train_learned(eng, E, n, args.eps, epochs=args.epochs)
         log("learned: %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
             learned_train["samples"], learned_train["t_collect_s"], learned_train["t_train_s"],
             learned_train["loss_first"], learned_train["loss_last"]))
+        if args.save_model:
+            params, norm = eng.mlp_get()
+            np.savez(args.save_model, params=params, norm=norm)
     m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
     eng.upload(m, x)
     rng = np.random.default_rng(123)
@@ -363,7 +374,7 @@ def main():
     best_plain = min(plain, key=lambda v: v["ms_per_step"])
     learned = None
     if not args.no_learned:
-        learned = variant(dict(eps=args.eps, usequad=1, group_size=args.group, learned=1))
+        learned = variant(dict(eps=args.eps, usequad=1, group_size=args.group, learned=1), args.learned_tol)
         log("learned walk: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, %d evaluations)" % (
             learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
             learned["t_mlp_kernel_ms"], learned["n_mlp"]))
@@ -407,7 +418,7 @@ def main():
     }
     if learned and learned["t_mlp_kernel_ms"] > 0:
         a_mlp = FLOP_PER_MLP * learned["n_mlp"] / (learned["t_mlp_kernel_ms"] * 1e-3) / 1e12
-        out["roofline_mlp"] = {"bound": "mfma", "kernel": "k_learned_flat_guard", "achieved": a_mlp,
+        out["roofline_mlp"] = {"bound": "mfma", "kernel": "k_learned_reg", "achieved": a_mlp,
                                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a_mlp / BF16_PEAK_TFLOPS,
                                "traffic": None, "flop_convention": "18,304 flop per MLP evaluation"}
     if not args.no_cpu:
