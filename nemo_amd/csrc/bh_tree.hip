@@ -707,7 +707,9 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
         t = tn;
     }
     // runs of consecutive small children; a run also ends when the box spanned
-    // by its octants would cover a big child's octant
+    // by its octants would cover a big child's octant.  A child that does not
+    // fit is split: its first bodies fill the run to G and the rest start the
+    // next run (every group stays inside this cell, and groups are full).
     int t = s, run = 0, rs = s;
     unsigned rmask = 0;
     while (t <= e) {
@@ -719,15 +721,22 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
             if (run) emit(rs, run);
             run = 0;
             rmask = 0;
-        } else if (run + size > G || (run && (octant_box(rmask | o) & big))) {
+        } else if (run && (octant_box(rmask | o) & big)) {
             emit(rs, run);
             rs = t;
             run = size;
             rmask = o;
         } else {
             if (run == 0) rs = t;
-            run += size;
-            rmask |= o;
+            if (run + size > G) {
+                emit(rs, G);
+                rs += G;
+                run = run + size - G;
+                rmask = o;
+            } else {
+                run += size;
+                rmask |= o;
+            }
         }
         t = tn;
     }

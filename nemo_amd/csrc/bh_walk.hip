@@ -91,17 +91,19 @@ struct Lane {
     float axh[NPL], axl[NPL], ayh[NPL], ayl[NPL], azh[NPL], azl[NPL], ph[NPL], pl[NPL];
 };
 
-template <int NPL, bool SOFT0>
+// NQ <= NPL: the particle slots in use (a group of at most 64 points uses
+// one slot per lane).
+template <int NPL, int NQ, bool SOFT0>
 __device__ __forceinline__ void eval_p2p(Lane<NPL> &L, const float4 *src, int cnt, float eps2)
 {
     float sx[NPL], sy[NPL], sz[NPL], sp[NPL];
 #pragma unroll
-    for (int q = 0; q < NPL; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
+    for (int q = 0; q < NQ; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
 #pragma unroll 4
     for (int j = 0; j < cnt; j++) {
         const float4 s = src[j];
 #pragma unroll
-        for (int q = 0; q < NPL; q++) {
+        for (int q = 0; q < NQ; q++) {
             float dx = s.x - L.px[q], dy = s.y - L.py[q], dz = s.z - L.pz[q];
             float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             float ri = rsq(r2);
@@ -115,7 +117,7 @@ __device__ __forceinline__ void eval_p2p(Lane<NPL> &L, const float4 *src, int cn
         }
     }
 #pragma unroll
-    for (int q = 0; q < NPL; q++) {
+    for (int q = 0; q < NQ; q++) {
         fold(L.axh[q], L.axl[q], sx[q]);
         fold(L.ayh[q], L.ayl[q], sy[q]);
         fold(L.azh[q], L.azl[q], sz[q]);
@@ -125,13 +127,13 @@ __device__ __forceinline__ void eval_p2p(Lane<NPL> &L, const float4 *src, int cn
 
 // P2C entry: e[0] = (dx0, dy0, dz0, m) relative to the group centre,
 // e[1] = (qxx, qxy, qxz, qyy), e[2] = (qyz, qzz, -, -).
-template <int NPL, bool QUAD, bool SOFT0>
+template <int NPL, int NQ, bool QUAD, bool SOFT0>
 __device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cnt, float eps2)
 {
     constexpr int S = QUAD ? 3 : 1;
     float sx[NPL], sy[NPL], sz[NPL], sp[NPL];
 #pragma unroll
-    for (int q = 0; q < NPL; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
+    for (int q = 0; q < NQ; q++) sx[q] = sy[q] = sz[q] = sp[q] = 0.f;
 #pragma unroll 2
     for (int j = 0; j < cnt; j++) {
         const float4 s = src[S * j];
@@ -141,7 +143,7 @@ __device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cn
             q1 = src[S * j + 2];
         }
 #pragma unroll
-        for (int q = 0; q < NPL; q++) {
+        for (int q = 0; q < NQ; q++) {
             float dx = s.x - L.px[q], dy = s.y - L.py[q], dz = s.z - L.pz[q];
             float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
             float ri = rsq(r2);
@@ -170,7 +172,7 @@ __device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cn
         }
     }
 #pragma unroll
-    for (int q = 0; q < NPL; q++) {
+    for (int q = 0; q < NQ; q++) {
         fold(L.axh[q], L.axl[q], sx[q]);
         fold(L.ayh[q], L.ayl[q], sy[q]);
         fold(L.azh[q], L.azl[q], sz[q]);
@@ -237,6 +239,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         if (g >= (int)a.stat[S_NGROUPS]) break;
         const int64_t gbase = a.gfirst[g];
         const int gcount = a.gcount[g];
+        const bool one_slot = NPL > 1 && gcount <= 64;   // wave-uniform
         // opening and softening of this group's system
         float tolsq = a.tolsq, eps2 = a.eps2;
         int sys = -1;
@@ -413,7 +416,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 }
                 wave_lds_sync();
                 while (np >= 64) {
-                    eval_p2p<NPL, SOFT0>(L, P, 64, eps2);
+                    if (one_slot) eval_p2p<NPL, 1, SOFT0>(L, P, 64, eps2);
+                    else eval_p2p<NPL, NPL, SOFT0>(L, P, 64, eps2);
                     tot_pp += 64;
                     np -= 64;
                     wave_lds_sync();
@@ -423,7 +427,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     wave_lds_sync();
                 }
                 while (nc >= 64) {
-                    eval_p2c<NPL, QUAD, SOFT0>(L, Cl, 64, eps2);
+                    if (one_slot) eval_p2c<NPL, 1, QUAD, SOFT0>(L, Cl, 64, eps2);
+                    else eval_p2c<NPL, NPL, QUAD, SOFT0>(L, Cl, 64, eps2);
                     tot_pc += 64;
                     nc -= 64;
                     wave_lds_sync();
@@ -455,8 +460,13 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             continue;
         }
         wave_lds_sync();
-        if (np > 0) eval_p2p<NPL, SOFT0>(L, P, np, eps2);
-        if (nc > 0) eval_p2c<NPL, QUAD, SOFT0>(L, Cl, nc, eps2);
+        if (one_slot) {
+            if (np > 0) eval_p2p<NPL, 1, SOFT0>(L, P, np, eps2);
+            if (nc > 0) eval_p2c<NPL, 1, QUAD, SOFT0>(L, Cl, nc, eps2);
+        } else {
+            if (np > 0) eval_p2p<NPL, NPL, SOFT0>(L, P, np, eps2);
+            if (nc > 0) eval_p2c<NPL, NPL, QUAD, SOFT0>(L, Cl, nc, eps2);
+        }
         tot_pp += np;
         tot_pc += nc;
         wave_lds_sync();
