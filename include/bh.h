@@ -137,6 +137,13 @@ bh_status bh_lf_drift(bh_ctx *ctx);
 bh_status bh_lf_run(bh_ctx *ctx, int64_t nsteps, int use_graph, double *diag, int64_t *n2b, int64_t *nbc);
 /* Copy the current state to the host; any pointer may be NULL. */
 bh_status bh_lf_get(bh_ctx *ctx, double *pos, double *vel, double *acc, double *phi, int64_t *nstep);
+/* Root box of the last force pass (hackcode1 keeps it across steps). */
+bh_status bh_lf_box(bh_ctx *ctx, double *rmin, double *rsize);
+/* Resume a saved run (restorestate, code_io.c:258-284) after bh_lf_init:
+ * the accelerations of the saved state (the next force pass corrects the
+ * velocities with their change), the step count and the box (rmin NULL:
+ * keep the box logic of a fresh start).  Single systems only. */
+bh_status bh_lf_restore(bh_ctx *ctx, const double *acc, int64_t nstep, const double *rmin, double rsize);
 
 /* Batched ensembles: nsys independent systems in one tree build and one
  * persistent walk launch (the reference runs one hackforce process per

@@ -15,7 +15,6 @@
  * (error.c:100-103,171-180); an existing out= file is refused
  * (stropen.c:174-176).
  */
-#include <errno.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -24,26 +23,11 @@
 
 #include "../../include/bh.h"
 #include "filestruct.h"
+#include "getparam.h"
 
 #define VERSION "1.6b-mi355x"
 
-static const char *prog = "hackforce";
-
-static void fatal(const char *fmt, const char *a, const char *b)
-{
-    fprintf(stderr, "### Fatal error [%s]: ", prog);
-    fprintf(stderr, fmt, a ? a : "", b ? b : "");
-    fprintf(stderr, "\n");
-    exit(255);
-}
-
-/* keyword table in the order positional arguments are matched */
-typedef struct {
-    const char *key, *def, *help;
-    const char *val;
-} keyword;
-
-static keyword keys[] = {
+static gp_keyword keys[] = {
     {"in", NULL, "Input (snapshot) file with mass coordinates", NULL},
     {"out", NULL, "Output (snapshot) file with f.c. results", NULL},
     {"test", "", "Input file with test coordinates (if blank, in= coords are used)", NULL},
@@ -62,62 +46,16 @@ static keyword keys[] = {
 };
 #define NKEYS ((int)(sizeof keys / sizeof keys[0]))
 
-static keyword *find_key(const char *k, size_t len)
-{
-    for (int i = 0; i < NKEYS; i++)
-        if (strlen(keys[i].key) == len && !strncmp(keys[i].key, k, len)) return &keys[i];
-    return NULL;
-}
-
-static const char *param(const char *k)
-{
-    keyword *w = find_key(k, strlen(k));
-    if (w->val) return w->val;
-    if (!w->def) fatal("Parameter \"%s\" must be given a value", k, NULL);
-    return w->def;
-}
-
-static double dparam(const char *k)
-{
-    const char *s = param(k);
-    char *end;
-    errno = 0;
-    double v = strtod(s, &end);
-    while (*end == ' ') end++;
-    if (*s == 0 || *end || errno) fatal("Parameter %s=%s is not a number", k, s);
-    return v;
-}
-
-static int bparam(const char *k, int def)
-{
-    const char *s = param(k);
-    if (!*s) return def;
-    if (strchr("tTyY1", s[0])) return 1;
-    if (strchr("fFnN0", s[0])) return 0;
-    fatal("Parameter %s=%s is not a boolean", k, s);
-    return 0;
-}
-
-/* scanopt: is word an element of the comma/blank separated list? */
-static int scanopt(const char *list, const char *word)
-{
-    size_t n = strlen(word);
-    const char *p = list;
-    while (*p) {
-        while (*p == ',' || *p == ' ') p++;
-        const char *q = p;
-        while (*q && *q != ',' && *q != ' ') q++;
-        if ((size_t)(q - p) == n && !strncmp(p, word, n)) return 1;
-        p = q;
-    }
-    return 0;
-}
+#define fatal(...) gp_fatal(__VA_ARGS__)
+#define param gp_param
+#define dparam gp_dparam
+#define scanopt gp_scanopt
 
 static FILE *open_in(const char *name)
 {
     if (!strcmp(name, "-")) return stdin;
     FILE *f = fopen(name, "rb");
-    if (!f) fatal("stropen: cannot open file \"%s\" for reading", name, NULL);
+    if (!f) fatal("stropen: cannot open file \"%s\" for reading", name);
     return f;
 }
 
@@ -126,37 +64,7 @@ int main(int argc, char **argv)
     const char *base = strrchr(argv[0], '/');
     base = base ? base + 1 : argv[0];
     const int qp_name = strstr(base, "_qp") != NULL;
-    prog = "hackforce";
-    size_t cmdlen = 1;
-    for (int i = 0; i < argc; i++) cmdlen += strlen(argv[i]) + 1;
-    char *cmdline = (char *)calloc(cmdlen + 32, 1);
-    strcpy(cmdline, base);
-    int npos = 0, seen_kw = 0;
-    for (int i = 1; i < argc; i++) {
-        strcat(cmdline, " ");
-        strcat(cmdline, argv[i]);
-        if (!strcmp(argv[i], "help") || !strncmp(argv[i], "help=", 5) || !strcmp(argv[i], "--help")) {
-            for (int k = 0; k < NKEYS; k++)
-                printf("%s=%s\n    %s\n", keys[k].key, keys[k].def ? keys[k].def : "???", keys[k].help);
-            return 0;
-        }
-        const char *eq = strchr(argv[i], '=');
-        if (!eq) {
-            if (seen_kw || npos >= NKEYS) fatal("Positional argument \"%s\" after keywords", argv[i], NULL);
-            keys[npos++].val = argv[i];
-            continue;
-        }
-        seen_kw = 1;
-        keyword *w = find_key(argv[i], (size_t)(eq - argv[i]));
-        if (!w) {
-            /* NEMO system keywords that do not change the result */
-            if (!strncmp(argv[i], "debug=", 6) || !strncmp(argv[i], "error=", 6) || !strncmp(argv[i], "yapp=", 5))
-                continue;
-            fatal("Parameter \"%s\" unknown", argv[i], NULL);
-        }
-        w->val = eq + 1;
-    }
-
+    gp_init("hackforce", keys, NKEYS, argc, argv);
     const char *in = param("in"), *out = param("out"), *test = param("test");
     const char *options = param("options");
     bh_params p;
@@ -165,7 +73,7 @@ int main(int argc, char **argv)
     p.eps = dparam("eps");
     p.rsize = dparam("rsize");
     (void)dparam("fcells");
-    p.usequad = bparam("usequad", qp_name);
+    p.usequad = gp_bparam("usequad", qp_name);
     p.group_size = (int)dparam("group");
     p.nleaf = (int)dparam("nleaf");
     const char *mode = param("mode");
@@ -173,8 +81,8 @@ int main(int argc, char **argv)
     if (!strcmp(mode, "fast")) p.mode = BH_MODE_FAST;
     else if (!strcmp(mode, "strict")) p.mode = BH_MODE_STRICT;
     else if (!strcmp(mode, "direct")) direct = 1;
-    else fatal("Parameter mode=%s: fast, strict or direct", mode, NULL);
-    if (direct && *test) fatal("mode=direct evaluates the mass set only (no test=)", NULL, NULL);
+    else fatal("Parameter mode=%s: fast, strict or direct", mode);
+    if (direct && *test) fatal("mode=direct evaluates the mass set only (no test=)");
     {
         /* burststring(rmin, ", "): fewer than 3 values -> -rsize/2 */
         const char *s = param("rmin");
@@ -185,7 +93,7 @@ int main(int argc, char **argv)
             if (!*s) break;
             char *end;
             v[k] = strtod(s, &end);
-            if (end == s) fatal("Parameter rmin=%s is not a list of numbers", param("rmin"), NULL);
+            if (end == s) fatal("Parameter rmin=%s is not a list of numbers", param("rmin"));
             k++;
             s = end;
         }
@@ -196,7 +104,7 @@ int main(int argc, char **argv)
     }
     if (strcmp(out, "-")) {
         struct stat sb;
-        if (stat(out, &sb) == 0) fatal("stropen: file \"%s\" already exists", out, NULL);
+        if (stat(out, &sb) == 0) fatal("stropen: file \"%s\" already exists", out);
     }
 
     fs_reader rin = {open_in(in), -1, ""}, rtest = {NULL, -1, ""};
@@ -236,11 +144,11 @@ int main(int argc, char **argv)
         /* force_calc() */
         if (!ctx) {
             bh_status s = bh_create(&ctx, dev);
-            if (s != BH_OK) fatal("cannot open GPU device %s (bh_create failed)", param("device"), NULL);
+            if (s != BH_OK) fatal("cannot open GPU device %s (bh_create failed)", param("device"));
         }
         double *phi = (double *)malloc((size_t)nt * sizeof(double));
         double *acc = (double *)malloc((size_t)nt * 3 * sizeof(double));
-        if (!phi || !acc) fatal("out of memory", NULL, NULL);
+        if (!phi || !acc) fatal("out of memory");
         bh_stats stt;
         memset(&stt, 0, sizeof stt);
         bh_status s;
@@ -254,21 +162,23 @@ int main(int argc, char **argv)
             s = bh_force(ctx, &p, mass.n, mass.mass, mass.pos, rtest.f ? nt : 0, rtest.f ? ts->pos : NULL, phi, acc,
                          &stt);
         }
-        if (s != BH_OK) fatal("force calculation failed: %s", bh_last_error(ctx), NULL);
+        if (s != BH_OK) fatal("force calculation failed: %s", bh_last_error(ctx));
 
         /* out_result() */
         if (!fo) {
             if (!strcmp(out, "-")) fo = stdout;
             else {
                 struct stat sb;
-                if (stat(out, &sb) == 0) fatal("stropen: file \"%s\" already exists", out, NULL);
+                if (stat(out, &sb) == 0) fatal("stropen: file \"%s\" already exists", out);
                 fo = fopen(out, "wb");
-                if (!fo) fatal("stropen: cannot open file \"%s\" for writing", out, NULL);
+                if (!fo) fatal("stropen: cannot open file \"%s\" for writing", out);
             }
             const char *h = hist;
             for (int k = 0; k < nhist; k++, h += strlen(h) + 1) fs_put_history(fo, h);
-            strcat(cmdline, " VERSION=" VERSION);
+            char *cmdline = (char *)malloc(strlen(gp_cmdline()) + 32);
+            sprintf(cmdline, "%s VERSION=%s", gp_cmdline(), VERSION);
             fs_put_history(fo, cmdline);
+            free(cmdline);
         }
         const int n32 = (int)nt, cs = FS_CS_CARTESIAN_3D_PHASE;
         int n2b = stt.n2btot > INT32_MAX ? INT32_MAX : (int)stt.n2btot;
@@ -292,7 +202,7 @@ int main(int argc, char **argv)
         }
         if (scanopt(options, "phase")) {
             double *ps = (double *)malloc((size_t)nt * 6 * sizeof(double));
-            if (!ps) fatal("out of memory", NULL, NULL);
+            if (!ps) fatal("out of memory");
             for (int64_t i = 0; i < nt; i++)
                 for (int k = 0; k < 3; k++) {
                     ps[6 * i + k] = ts->pos[3 * i + k];
@@ -311,16 +221,15 @@ int main(int argc, char **argv)
         werr |= fs_put_data(fo, "cpufcal", "d", &cpufcal, 0, NULL);
         werr |= fs_put_tes(fo);
         werr |= fs_put_tes(fo);
-        if (werr) fatal("write error on %s", out, NULL);
+        if (werr) fatal("write error on %s", out);
         free(phi);
         free(acc);
     }
-    if (fo && fo != stdout && fclose(fo)) fatal("write error on %s", out, NULL);
+    if (fo && fo != stdout && fclose(fo)) fatal("write error on %s", out);
     if (fo == stdout) fflush(stdout);
     fs_free_snapshot(&mass);
     fs_free_snapshot(&tst);
     free(hist);
-    free(cmdline);
     if (ctx) bh_destroy(ctx);
     return 0;
 }

@@ -688,6 +688,42 @@ bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best
     });
 }
 
+bh_status bh_lf_box(bh_ctx *ctx, double *rmin, double *rsize)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_box before bh_lf_init");
+        if (!ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no force pass yet");
+        if (rmin)
+            for (int k = 0; k < 3; k++) rmin[k] = ctx->box[k];
+        if (rsize) *rsize = ctx->box[3];
+    });
+}
+
+bh_status bh_lf_restore(bh_ctx *ctx, const double *acc, int64_t nstep, const double *rmin, double rsize)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->lf_ready) throw ApiFail(BH_ERR_STATE, "bh_lf_restore before bh_lf_init");
+        if (ctx->nsys) throw ApiFail(BH_ERR_ARG, "bh_lf_restore handles single systems");
+        if (nstep < 0 || nstep > INT32_MAX) throw ApiFail(BH_ERR_ARG, "bad nstep");
+        if (rmin && !(rsize > 0.0 && std::isfinite(rsize) && std::isfinite(rmin[0]) && std::isfinite(rmin[1]) &&
+                      std::isfinite(rmin[2])))
+            throw ApiFail(BH_ERR_ARG, "bad box");
+        lf_drop_graph(ctx);
+        const int64_t n = ctx->nmass;
+        if (acc)
+            BH_HIP(hipMemcpy(ctx->acc_old.p, acc, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        const int ns = (int)nstep;
+        BH_HIP(hipMemcpy(ctx->lfstep.p, &ns, sizeof ns, hipMemcpyHostToDevice));
+        ctx->lf_nstep = nstep;
+        if (rmin) {
+            const double b[4] = {rmin[0], rmin[1], rmin[2], rsize};
+            BH_HIP(hipMemcpy(bh::ensure<double4>(ctx->dbox, 1), b, sizeof b, hipMemcpyHostToDevice));
+            for (int k = 0; k < 4; k++) ctx->box[k] = b[k];
+            ctx->box_valid = true;
+        }
+    });
+}
+
 // ---- learned correction ----------------------------------------------------------
 
 bh_status bh_mlp_init(bh_ctx *ctx, uint64_t seed)

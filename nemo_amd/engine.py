@@ -82,6 +82,8 @@ SIGNATURES = {
     "bh_lf_drift": (C.c_int, [_vp]),
     "bh_lf_run": (C.c_int, [_vp, C.c_int64, C.c_int, _d, _i64, _i64]),
     "bh_lf_get": (C.c_int, [_vp, _d, _d, _d, _d, _i64]),
+    "bh_lf_box": (C.c_int, [_vp, _d, _d]),
+    "bh_lf_restore": (C.c_int, [_vp, _d, C.c_int64, _d, C.c_double]),
     "bh_mlp_init": (C.c_int, [_vp, C.c_uint64]),
     "bh_mlp_set": (C.c_int, [_vp, _f, _f]),
     "bh_mlp_get": (C.c_int, [_vp, _f, _f]),

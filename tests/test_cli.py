@@ -198,3 +198,72 @@ def test_hackforce_fast_and_direct_modes(tmp_path):
     np.testing.assert_allclose(da[idx], oacc, rtol=1e-12, atol=1e-12)
     e = np.linalg.norm(fa - da, axis=1) / np.linalg.norm(da, axis=1)
     assert np.median(e) < 1e-3 and np.percentile(e, 99) < 1e-2
+
+
+# ---- hackcode1 -------------------------------------------------------------------------
+
+def test_hackcode1_help_and_argument_errors(tmp_path):
+    r = run("hackcode1", "help")
+    assert r.returncode == 0
+    for k in (b"in=", b"out=", b"restart=", b"continue=", b"save=", b"nbody=128", b"freq=32.0", b"eps=0.05",
+              b"tol=1.0", b"options=mass,phase", b"tstop=2.0", b"freqout=4.0", b"minor_freqout=32.0", b"log=-"):
+        assert k in r.stdout
+    (tmp_path / "exists").write_bytes(b"x")
+    for args, msg in ((["out=exists"], b"already exists"), (["bogus=1"], b"unknown"),
+                      (["continue=/nonexistent/state"], b"cannot open"), (["in=/nonexistent/snap"], b"cannot open")):
+        r = run("hackcode1", *args, cwd=str(tmp_path))
+        assert r.returncode == 255 and b"### Fatal error [hackcode1]" in r.stderr and msg in r.stderr, (args, r.stderr)
+
+
+def _frames(path):
+    """(time, pos, vel) of the frames that hold particles."""
+    out = []
+    for it in snapshot.parse(open(path, "rb").read()):
+        if it.tag != "SnapShot":
+            continue
+        t = float(it.find("Parameters").find("Time").data.reshape(-1)[0])
+        prt = it.find("Particles")
+        if prt is not None and prt.find("PhaseSpace") is not None:
+            ps = prt.find("PhaseSpace").data
+            out.append((t, ps[:, 0, :], ps[:, 1, :]))
+    return out
+
+
+@gpu
+def test_hackcode1_default_run_reproduces_reference_kats(golden, tmp_path):
+    """The 64-step default run from the reference's t=0 frame, strict mode:
+    the output's bsf summary equals the reference Testfile KAT and the log's
+    interaction totals equal the published 3864 (t=0) and 4899 (t=2)."""
+    import json
+
+    kats = json.load(open(os.path.join(golden, "kats.json")))
+    src = os.path.join(golden, "hack128.sun4")
+    r = run("hackcode1", "in=" + src, "out=run.out", "mode=strict", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert snapshot.bsf(open(str(tmp_path / "run.out"), "rb").read()) == kats["hackcode1_default_bsf"]["value"]
+    rows = [l.split() for l in r.stdout.decode().splitlines()]
+    data = [x for x in rows if len(x) == 7 and x[0].replace(".", "").isdigit()]
+    assert len(data) == 65
+    assert int(data[0][3]) == kats["hackcode1_t0"]["nttot"] and int(data[-1][3]) == kats["hackcode1_t2"]["nttot"]
+    assert b"particle data written" in r.stdout
+
+
+@gpu
+def test_hackcode1_save_and_restart_continue_bitwise(golden, tmp_path):
+    """A run saved at t=1 and restarted to t=2 ends bit-identical to a
+    straight run to t=2 (strict and fast walk)."""
+    src = os.path.join(golden, "hack128.sun4")
+    for mode in ("strict", "fast"):
+        d = tmp_path / mode
+        d.mkdir()
+        assert run("hackcode1", "in=" + src, "out=a", "mode=" + mode, "usequad=t", "tol=0.7",
+                   cwd=str(d)).returncode == 0
+        r = run("hackcode1", "in=" + src, "out=b1", "tstop=1", "save=state", "mode=" + mode, "usequad=t", "tol=0.7",
+                cwd=str(d))
+        assert r.returncode == 0, r.stderr
+        r = run("hackcode1", "restart=state", "out=b2", "tstop=2", "mode=" + mode, "usequad=t", "tol=0.7",
+                cwd=str(d))
+        assert r.returncode == 0, r.stderr
+        fa, fb = _frames(str(d / "a")), _frames(str(d / "b2"))
+        assert fa[-1][0] == fb[-1][0] == 2.0
+        assert np.array_equal(fa[-1][1], fb[-1][1]) and np.array_equal(fa[-1][2], fb[-1][2]), mode
