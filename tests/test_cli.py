@@ -267,3 +267,66 @@ def test_hackcode1_save_and_restart_continue_bitwise(golden, tmp_path):
         fa, fb = _frames(str(d / "a")), _frames(str(d / "b2"))
         assert fa[-1][0] == fb[-1][0] == 2.0
         assert np.array_equal(fa[-1][1], fb[-1][1]) and np.array_equal(fa[-1][2], fb[-1][2]), mode
+
+
+# ---- acceleration(5) plugin ------------------------------------------------------------
+
+def _plugin():
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(BIN, "bhtree.so"))
+    ACC = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                      C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_int, C.c_char)
+    lib.iniacceleration.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_char_p, C.POINTER(ACC),
+                                    C.POINTER(C.c_bool), C.POINTER(C.c_bool)]
+    lib.iniacceleration.restype = None
+    return lib, ACC
+
+
+def test_plugin_exports_iniacceleration():
+    import ctypes as C
+
+    lib, ACC = _plugin()
+    pars = (C.c_double * 3)(0.7, 0.02, 1.0)
+    f = ACC()
+    nm, nv = C.c_bool(False), C.c_bool(True)
+    lib.iniacceleration(pars, 3, None, C.byref(f), C.byref(nm), C.byref(nv))
+    assert nm.value is True and nv.value is False and bool(f)
+
+
+@gpu
+def test_plugin_matches_bh_force():
+    """acceleration(5) through bhtree.so: double and float arrays, active
+    flags and the add/assign indicator, against bh_force."""
+    import ctypes as C
+
+    from nemo_amd import engine as E
+
+    lib, ACC = _plugin()
+    pars = (C.c_double * 3)(0.7, 0.02, 1.0)
+    f = ACC()
+    nm, nv = C.c_bool(), C.c_bool()
+    lib.iniacceleration(pars, 3, None, C.byref(f), C.byref(nm), C.byref(nv))
+    m, x, _ = E.mkplummer(5000, seed=4)
+    eng = E.Engine(0)
+    phi0, acc0, _ = eng.force(m, x, E.default_params(tol=0.7, eps=0.02, usequad=1))
+    eng.close()
+    pot = np.zeros(5000)
+    acc = np.zeros((5000, 3))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    f(3, 0.0, 5000, vp(m), vp(x), None, None, vp(pot), vp(acc), 0, b"d")
+    assert np.array_equal(pot, phi0) and np.array_equal(acc, acc0)
+    # add to existing values, only the active bodies
+    flags = (np.arange(5000) % 3 == 0).astype(np.int32)
+    pot2 = np.ones(5000)
+    acc2 = np.ones((5000, 3))
+    f(3, 0.0, 5000, vp(m), vp(x), None, flags.ctypes.data_as(C.POINTER(C.c_int)), vp(pot2), vp(acc2), 3, b"d")
+    on = flags == 1
+    assert np.array_equal(pot2[on], 1.0 + phi0[on]) and np.all(pot2[~on] == 1.0)
+    assert np.array_equal(acc2[on], 1.0 + acc0[on]) and np.all(acc2[~on] == 1.0)
+    # float arrays
+    mf, xf = m.astype(np.float32), x.astype(np.float32)
+    potf = np.zeros(5000, np.float32)
+    accf = np.zeros((5000, 3), np.float32)
+    f(3, 0.0, 5000, vp(mf), vp(xf), None, None, vp(potf), vp(accf), 0, b"f")
+    np.testing.assert_allclose(accf, acc0, rtol=1e-3, atol=1e-3 * np.abs(acc0).max())
