@@ -222,7 +222,7 @@ def run_ensemble(args, E, log):
                                 "ms_per_step": t_lf / args.lf_steps * 1e3,
                                 "interactions_per_s": float((n2b + nbc).sum() / t_lf),
                                 "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}},
-        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false>" % (2 if best.group_size == 128 else 1)),
+        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false,false>" % (2 if best.group_size == 128 else 1)),
     }
 
 
@@ -260,7 +260,7 @@ def run_diskhalo(args, E, log):
                    "relative_energy_drift": float(drift), "energy_first_last": [float(d[0, 0]), float(d[-1, 0])],
                    "first_eval": {"median_rel_err": float(np.median(e)), "p99_rel_err": float(np.percentile(e, 99)),
                                   "t_tree_ms": st.t_tree_ms, "t_walk_kernel_ms": st.t_walk_kernel_ms}},
-        "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3, "k_walk<2,true,false,false>"),
+        "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3, "k_walk<2,true,false,false,false>"),
     }
 
 
@@ -406,8 +406,9 @@ def main():
         "roofline": {
             "bound": "mfma",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
-            "kernel": "k_walk<%d,%s,false,%s>" % (2 if args.group == 128 else 1, "true" if head_v["usequad"] else "false",
-                                                 "true" if head_v["learned"] else "false"),
+            "kernel": "k_walk<%d,%s,false,%s,false>" % (2 if args.group == 128 else 1,
+                                                       "true" if head_v["usequad"] else "false",
+                                                       "true" if head_v["learned"] else "false"),
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",

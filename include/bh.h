@@ -48,6 +48,24 @@ typedef enum {
     BH_MODE_DIRECT = 2   /* FP64 direct sum over all massive bodies (tol -> 0 limit)              */
 } bh_mode;
 
+/* Cell opening criterion.  BH_CRIT_HACK is hackcode1's (grav.c:122-131): a
+ * cell opens when tol^2 |com - x|^2 < size^2 for the body x.  The others are
+ * treecode1's (treeload.c:232-252, treegrav.c:137-156): each cell gets a
+ * critical radius Rcrit2 (tol = theta; tol = 0 opens everything), and a cell
+ * c of side s is accepted for body x only when the squared distance from c's
+ * c.o.m. to the side-s cube that holds x exceeds Rcrit2 and that cube is not
+ * adjacent (some |com - centre| component > 1.5 s).  Strict mode applies it
+ * per body exactly; fast mode applies it per group to the union of the
+ * side-s cubes that hold the group's points (never accepts a cell that a
+ * body of the group would open).  No counterpart in hackcode1: treecode1
+ * selects them with options=bh86 / sw94 (treeload.c:51-54). */
+typedef enum {
+    BH_CRIT_HACK = 0,    /* hackcode1: (size/tol)^2 against the body's distance to the c.o.m. */
+    BH_CRIT_RCRIT = 1,   /* treecode1 default: Rcrit = size/theta + |com - centre|           */
+    BH_CRIT_BH86 = 2,    /* treecode1 options=bh86: Rcrit = size/theta                       */
+    BH_CRIT_SW94 = 3     /* treecode1 options=sw94: Rcrit = max corner distance from com / theta */
+} bh_criterion;
+
 typedef struct {
     double tol;          /* opening parameter: open iff tol^2 |com-x|^2 < size^2 (grav.c:122-131) */
     double eps;          /* Plummer softening */
@@ -61,6 +79,7 @@ typedef struct {
     int keep_box;        /* bh_eval: start from the box of the previous call (hackcode1 semantics) */
     int nleaf;           /* fast mode: a cell holding <= nleaf bodies interacts body by body (exact);
                             0 keeps the reference's cell interactions; -1 = tuned default */
+    int criterion;       /* bh_criterion (default BH_CRIT_HACK); not with learned or batches */
 } bh_params;
 
 typedef struct {

@@ -7,7 +7,8 @@
  * "bhtree.so" in $ACCPATH / $POTPATH / $NEMO/obj/potential and calls
  * iniacceleration(pars, npar, file, &acc, &need_mass, &need_vel).
  * pars: tol (default 1.0), eps (0.05), usequad (0), as hackforce's
- * defaults; file is not used.  acc(ndim, t, n, m, x, v, f, pot, a,
+ * defaults, and criterion (0 hackcode1's, 1-3 treecode1's default / bh86 /
+ * sw94, bh_criterion); file is not used.  acc(ndim, t, n, m, x, v, f, pot, a,
  * indicator, type): all n bodies are the mass set, each body skips itself;
  * with flags f only bodies with f[i] & 1 receive pot/acc; indicator bit 0
  * adds the potential, bit 1 adds the acceleration (otherwise assigned);
@@ -101,7 +102,9 @@ void iniacceleration(const double *pars, int npar, const char *file, acc_pter *a
     if (npar > 0) g_par.tol = pars[0];
     if (npar > 1) g_par.eps = pars[1];
     if (npar > 2) g_par.usequad = pars[2] != 0.0;
-    if (npar > 3) fprintf(stderr, "bhtree: parameters beyond (tol, eps, usequad) are ignored\n");
+    if (npar > 3) g_par.criterion = (int)pars[3];
+    if (npar > 4) fprintf(stderr, "bhtree: parameters beyond (tol, eps, usequad, criterion) are ignored\n");
+    if (g_par.criterion < BH_CRIT_HACK || g_par.criterion > BH_CRIT_SW94) die("criterion must be 0..3", NULL);
     if (!(g_par.tol >= 0.0) || !(g_par.eps >= 0.0)) die("tol and eps must be >= 0", NULL);
     if (accf) *accf = bh_acceleration;
     if (need_mass) *need_mass = true;

@@ -109,6 +109,14 @@ int gp_iparam(const char *key)
     return (int)v;
 }
 
+int gp_choice(const char *key, const char *const *names, int n)
+{
+    const char *s = gp_param(key);
+    for (int i = 0; i < n; i++)
+        if (!strcmp(s, names[i])) return i;
+    gp_fatal("Parameter %s=%s is not one of the allowed values", key, s);
+}
+
 int gp_bparam(const char *key, int dflt)
 {
     const char *s = gp_param(key);

@@ -19,6 +19,7 @@ double gp_dparam(const char *key);
 int gp_iparam(const char *key);
 int gp_bparam(const char *key, int dflt);   /* blank -> dflt */
 int gp_given(const char *key);              /* set on the command line */
+int gp_choice(const char *key, const char *const *names, int n);   /* index of the value in names */
 int gp_scanopt(const char *list, const char *word);
 const char *gp_cmdline(void);                /* the invocation, for History */
 void gp_fatal(const char *fmt, ...) __attribute__((format(printf, 1, 2), noreturn));

@@ -18,7 +18,8 @@
  * Plummer sphere of nbody bodies from seed (the reference draws its own
  * test model with its generator; cencon= is accepted).  Keywords added:
  * usequad= (default: true when invoked as hackcode1_qp), mode=fast|strict,
- * group=, nleaf=, device=.  fcells= is accepted and ignored.
+ * group=, nleaf=, criterion=hack|rcrit|bh86|sw94 (treecode1's opening tests),
+ * device=.  fcells= is accepted and ignored.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -56,10 +57,12 @@ static gp_keyword keys[] = {
     {"mode", "fast", "fast (FP32 group walk) or strict (FP64, reference order)", NULL},
     {"group", "0", "particles per group in fast mode (0 = default, 64 or 128)", NULL},
     {"nleaf", "-1", "cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
+    {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},
     {"VERSION", VERSION, "", NULL},
 };
 #define NKEYS ((int)(sizeof keys / sizeof keys[0]))
+static const char *const criteria[] = {"hack", "rcrit", "bh86", "sw94"};   /* bh_criterion order */
 
 /* run state (code.h globals) */
 static double freq, eps, tol, fcells, tstop, freqout, minor_freqout, tnow, tout, minor_tout;
@@ -400,6 +403,7 @@ int main(int argc, char **argv)
     else gp_fatal("Parameter mode=%s: fast or strict", mode);
     p.group_size = gp_iparam("group");
     p.nleaf = gp_iparam("nleaf");
+    p.criterion = gp_choice("criterion", criteria, 4);
     p.rsize = rsize;
     p.rmin_given = 1;
     for (int k = 0; k < 3; k++) p.rmin[k] = rmin[k];

@@ -9,7 +9,8 @@
  * hackforce_qp (Makefile:86-91 of the reference) it uses quadrupole cells.
  *
  * Keywords added by this build: usequad= (default: by program name),
- * mode=fast|strict|direct, group=, nleaf=, device=.  fcells= is accepted and
+ * mode=fast|strict|direct, group=, nleaf=, criterion=hack|rcrit|bh86|sw94
+ * (bh_criterion: treecode1's opening tests, treeload.c:232-252), device=.  fcells= is accepted and
  * ignored: the device tree sizes itself.  Errors print
  * "### Fatal error [hackforce]: ..." and exit with status 255
  * (error.c:100-103,171-180); an existing out= file is refused
@@ -41,10 +42,13 @@ static gp_keyword keys[] = {
     {"mode", "fast", "fast (FP32 group walk), strict (FP64, reference order), direct (FP64 direct sum)", NULL},
     {"group", "0", "Particles per group in fast mode (0 = default, 64 or 128)", NULL},
     {"nleaf", "-1", "Cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
+    {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},
     {"VERSION", VERSION, "", NULL},
 };
 #define NKEYS ((int)(sizeof keys / sizeof keys[0]))
+
+static const char *const criteria[] = {"hack", "rcrit", "bh86", "sw94"};   /* bh_criterion order */
 
 #define fatal(...) gp_fatal(__VA_ARGS__)
 #define param gp_param
@@ -76,6 +80,7 @@ int main(int argc, char **argv)
     p.usequad = gp_bparam("usequad", qp_name);
     p.group_size = (int)dparam("group");
     p.nleaf = (int)dparam("nleaf");
+    p.criterion = gp_choice("criterion", criteria, 4);
     const char *mode = param("mode");
     int direct = 0;
     if (!strcmp(mode, "fast")) p.mode = BH_MODE_FAST;

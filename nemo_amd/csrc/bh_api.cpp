@@ -57,6 +57,9 @@ void check_params(const bh_params *p)
     if (p->rmin_given)
         for (int k = 0; k < 3; k++)
             if (!std::isfinite(p->rmin[k])) throw ApiFail(BH_ERR_ARG, "rmin must be finite");
+    if (p->criterion < BH_CRIT_HACK || p->criterion > BH_CRIT_SW94) throw ApiFail(BH_ERR_ARG, "unknown criterion");
+    if (p->criterion != BH_CRIT_HACK && p->learned)
+        throw ApiFail(BH_ERR_ARG, "the learned correction is trained for the hackcode1 criterion only");
 }
 
 void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, int64_t ntest, const double *tpos)
@@ -102,6 +105,8 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     if (q.mode == BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
     if (ctx->nsys && (q.mode != BH_MODE_FAST || q.learned))
         throw ApiFail(BH_ERR_ARG, "batches are evaluated by the fast walk without the learned correction");
+    if (ctx->nsys && q.criterion != BH_CRIT_HACK)
+        throw ApiFail(BH_ERR_ARG, "batches use the hackcode1 criterion");
     if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
     unsigned long long stat[bh::S_NSLOTS];
     double box[4];
@@ -265,6 +270,7 @@ void bh_default_params(bh_params *p)
     p->learned = 0;
     p->keep_box = 0;
     p->nleaf = -1;
+    p->criterion = BH_CRIT_HACK;
 }
 
 const char *bh_version(void) { return "nemo_amd-bh 0.1 (gfx950)"; }
@@ -304,8 +310,8 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->tkeys1, &ctx->tvals0, &ctx->tvals1, &ctx->nmassive, &ctx->bd,    &ctx->bhi,
                           &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself,
                           &ctx->cpl,    &ctx->ccnt,   &ctx->cbase,  &ctx->cmin1,   &ctx->cmin2,  &ctx->cfirst,
-                          &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->cnkc,   &ctx->cparent,
-                          &ctx->cctr,   &ctx->cskip,  &ctx->pre,    &ctx->cmo,
+                          &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->crc2,
+                          &ctx->cskip,  &ctx->pre,    &ctx->cmo,
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
                           &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
@@ -642,6 +648,7 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
         if (!vel) throw ApiFail(BH_ERR_ARG, "vel is NULL");
         if (!(dt > 0.0) || !std::isfinite(dt)) throw ApiFail(BH_ERR_ARG, "dt must be finite and > 0");
         if (p->mode != BH_MODE_FAST || p->learned) throw ApiFail(BH_ERR_ARG, "batches use the fast walk");
+        if (p->criterion != BH_CRIT_HACK) throw ApiFail(BH_ERR_ARG, "batches use the hackcode1 criterion");
         lf_drop_graph(ctx);
         batch_upload(ctx, nsys, off, mass, pos, eps, tol);
         lf_upload_state(ctx, off[nsys], vel);
@@ -816,6 +823,8 @@ bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo
         check_params(p);
         if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_mlp_collect before bh_upload");
         if (!(rlo >= 0.0 && rhi > rlo)) throw ApiFail(BH_ERR_ARG, "bad size/distance range");
+        if (p->criterion != BH_CRIT_HACK)
+            throw ApiFail(BH_ERR_ARG, "training samples are collected with the hackcode1 criterion");
         ctx->mlp.norm[1] = (float)rlo;
         ctx->mlp.norm[2] = (float)rhi;
         bh_params q = *p;

@@ -48,7 +48,8 @@ struct bh_ctx {
     bh::DevBuf thi, tlo, td, tperm, tself;
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
-    bh::DevBuf cfirst, clast, clevel, csub, cnkids, cnkc, cparent, cctr, cskip, pre, cowner, gowner;
+    bh::DevBuf cfirst, clast, clevel, csub, cnkids, cskip, pre, cowner, gowner;
+    bh::DevBuf crc2;            // treecode1 Rcrit2 per cell (criterion != BH_CRIT_HACK)
     bh::DevBuf cmo, cfnb, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
@@ -136,6 +137,8 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
                       int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout);
 // bh_exact.hip
 void walk_strict(bh_ctx *ctx, const bh_params &p);
+double *cell_rcrit(bh_ctx *ctx, const bh_params &p, const uint64_t *keys, const int *cfirst, const uint8_t *level,
+                   const double *mo, const unsigned long long *ncellp, int cap, const double4 *box, float4 *cB);
 void direct_sum(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *d_idx, double *d_phi, double *d_acc);
 // bh_walk.hip
 void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode = 0, int lstride = 1);

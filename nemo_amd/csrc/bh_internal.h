@@ -45,7 +45,8 @@ struct Tree {
     const int *pre = nullptr;       // pre-order node refs (nb + ncell entries)
     const double *mo = nullptr;     // FP64 moments, 10 per cell: m, x, y, z, qxx qxy qxz qyy qyz qzz
     const float4 *cA = nullptr;     // FP32 com hi xyz, mass
-    const float4 *cB = nullptr;     // FP32 com lo xyz, size^2
+    const float4 *cB = nullptr;     // FP32 com lo xyz, size^2 (treecode1 criteria: Rcrit2)
+    const double *rc2 = nullptr;    // treecode1 criteria: FP64 Rcrit2 per cell
     const float4 *cQ0 = nullptr;    // qxx qxy qxz qyy
     const float4 *cQ1 = nullptr;    // qyz qzz 0 0
     // test set in its sorted order

@@ -70,12 +70,43 @@ __device__ __forceinline__ bool fits(const double *x, const double4 b)
     return (0.0 <= r0 && r0 < 1.0) && (0.0 <= r1 && r1 < 1.0) && (0.0 <= r2 && r2 < 1.0);
 }
 
+// The same test for many bodies against one box.  When the side is a power
+// of two, d * (1/side) is the correctly rounded d / side (both round the same
+// real number), so the three divisions become multiplications with identical
+// results; any other side divides.
+struct BoxTest {
+    double4 b;
+    double inv;
+    bool p2;
+};
+
+__device__ __forceinline__ BoxTest box_test(const double4 b)
+{
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(b.w);
+    const unsigned e = (unsigned)((bits >> 52) & 0x7ff);
+    const bool p2 = (bits & 0x000FFFFFFFFFFFFFULL) == 0 && e - 1u < 0x7feu;
+    return {b, p2 ? 1.0 / b.w : 0.0, p2};
+}
+
+__device__ __forceinline__ bool fits(const double *x, const BoxTest &t)
+{
+    if (!t.p2) return fits(x, t.b);
+    const double r0 = (x[0] - t.b.x) * t.inv, r1 = (x[1] - t.b.y) * t.inv, r2 = (x[2] - t.b.z) * t.inv;
+    return (0.0 <= r0 && r0 < 1.0) && (0.0 <= r1 && r1 < 1.0) && (0.0 <= r2 && r2 < 1.0);
+}
+
+__device__ __forceinline__ bool fits(const double4 c, const BoxTest &t)
+{
+    const double x[3] = {c.x, c.y, c.z};
+    return fits(x, t);
+}
+
 // Massive bodies outside the starting box (the only ones that can ever make
 // expandbox grow it, since the box only grows), compacted in input order:
 // per-block counts, a scan, then per-block writes.  Block b owns the input
 // range [b*chunk, (b+1)*chunk).
 __device__ __forceinline__ bool outside_massive(const double *__restrict__ pos, const double *__restrict__ mass,
-                                                int64_t i, int64_t n, const double4 b, int *bad)
+                                                int64_t i, int64_t n, const BoxTest &b, int *bad)
 {
     if (i >= n || mass[i] == 0.0) return false;
     const double *x = pos + 3 * i;
@@ -84,78 +115,43 @@ __device__ __forceinline__ bool outside_massive(const double *__restrict__ pos, 
     return true;
 }
 
-__global__ void k_box_count(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                            const double4 *__restrict__ box, int64_t chunk, uint32_t *bcount,
-                            unsigned long long *stat)
+// Compaction in any order (the rounds take the lowest index).  Block b owns
+// bodies [b*kBoxChunk, (b+1)*kBoxChunk); in step k thread t tests body
+// b*kBoxChunk + k*kTpb + t (coalesced) into bit k of a mask, a block scan of
+// the counts gives the slots and one atomic per block (boxi[4]) reserves the
+// block's range.  A candidate is stored with its position (x, y, z, index),
+// so the rounds read it contiguously.
+constexpr int kBoxPer = 16, kBoxChunk = kTpb * kBoxPer;
+
+__global__ __launch_bounds__(kTpb) void k_box_compact(const double *__restrict__ pos, const double *__restrict__ mass,
+                                                      int64_t n, const double4 *__restrict__ box, double4 *cand,
+                                                      int *boxi, unsigned long long *stat)
 {
-    const double4 b = *box;
-    const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-    int cnt = 0, bad = 0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) cnt += outside_massive(pos, mass, i, hi, b, &bad);
+    __shared__ uint32_t base;
+    const BoxTest b = box_test(*box);
+    const int64_t i0 = (int64_t)blockIdx.x * kBoxChunk + threadIdx.x;
+    int bad = 0;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < kBoxPer; k++)
+        if (outside_massive(pos, mass, i0 + k * kTpb, n, b, &bad)) mask |= 1u << k;
     uint32_t tot;
-    block_excl_scan((uint32_t)cnt, &tot);
-    if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
+    uint32_t off = block_excl_scan((uint32_t)__popc(mask), &tot);
+    if (threadIdx.x == 0 && tot) base = atomicAdd(reinterpret_cast<uint32_t *>(&boxi[4]), tot);
+    __syncthreads();
+    while (mask) {
+        const int k = __ffs(mask) - 1;
+        mask &= mask - 1;
+        const int64_t i = i0 + k * kTpb;
+        const double *x = pos + 3 * i;
+        cand[base + off++] = make_double4(x[0], x[1], x[2], (double)i);
+    }
     if (bad) atomicOr(&stat[S_FLAGS], (unsigned long long)F_NONFINITE);
 }
 
-__global__ void k_box_write(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                            const double4 *__restrict__ box, int64_t chunk, const uint32_t *__restrict__ boff,
-                            int *cand)
+// Doubling of expandbox (load.c:68-95) until body x fits.
+__device__ __forceinline__ double4 expand_for(const double *x, double4 b)
 {
-    const double4 b = *box;
-    const int64_t lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-    uint32_t run = boff[blockIdx.x];
-    int bad = 0;
-    for (int64_t i0 = lo; i0 < hi; i0 += blockDim.x) {
-        const int64_t i = i0 + threadIdx.x;
-        const bool out = outside_massive(pos, mass, i, hi, b, &bad);
-        uint32_t tot;
-        const uint32_t off = block_excl_scan(out ? 1u : 0u, &tot);
-        if (out) cand[run + off] = (int)i;
-        run += tot;
-    }
-}
-
-// First candidate (lowest input index >= start) outside the current box.
-__global__ void k_box_first_c(const double *__restrict__ pos, const int *__restrict__ cand,
-                              const uint32_t *__restrict__ ncand, const double4 *__restrict__ box, int *boxi)
-{
-    const double4 b = *box;
-    const int start = boxi[1], nc = (int)*ncand;
-    int best = INT32_MAX;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
-        int i = cand[k];
-        if (i >= start && i < best && !fits(pos + 3 * (int64_t)i, b)) best = i;
-    }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
-    if (lane_id() == 0 && best != INT32_MAX) atomicMin(&boxi[0], best);
-}
-
-// Starting box: the caller's (rmin, rsize), or the previous box when keep.
-// boxi: [0] first outside candidate (min), [1] start index, [2] expansions.
-__global__ void k_box_init(double4 *box, int *boxi, double rx, double ry, double rz, double rsize, int keep)
-{
-    if (!keep) *box = make_double4(rx, ry, rz, rsize);
-    boxi[0] = INT32_MAX;
-    boxi[1] = 0;
-    boxi[2] = 0;
-}
-
-// After the expansion rounds: flag a candidate that still lies outside.
-__global__ void k_box_check(const int *boxi, unsigned long long *stat)
-{
-    if (boxi[0] != INT32_MAX) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
-}
-
-// Expand the box for the body found by k_box_first (single thread).
-__global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *boxi)
-{
-    int first = boxi[0];
-    if (first == INT32_MAX) return;
-    boxi[2] += 1;
-    const double *x = pos + 3 * (int64_t)first;
-    double4 b = *box;
     for (int guard = 0; guard < 2000 && !fits(x, b); guard++) {
         double rmid0 = b.x + 0.5 * b.w, rmid1 = b.y + 0.5 * b.w, rmid2 = b.z + 0.5 * b.w;
         if (x[0] < rmid0) b.x -= b.w;
@@ -163,9 +159,74 @@ __global__ void k_box_expand(const double *__restrict__ pos, double4 *box, int *
         if (x[2] < rmid2) b.z -= b.w;
         b.w = 2.0 * b.w;
     }
-    *box = b;
+    return b;
+}
+
+// One expansion round: every block reduces the lowest candidate index >=
+// start that lies outside the box (atomicMin into boxi[0]); the last block
+// to finish (ticket in boxi[3]) expands the box toward that body and moves
+// start past it, or, when no candidate is outside, sets boxi[5] (done), after
+// which the remaining rounds return at once.
+__global__ __launch_bounds__(kTpb) void k_box_round(const double *__restrict__ pos, const double4 *__restrict__ cand,
+                                                    double4 *box, int *boxi)
+{
+    __shared__ int wbest[kTpb / 64];
+    if (boxi[5]) return;
+    const double4 b0 = *box;
+    const BoxTest b = box_test(b0);
+    const int start = boxi[1], nc = boxi[4];
+    int best = INT32_MAX;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += gridDim.x * blockDim.x) {
+        const double4 c = cand[k];
+        const int i = (int)c.w;
+        if (i >= start && i < best && !fits(c, b)) best = i;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) best = min(best, __shfl_xor(best, d, 64));
+    if (lane_id() == 0) wbest[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = wbest[0];
+#pragma unroll
+        for (int k = 1; k < kTpb / 64; k++) m = min(m, wbest[k]);
+        if (m != INT32_MAX) atomicMin(&boxi[0], m);
+        __threadfence();
+        const unsigned t = atomicAdd(reinterpret_cast<unsigned *>(&boxi[3]), 1u);
+        if (t == gridDim.x - 1) {
+            __threadfence();
+            const int first = atomicMin(&boxi[0], INT32_MAX);   // atomic read
+            boxi[3] = 0;
+            if (first != INT32_MAX) {
+                *box = expand_for(pos + 3 * (int64_t)first, b0);
+                boxi[2] += 1;
+                boxi[0] = INT32_MAX;
+                boxi[1] = first + 1;
+            } else {
+                boxi[5] = 1;
+            }
+        }
+    }
+}
+
+// Starting box: the caller's (rmin, rsize), or the previous box when keep.
+// boxi: [0] first outside candidate (min), [1] start index, [2] expansions,
+// [3] round ticket, [4] candidate count, [5] done.
+__global__ void k_box_init(double4 *box, int *boxi, double rx, double ry, double rz, double rsize, int keep)
+{
+    if (!keep) *box = make_double4(rx, ry, rz, rsize);
     boxi[0] = INT32_MAX;
-    boxi[1] = first + 1;
+    boxi[1] = 0;
+    boxi[2] = 0;
+    boxi[3] = 0;
+    boxi[4] = 0;
+    boxi[5] = 0;
+}
+
+// After the expansion rounds: flag a box that may still be open (no round
+// found every candidate inside).
+__global__ void k_box_check(const int *boxi, unsigned long long *stat)
+{
+    if (!boxi[5]) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
 }
 
 // ---- batched ensembles: one box per system -----------------------------------
@@ -501,7 +562,8 @@ __global__ void k_gather_test(const uint32_t *__restrict__ perm, int64_t nt, con
 
 // ---- topology ----------------------------------------------------------------
 
-__global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, uint32_t *ccnt)
+// cpl and its copy cq = cpl + 1 (0..22, for the 8-entry range scans).
+__global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, uint8_t *cq, uint32_t *ccnt)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nb) return;
@@ -509,6 +571,7 @@ __global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, ui
     if (s + 1 < nb) c = min(kMaxLevel, (__clzll(keys[s] ^ keys[s + 1]) - 1) / 3);
     if (s > 0) cp = min(kMaxLevel, (__clzll(keys[s - 1] ^ keys[s]) - 1) / 3);
     cpl[s] = (int8_t)c;
+    cq[s] = (uint8_t)(c + 1);
     ccnt[s] = (uint32_t)max(0, c - cp);
 }
 
@@ -524,16 +587,28 @@ __global__ void k_cmin(const int8_t *__restrict__ cpl, int nb, int8_t *m1, int n
 }
 
 // first j >= s with cpl[j] < L (exists: cpl[nb-1] = -1)
-__device__ int range_end(const int8_t *__restrict__ cpl, const int8_t *__restrict__ m1,
+// The scan reads cq = cpl + 1 eight entries at a time (8-byte aligned words;
+// cq holds nb + 8 bytes): a byte below n = L + 1 <= 22 in a word of bytes
+// below 128 is flagged by (x - n*0x01..01) & ~x & 0x80..80, exactly for the
+// lowest such byte (borrows only reach higher bytes).  Bytes before j are
+// set to 0x7f so that they never match.
+__device__ int range_end(const uint8_t *__restrict__ cq, const int8_t *__restrict__ m1,
                          const int8_t *__restrict__ m2, const int8_t *__restrict__ m3, int s, int L)
 {
+    const uint64_t ones = 0x0101010101010101ULL, highs = 0x8080808080808080ULL;
+    const uint64_t nl = ones * (uint64_t)(L + 1);
     int j = s;
     for (;;) {
         if ((j & 262143) == 0 && m3[j >> 18] >= L) { j += 262144; continue; }
         if ((j & 4095) == 0 && m2[j >> 12] >= L) { j += 4096; continue; }
         if ((j & 63) == 0 && m1[j >> 6] >= L) { j += 64; continue; }
-        if (cpl[j] < L) return j;
-        j++;
+        const int w = j & ~7;
+        uint64_t x = *reinterpret_cast<const uint64_t *>(cq + w);
+        const int sh = (j - w) * 8;
+        if (sh) x |= 0x7f7f7f7f7f7f7f7fULL & ((1ULL << sh) - 1);
+        const uint64_t t = (x - nl) & ~x & highs;
+        if (t) return w + (__builtin_ctzll(t) >> 3);
+        j = w + 8;
     }
 }
 
@@ -550,7 +625,8 @@ __global__ void k_owner(const uint32_t *__restrict__ cbase, int nb, const unsign
 // that start at body s are the levels cpl[s-1]+1 .. cpl[s] with ids from
 // cbase[s]; the range of a cell of level L ends before the first j >= s with
 // cpl[j] < L.
-__global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restrict__ cbase, const int *__restrict__ owner,
+__global__ void k_cells(const int8_t *__restrict__ cpl, const uint8_t *__restrict__ cq, const uint32_t *__restrict__ cbase,
+                        const int *__restrict__ owner,
                         const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
                         const unsigned long long *ncellp, int *cfirst, int *clast, uint8_t *clevel, int *cskip,
                         int *pre, int2 *fnb, unsigned long long *deep)
@@ -564,7 +640,7 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint32_t *__restri
             const int s = owner[id];
             const int l0 = (s > 0 ? cpl[s - 1] : -1) + 1;
             const int L = l0 + (id - (int)cbase[s]);
-            const int e = range_end(cpl, m1, m2, m3, s, L);
+            const int e = range_end(cq, m1, m2, m3, s, L);
             cfirst[id] = s;
             clast[id] = e;
             clevel[id] = (uint8_t)L;
@@ -787,54 +863,49 @@ __global__ void k_sysroot_cells(const uint8_t *__restrict__ clevel, const int *_
 __device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                             const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                             const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell, int *csub,
-                            int *cnkids, int *cnkc, int *cparent, int *cctr);
+                            int *cnkids);
 
 __global__ void k_children(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                            const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                            const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
-                           const unsigned long long *ncellp, int *csub, int *cnkids, int *cnkc, int *cparent,
-                           int *cctr)
+                           const unsigned long long *ncellp, int *csub, int *cnkids)
 {
     const int ncell = (int)*ncellp;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
-        children_of(c, keys, cpl, cbase, cfirst, clast, clevel, ncell, csub, cnkids, cnkc, cparent, cctr);
+        children_of(c, keys, cpl, cbase, cfirst, clast, clevel, ncell, csub, cnkids);
 }
 
 __device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                             const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                             const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int ncell, int *csub,
-                            int *cnkids, int *cnkc, int *cparent, int *cctr)
+                            int *cnkids)
 {
     int s = cfirst[c], e = clast[c], L = clevel[c];
-    int *sub = csub + 8 * (int64_t)c;
+    int r[8];
 #pragma unroll
-    for (int o = 0; o < 8; o++) sub[o] = kNoChild;
-    cctr[c] = 0;
-    if (c == 0) cparent[0] = -1;
+    for (int o = 0; o < 8; o++) r[o] = kNoChild;
+    int nk = 0;
     if (L == kMaxLevel) {            // bucket: bodies s..e are the children
-        cnkids[c] = e - s + 1;
-        cnkc[c] = 0;
-        return;
-    }
-    const int shift = 60 - 3 * L;    // octant digit of level L+1
-    int nk = 0, nkc = 0;
-    int t = s;
-    while (t <= e) {
-        const int o = (int)((keys[t] >> shift) & 7);
-        int cc;
-        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
-        if (cc >= 0) {
-            sub[o] = ~cc;
-            cparent[cc] = c;
-            nkc++;
-        } else {
-            sub[o] = t;
+        nk = e - s + 1;
+    } else {
+        const int shift = 60 - 3 * L;    // octant digit of level L+1
+        int t = s;
+        while (t <= e) {
+            const int o = (int)((keys[t] >> shift) & 7);
+            int cc;
+            const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
+            const int ref = cc >= 0 ? ~cc : t;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k == o) r[k] = ref;
+            t = tn;
+            nk++;
         }
-        t = tn;
-        nk++;
     }
+    int4 *sub = reinterpret_cast<int4 *>(csub + 8 * (int64_t)c);
+    sub[0] = make_int4(r[0], r[1], r[2], r[3]);
+    sub[1] = make_int4(r[4], r[5], r[6], r[7]);
     cnkids[c] = nk;
-    cnkc[c] = nkc;
 }
 
 // Bottom-up moments.  Threads start at cells without child cells; the last
@@ -1164,10 +1235,11 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     Topo t{};
     t.cap = cap;
     t.ncellp = stat + slot;
-    t.cpl = ensure<int8_t>(bcpl, n);
+    t.cpl = ensure<int8_t>(bcpl, 2 * (size_t)n + 32);
+    uint8_t *cq = reinterpret_cast<uint8_t *>(t.cpl) + ((n + 15) & ~7);   // 8-aligned, n + 8 bytes
     uint32_t *ccnt = ensure<uint32_t>(bcnt, n + 1);
     t.cbase = ensure<uint32_t>(bbase, n + 1);
-    k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, ccnt);
+    k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, cq, ccnt);
     exclusive_scan_u32(ctx, ccnt, t.cbase, n);
     k_clamp_cells<<<1, 1, 0, st>>>(t.cbase, n, (unsigned long long)cap, stat, slot, slot_raw, flag);
     int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64, n3 = (n2 + 63) / 64;
@@ -1184,7 +1256,7 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     int2 *fnb = bfnb ? ensure<int2>(*bfnb, cap) : nullptr;
     int *owner = ensure<int>(bowner, cap);
     k_owner<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, owner);
-    k_cells<<<capgrid(ctx, cap), kTpb, 0, st>>>(t.cpl, t.cbase, owner, m1, m2, m3, t.ncellp, t.first, t.last, t.level,
+    k_cells<<<capgrid(ctx, cap), kTpb, 0, st>>>(t.cpl, cq, t.cbase, owner, m1, m2, m3, t.ncellp, t.first, t.last, t.level,
                                                 cskip, pre, fnb, bskip ? stat + S_DEEP : nullptr);
     if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, pre);
     BH_HIP(hipGetLastError());
@@ -1278,21 +1350,11 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
                                          16 * ctx->box_rounds, stat);
     } else {
     k_box_init<<<1, 1, 0, st>>>(d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
-    const int box_grid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 4);
-    int *cand = ensure<int>(ctx->bcand, n);
-    const int64_t chunk = std::max<int64_t>(kTpb, (n + box_grid - 1) / box_grid);
-    const int nchunk = (int)((n + chunk - 1) / chunk);
-    uint32_t *bcount = ensure<uint32_t>(ctx->bcount, 2 * (size_t)nchunk + 2);
-    uint32_t *boff = bcount + nchunk + 1;
-    k_box_count<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, bcount, stat);
-    exclusive_scan_u32(ctx, bcount, boff, nchunk);
-    k_box_write<<<nchunk, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, chunk, boff, cand);
+    double4 *cand = ensure<double4>(ctx->bcand, n);
+    k_box_compact<<<(int)((n + kBoxChunk - 1) / kBoxChunk), kTpb, 0, st>>>(d_pos, d_mass, n, d_box, cand, d_boxi,
+                                                                            stat);
     const int rounds = std::max(1, keep ? ctx->box_rounds_keep : ctx->box_rounds);
-    for (int r = 0; r < rounds; r++) {
-        k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
-        k_box_expand<<<1, 1, 0, st>>>(d_pos, d_box, d_boxi);
-    }
-    k_box_first_c<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, boff + nchunk, d_box, d_boxi);
+    for (int r = 0; r <= rounds; r++) k_box_round<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, d_box, d_boxi);
     k_box_check<<<1, 1, 0, st>>>(d_boxi, stat);
     }
 
@@ -1383,10 +1445,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     uint8_t *clevel = tt.level;
     T.pre = ptr<int>(ctx->pre);
     int *csub = ensure<int>(ctx->csub, 8 * (size_t)cap);
-    int *cnkids = ensure<int>(ctx->cnkids, cap), *cnkc = ensure<int>(ctx->cnkc, cap);
-    int *cparent = ensure<int>(ctx->cparent, cap), *cctr = ensure<int>(ctx->cctr, cap);
-    k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids, cnkc,
-                                              cparent, cctr);
+    int *cnkids = ensure<int>(ctx->cnkids, cap);
+    k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids);
     double *mo = ensure<double>(ctx->cmo, 10 * (size_t)cap);
     float4 *cA = ensure<float4>(ctx->cA, cap), *cB = ensure<float4>(ctx->cB, cap);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
@@ -1407,6 +1467,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst, clast,
                                                 clevel, bd, mo, cA,
                                                 cB, cQ0, cQ1, d_box, p.usequad, lsys, ctx->sysbits, sysbox);
+    T.rc2 = p.criterion != BH_CRIT_HACK ? cell_rcrit(ctx, p, ks, cfirst, clevel, mo, tt.ncellp, cap, d_box, cB)
+                                         : nullptr;
     if (nsys)
         k_sysroot_cells<<<nblocks(cap), kTpb, 0, st>>>(clevel, cfirst, ks, lsys, ctx->sysbits, tt.ncellp,
                                                        ptr<int>(ctx->sysroot));

@@ -56,6 +56,18 @@ __device__ __forceinline__ float wave_max(float v)
     for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
     return v;
 }
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
 __device__ __forceinline__ int wave_excl_scan(int v, int *total)
 {
     const int lane = lane_id();
@@ -215,9 +227,18 @@ struct WalkArgs {
     const int64_t *boff;
     const float2 *sysp;
     const int *sysroot;
+    // treecode1 criteria: root box; cB.w holds each cell's Rcrit2
+    const double4 *box;
 };
 
-template <int NPL, bool QUAD, bool SOFT0, bool LREC>
+// CRIT: treecode1's accept (treegrav.c:137-156) for the whole group.  With
+// U the union of the side-s cubes (s = the cell's side, cubes of the cell's
+// level) that hold the group's points, a cell is accepted when the squared
+// distance from its c.o.m. to U exceeds its Rcrit2 and, along some axis, the
+// c.o.m. lies more than s beyond U.  Every body's own sink cube lies in U, so
+// each body's treecode1 test passes whenever the group's does.  U is formed
+// in integer coordinates of the finest level (2^21 per side).
+template <int NPL, bool QUAD, bool SOFT0, bool LREC, bool CRIT>
 __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
 {
     constexpr int G = 64 * NPL;
@@ -289,6 +310,31 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         }
         bminx = wave_min(bminx); bminy = wave_min(bminy); bminz = wave_min(bminz);
         bmaxx = wave_max(bmaxx); bmaxy = wave_max(bmaxy); bmaxz = wave_max(bmaxz);
+        int ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};
+        float uoff[3] = {0.f, 0.f, 0.f}, uunit = 0.f, urs = 0.f;
+        if (CRIT) {
+            const double4 bb = *a.box;
+            const double org[3] = {bb.x, bb.y, bb.z}, cen[3] = {cx, cy, cz};
+            const double sc = 2097152.0 / bb.w;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                int lo = INT32_MAX, hi = INT32_MIN;
+#pragma unroll
+                for (int q = 0; q < NPL; q++) {
+                    const float hk = k == 0 ? h[q].x : (k == 1 ? h[q].y : h[q].z);
+                    const float lk = k == 0 ? l[q].x : (k == 1 ? l[q].y : l[q].z);
+                    double f = floor((((double)hk + (double)lk) - org[k]) * sc);
+                    f = fmin(fmax(f, -536870912.0), 536870912.0);
+                    lo = min(lo, (int)f);
+                    hi = max(hi, (int)f);
+                }
+                ulo[k] = wave_min_i(lo);
+                uhi[k] = wave_max_i(hi);
+                uoff[k] = (float)((org[k] + (double)ulo[k] * (bb.w / 2097152.0)) - cen[k]);
+            }
+            uunit = (float)(bb.w / 2097152.0);
+            urs = (float)bb.w;
+        }
 #pragma unroll
         for (int q = 0; q < NPL; q++) sSelf[w][q * 64 + lane] = 0;
 
@@ -344,8 +390,24 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     float dz = fmaxf(fmaxf(bminz - rz, rz - bmaxz), 0.f);
                     float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
                     const int2 fn = a.fnb[c];
-                    const bool small = fn.y <= a.nleaf || a.level[c] == kMaxLevel;
-                    open = tolsq * d2 < B.w || fn.y <= a.nleaf;
+                    const int lv = a.level[c];
+                    const bool small = fn.y <= a.nleaf || lv == kMaxLevel;
+                    if (CRIT) {
+                        const int sh = kMaxLevel - lv, step = 1 << sh;
+                        const float side = ldexpf(urs, -lv), rr[3] = {rx, ry, rz};
+                        float dd[3];
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const float flo = (float)((ulo[k] >> sh) * step - ulo[k]) * uunit + uoff[k];
+                            const float fhi = (float)(((uhi[k] >> sh) + 1) * step - ulo[k]) * uunit + uoff[k];
+                            dd[k] = fmaxf(fmaxf(flo - rr[k], rr[k] - fhi), 0.f);
+                        }
+                        const float usq = fmaf(dd[0], dd[0], fmaf(dd[1], dd[1], dd[2] * dd[2]));
+                        const bool adj_ok = fmaxf(dd[0], fmaxf(dd[1], dd[2])) > side;
+                        open = !(usq > B.w && adj_ok) || fn.y <= a.nleaf;
+                    } else {
+                        open = tolsq * d2 < B.w || fn.y <= a.nleaf;
+                    }
                     accept = !open;
                     if (open) nk = small ? fn.y : a.nkids[c];
                     bfirst = small ? fn.x : -1;
@@ -518,10 +580,11 @@ __global__ void k_clear_regions(int *lrec, int cap)
 }
 
 template <int NPL, bool QUAD, bool SOFT0>
-void launch(bh_ctx *ctx, WalkArgs &a, int grid)
+void launch(bh_ctx *ctx, WalkArgs &a, int grid, bool crit)
 {
-    if (a.lmode) k_walk<NPL, QUAD, SOFT0, true><<<grid, kTpb, 0, ctx->stream>>>(a);
-    else k_walk<NPL, QUAD, SOFT0, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    if (crit) k_walk<NPL, QUAD, SOFT0, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else if (a.lmode) k_walk<NPL, QUAD, SOFT0, true, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else k_walk<NPL, QUAD, SOFT0, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
 }
 
 }  // namespace
@@ -596,12 +659,14 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     ctx->lnwave = nwave;
     const bool soft0 = ctx->nsys ? ctx->sys_eps0 : p.eps == 0.0;
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    const bool crit = p.criterion != BH_CRIT_HACK;
+    a.box = T.dbox;
     if (npl == 2) {
-        if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid) : launch<2, true, false>(ctx, a, grid);
-        else soft0 ? launch<2, false, true>(ctx, a, grid) : launch<2, false, false>(ctx, a, grid);
+        if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid, crit) : launch<2, true, false>(ctx, a, grid, crit);
+        else soft0 ? launch<2, false, true>(ctx, a, grid, crit) : launch<2, false, false>(ctx, a, grid, crit);
     } else {
-        if (p.usequad) soft0 ? launch<1, true, true>(ctx, a, grid) : launch<1, true, false>(ctx, a, grid);
-        else soft0 ? launch<1, false, true>(ctx, a, grid) : launch<1, false, false>(ctx, a, grid);
+        if (p.usequad) soft0 ? launch<1, true, true>(ctx, a, grid, crit) : launch<1, true, false>(ctx, a, grid, crit);
+        else soft0 ? launch<1, false, true>(ctx, a, grid, crit) : launch<1, false, false>(ctx, a, grid, crit);
     }
     BH_HIP(hipGetLastError());
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[5], ctx->stream));

@@ -33,7 +33,8 @@ class BHError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("eps", C.c_double), ("usequad", C.c_int), ("mode", C.c_int),
                 ("rsize", C.c_double), ("rmin", C.c_double * 3), ("rmin_given", C.c_int),
-                ("group_size", C.c_int), ("learned", C.c_int), ("keep_box", C.c_int), ("nleaf", C.c_int)]
+                ("group_size", C.c_int), ("learned", C.c_int), ("keep_box", C.c_int), ("nleaf", C.c_int),
+                ("criterion", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -99,6 +100,9 @@ SIGNATURES = {
     "bh_autotune": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.POINTER(Params), _d]),
 }
 
+# bh_criterion: hackcode1's opening test, treecode1's default / bh86 / sw94
+CRITERIA = {"hack": 0, "rcrit": 1, "bh86": 2, "sw94": 3}
+
 AUTOTUNE_CANDIDATES = [(g, nl) for g in (64, 128) for nl in (0, 1, 2, 4, 8)]
 
 
@@ -131,6 +135,8 @@ def default_params(**kw) -> Params:
                 p.rmin_given = 1
         elif k == "mode" and isinstance(v, str):
             p.mode = {"fast": MODE_FAST, "strict": MODE_STRICT, "direct": MODE_DIRECT}[v]
+        elif k == "criterion" and isinstance(v, str):
+            p.criterion = CRITERIA[v]
         else:
             setattr(p, k, v)
     return p

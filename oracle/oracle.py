@@ -53,6 +53,9 @@ def lib():
         L.orc_tree_ncell.argtypes = [C.c_void_p]
         L.orc_tree_dump.argtypes = [C.c_void_p, _d, _d, _d, _i32, _i32]
         L.orc_tree_dump.restype = C.c_int
+        L.tc_force.argtypes = [C.c_int, _d, _d, C.c_int, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int,
+                               C.c_int, _d, _d, _i64, _i64, _i64, C.c_int, _i32, _d, _d, _d]
+        L.tc_force.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -203,3 +206,41 @@ class Leapfrog:
         d = self.diagnostics()
         self.drift()
         return d, n2b, nbc
+
+
+TC_CRITERIA = {"rcrit": 0, "bh86": 1, "sw94": 2}
+
+
+def tc_force(mass, pos, rmin, rsize, theta=1.0, eps=0.025, criterion="rcrit", usequad=False, tpos=None,
+             cells=False):
+    """treecode1 gravcalc over the root box (rmin, rsize): returns a dict with
+    phi, acc, n2b, nbc per target (bodies, then test points), ncell and, with
+    cells=True, the cells' level, mass, com and rcrit2 in DFS order."""
+    mass = np.ascontiguousarray(mass, np.float64)
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    n = len(mass)
+    tp = None if tpos is None else np.ascontiguousarray(tpos, np.float64).reshape(-1, 3)
+    nt = 0 if tp is None else len(tp)
+    phi = np.zeros(n + nt)
+    acc = np.zeros((n + nt, 3))
+    n2b = np.zeros(n + nt, np.int64)
+    nbc = np.zeros(n + nt, np.int64)
+    nc = np.zeros(1, np.int64)
+    cap = 2 * n + 8 if cells else 0
+    lev = np.zeros(max(cap, 1), np.int32)
+    cm_ = np.zeros(max(cap, 1))
+    ccm = np.zeros((max(cap, 1), 3))
+    rc = np.zeros(max(cap, 1))
+    rmin = np.ascontiguousarray(rmin, np.float64)
+    err = lib().tc_force(n, _p(mass), _p(pos), nt, None if tp is None else _p(tp), _p(rmin), float(rsize),
+                         float(theta), float(eps), TC_CRITERIA[criterion], int(bool(usequad)), _p(phi), _p(acc),
+                         _p(n2b, _i64), _p(nbc, _i64), _p(nc, _i64), cap,
+                         _p(lev, _i32) if cells else None, _p(cm_) if cells else None,
+                         _p(ccm) if cells else None, _p(rc) if cells else None)
+    if err:
+        raise RuntimeError(f"tc_force failed with code {err}")
+    out = dict(phi=phi, acc=acc, n2b=n2b, nbc=nbc, ncell=int(nc[0]))
+    if cells:
+        k = int(nc[0])
+        out.update(level=lev[:k], mass=cm_[:k], com=ccm[:k], rcrit2=rc[:k])
+    return out
