@@ -44,7 +44,7 @@ struct bh_ctx {
     bh::DevBuf tkeys0, tkeys1, tvals0, tvals1;
     bh::DevBuf nmassive;        // int
     // sorted bodies of the tree and of the test set
-    bh::DevBuf bd, bhi, blo;
+    bh::DevBuf bd, bhi, blo, bpk;   // bpk: (x, y, z, m) records in input order
     bh::DevBuf thi, tlo, td, tperm, tself;
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;

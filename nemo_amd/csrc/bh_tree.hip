@@ -338,15 +338,20 @@ __global__ void k_keys_batch(const double *__restrict__ pos, int64_t n, const in
 }
 
 // Mass set: massless bodies get bit 63 so they sort after every massive body.
+// pk (optional): the body packed as (x, y, z, m) in input order, so that the
+// gather into sorted order reads one aligned 32-byte record per body.
 __global__ void k_keys(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals)
+                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals, double4 *pk)
 {
     const double4 b = *box;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t k = morton_of(pos + 3 * i, b);
-        if (mass != nullptr && mass[i] == 0.0) k |= (1ULL << 63);
+        const double *x = pos + 3 * i;
+        uint64_t k = morton_of(x, b);
+        const double m = mass != nullptr ? mass[i] : 0.0;
+        if (mass != nullptr && m == 0.0) k |= (1ULL << 63);
         keys[i] = k;
         vals[i] = (uint32_t)i;
+        if (pk) pk[i] = make_double4(x[0], x[1], x[2], m);
     }
 }
 
@@ -440,6 +445,31 @@ __global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int s
     hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
+// Exclusive scan of each digit's row of the digit-major block histogram
+// (block d scans hist[d*nblk .. d*nblk+nblk)); the row total goes to dtot[d].
+// The scatter adds the exclusive scan of dtot (the digit's first output slot).
+__global__ __launch_bounds__(kTpb) void k_hist_rows(const uint32_t *__restrict__ hist, int nblk, uint32_t *hscan,
+                                                    uint32_t *dtot)
+{
+    const int64_t row = (int64_t)blockIdx.x * nblk;
+    uint32_t run = 0;
+    for (int b0 = 0; b0 < nblk; b0 += kTpb * 4) {
+        const int i0 = b0 + threadIdx.x * 4;
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = i0 + j < nblk ? hist[row + i0 + j] : 0u;
+        uint32_t tot;
+        uint32_t acc = run + block_excl_scan(v[0] + v[1] + v[2] + v[3], &tot);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (i0 + j < nblk) hscan[row + i0 + j] = acc;
+            acc += v[j];
+        }
+        run += tot;
+    }
+    if (threadIdx.x == 0) dtot[blockIdx.x] = run;
+}
+
 // Scatter one tile of kSortTile keys.  The tile is first ranked into LDS in
 // digit order: within a round of 256 consecutive keys, ranks among equal
 // digits come from a wave-level match (8 ballots) and a per-digit prefix over
@@ -449,7 +479,8 @@ __global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int s
 __global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                         uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                         int64_t n, int shift, const uint32_t *__restrict__ hist,
-                                                        const uint32_t *__restrict__ hist_scan, int nblk)
+                                                        const uint32_t *__restrict__ hist_scan,
+                                                        const uint32_t *__restrict__ dtot, int nblk)
 {
     __shared__ uint64_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
@@ -462,9 +493,10 @@ __global__ __launch_bounds__(kTpb) void k_radix_scatter(const uint64_t *__restri
         uint32_t tot;
         const uint32_t c = hist[(int64_t)t * nblk + blockIdx.x];
         const uint32_t o = block_excl_scan(c, &tot);
+        const uint32_t dbase = block_excl_scan(dtot[t], &tot);
         toff[t] = o;
         run[t] = o;
-        gofs[t] = hist_scan[(int64_t)t * nblk + blockIdx.x] - o;
+        gofs[t] = dbase + hist_scan[(int64_t)t * nblk + blockIdx.x] - o;
     }
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
     const int cnt = (int)min<int64_t>(kSortTile, n - base);
@@ -521,13 +553,21 @@ __device__ __forceinline__ void split2(double x, float &hi, float &lo)
     lo = (float)(x - (double)hi);
 }
 
+// pk: packed (x, y, z, m) records in input order, or nullptr to read pos/mass.
 __global__ void k_gather_tree(const uint32_t *__restrict__ perm, int nb, const double *__restrict__ pos,
-                              const double *__restrict__ mass, double4 *bd, float4 *bhi, float4 *blo)
+                              const double *__restrict__ mass, const double4 *__restrict__ pk, double4 *bd,
+                              float4 *bhi, float4 *blo)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nb) return;
     int64_t o = perm[s];
-    double x = pos[3 * o], y = pos[3 * o + 1], z = pos[3 * o + 2], m = mass[o];
+    double x, y, z, m;
+    if (pk) {
+        const double4 r = pk[o];
+        x = r.x, y = r.y, z = r.z, m = r.w;
+    } else {
+        x = pos[3 * o], y = pos[3 * o + 1], z = pos[3 * o + 2], m = mass[o];
+    }
     bd[s] = make_double4(x, y, z, m);
     float4 h, l;
     split2(x, h.x, l.x);
@@ -745,9 +785,20 @@ template <bool WRITE>
 __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                                 const uint32_t *__restrict__ cbase, const int *__restrict__ cfirst,
                                 const int *__restrict__ clast, const uint8_t *__restrict__ clevel, int G, int lsys,
-                                int ncell, int *gfirst, int *gcount, uint32_t off)
+                                int ncell, const int *__restrict__ csub, const int2 *__restrict__ fnb, int *gfirst,
+                                int *gcount, uint32_t off)
 {
-    const int s = cfirst[c], e = clast[c], L = clevel[c];
+    int s, e;
+    if (fnb) {
+        const int2 f = fnb[c];
+        if (f.y <= G && lsys == 0) return 0;
+        s = f.x;
+        e = f.x + f.y - 1;
+    } else {
+        s = cfirst[c];
+        e = clast[c];
+    }
+    const int L = clevel[c];
     uint32_t k = 0;
     auto emit = [&](int start, int count) {
         if (WRITE) {
@@ -774,25 +825,16 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
         return k;
     }
     const int shift = 60 - 3 * L;
-    // octants of the children holding more than G points
-    unsigned big = 0;
-    for (int t = s; t <= e;) {
-        int cc;
-        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
-        if (tn - t > G) big |= 1u << ((keys[t] >> shift) & 7);
-        t = tn;
-    }
     // runs of consecutive small children; a run also ends when the box spanned
-    // by its octants would cover a big child's octant.  A child that does not
-    // fit is split: its first bodies fill the run to G and the rest start the
-    // next run (every group stays inside this cell, and groups are full).
-    int t = s, run = 0, rs = s;
+    // by its octants would cover a big child's octant (a child holding more
+    // than G points).  A child that does not fit is split: its first bodies
+    // fill the run to G and the rest start the next run (every group stays
+    // inside this cell, and groups are full).
+    unsigned big = 0;
+    int run = 0, rs = s;
     unsigned rmask = 0;
-    while (t <= e) {
-        int cc;
-        const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
-        const int size = tn - t;
-        const unsigned o = 1u << ((keys[t] >> shift) & 7);
+    auto visit = [&](int t, int size, int oct) {
+        const unsigned o = 1u << oct;
         if (size > G) {
             if (run) emit(rs, run);
             run = 0;
@@ -814,7 +856,35 @@ __device__ uint32_t pack_groups(int c, const uint64_t *__restrict__ keys, const 
                 rmask |= o;
             }
         }
-        t = tn;
+    };
+    if (csub) {
+        // children from the tree's child slots: (first point, count) per octant
+        const int4 s0 = reinterpret_cast<const int4 *>(csub)[2 * (int64_t)c];
+        const int4 s1 = reinterpret_cast<const int4 *>(csub)[2 * (int64_t)c + 1];
+        const int r[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+        int2 f[8];
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+            f[o] = r[o] == kNoChild ? make_int2(0, 0) : (r[o] >= 0 ? make_int2(r[o], 1) : fnb[~r[o]]);
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+            if (f[o].y > G) big |= 1u << o;
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+            if (f[o].y > 0) visit(f[o].x, f[o].y, o);
+    } else {
+        for (int t = s; t <= e;) {
+            int cc;
+            const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
+            if (tn - t > G) big |= 1u << ((keys[t] >> shift) & 7);
+            t = tn;
+        }
+        for (int t = s; t <= e;) {
+            int cc;
+            const int tn = child_end(c, s, e, t, L, cpl, cbase, clast, ncell, &cc);
+            visit(t, tn - t, (int)((keys[t] >> shift) & 7));
+            t = tn;
+        }
     }
     if (run) emit(rs, run);
     return k;
@@ -824,22 +894,25 @@ __global__ void k_group_count(const uint64_t *__restrict__ keys, const int8_t *_
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
                               const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G,
-                              int lsys, uint32_t *gcnt)
+                              int lsys, const int *__restrict__ csub, const int2 *__restrict__ fnb, uint32_t *gcnt)
 {
     const int ncell = (int)*ncellp;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
-        gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, nullptr, nullptr, 0);
+        gcnt[c] = pack_groups<false>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, csub, fnb, nullptr,
+                                     nullptr, 0);
 }
 
 __global__ void k_group_write(const uint64_t *__restrict__ keys, const int8_t *__restrict__ cpl,
                               const uint32_t *__restrict__ cbase,
                               const int *__restrict__ cfirst, const int *__restrict__ clast,
                               const uint8_t *__restrict__ clevel, const unsigned long long *ncellp, int G, int lsys,
+                              const int *__restrict__ csub, const int2 *__restrict__ fnb,
                               const uint32_t *__restrict__ goff, int *gfirst, int *gcount)
 {
     const int ncell = (int)*ncellp;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x)
-        pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, gfirst, gcount, goff[c]);
+        pack_groups<true>(c, keys, cpl, cbase, cfirst, clast, clevel, G, lsys, ncell, csub, fnb, gfirst, gcount,
+                          goff[c]);
 }
 
 // Walk roots of a batch: the system's root cell (the cell at the first
@@ -1198,14 +1271,15 @@ void radix_sort_pairs(bh_ctx *ctx, uint64_t *k0, uint64_t *k1, uint32_t *v0, uin
                       int begin_bit, int end_bit, uint64_t **kout, uint32_t **vout)
 {
     int nblk = (int)std::max<int64_t>(1, (n + kSortTile - 1) / kSortTile);
-    uint32_t *hist = ensure<uint32_t>(ctx->hist, 2 * (256 * (size_t)nblk + 1));
+    uint32_t *hist = ensure<uint32_t>(ctx->hist, 2 * (256 * (size_t)nblk + 1) + 256);
     uint32_t *hscan = hist + 256 * (size_t)nblk + 1;
+    uint32_t *dtot = hscan + 256 * (size_t)nblk + 1;
     uint64_t *ka = k0, *kb = k1;
     uint32_t *va = v0, *vb = v1;
     for (int shift = begin_bit; shift < end_bit; shift += 8) {
         k_radix_hist<<<nblk, kTpb, 0, ctx->stream>>>(ka, n, shift, hist, nblk);
-        exclusive_scan_u32(ctx, hist, hscan, 256 * (int64_t)nblk);
-        k_radix_scatter<<<nblk, kTpb, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, hist, hscan, nblk);
+        k_hist_rows<<<256, kTpb, 0, ctx->stream>>>(hist, nblk, hscan, dtot);
+        k_radix_scatter<<<nblk, kTpb, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, hist, hscan, dtot, nblk);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -1265,7 +1339,10 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
 
 // Groups of the sorted test set for the fast walk (ctx->gfirst/gcount, count
 // in stat[S_NGROUPS]).
-static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G, int lsys = 0)
+// csub/fnb: the tree's child slots and (first, count) per cell, when the
+// groups are cut from the tree's own topology.
+static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_t n, int G, int lsys = 0,
+                        const int *csub = nullptr, const int2 *fnb = nullptr)
 {
     hipStream_t st = ctx->stream;
     unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
@@ -1278,10 +1355,10 @@ static void make_groups(bh_ctx *ctx, const Topo *t, const uint64_t *keys, int64_
     uint32_t *gcnt = ensure<uint32_t>(ctx->gscan, (size_t)t->cap + 1);
     uint32_t *goff = ensure<uint32_t>(ctx->goff, (size_t)t->cap + 1);
     k_group_count<<<capgrid(ctx, t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level,
-                                                         t->ncellp, G, lsys, gcnt);
+                                                         t->ncellp, G, lsys, csub, fnb, gcnt);
     exclusive_scan_u32(ctx, gcnt, goff, t->cap, t->ncellp);
     k_group_write<<<capgrid(ctx, t->cap), kTpb, 0, st>>>(keys, t->cpl, t->cbase, t->first, t->last, t->level,
-                                                         t->ncellp, G, lsys, goff, gf, gc);
+                                                         t->ncellp, G, lsys, csub, fnb, goff, gf, gc);
     k_copy_count<<<1, 1, 0, st>>>(goff + t->cap, stat + S_NGROUPS);
     BH_HIP(hipGetLastError());
 }
@@ -1293,7 +1370,7 @@ void debug_keys(bh_ctx *ctx, uint64_t *d_out)
     uint32_t *tmp;
     BH_HIP(hipMalloc(&tmp, ctx->nmass * sizeof(uint32_t)));
     k_keys<<<nblocks(ctx->nmass), kTpb, 0, ctx->stream>>>(ptr<double>(ctx->pos), ptr<double>(ctx->mass), ctx->nmass,
-                                                          d_box, d_out, tmp);
+                                                          d_box, d_out, tmp, nullptr);
     BH_HIP(hipStreamSynchronize(ctx->stream));
     BH_HIP(hipFree(tmp));
 }
@@ -1307,7 +1384,7 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo,
     const int64_t nt = ctx->ntest;
     const Tree &T = ctx->tree;
     if (ctx->self_test && nt == T.nb && tree_topo) {
-        make_groups(ctx, tree_topo, ks, nt, G, ctx->sysbits / 3);
+        make_groups(ctx, tree_topo, ks, nt, G, ctx->sysbits / 3, T.sub, T.fnb);
         return;
     }
     if (ctx->nsys > 1) throw ApiFail(BH_ERR_STATE, "batch groups need the system-aware topology");
@@ -1362,11 +1439,13 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     uint64_t *k0 = ensure<uint64_t>(ctx->keys0, n), *k1 = ensure<uint64_t>(ctx->keys1, n);
     uint32_t *v0 = ensure<uint32_t>(ctx->vals0, n), *v1 = ensure<uint32_t>(ctx->vals1, n);
     const int grid_n = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * 16);
+    double4 *pk = nullptr;
     if (nsys) {
         k_keys_batch<<<grid_n, kTpb, 0, st>>>(d_pos, n, sysoff, nsys, sysbox, ctx->sysbits, k0, v0);
         k_sysroot_init<<<nblocks(nsys), kTpb, 0, st>>>(sysoff, nsys, ensure<int>(ctx->sysroot, nsys));
     } else {
-        k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0);
+        pk = ensure<double4>(ctx->bpk, n);
+        k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, pk);
     }
     uint64_t *ks;
     uint32_t *vs;
@@ -1379,7 +1458,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
     float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
-    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, st>>>(vs, nb, d_pos, d_mass, bd, bhi, blo);
+    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, st>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo);
     T.bd = bd;
     T.bhi = bhi;
     T.blo = blo;
@@ -1416,7 +1495,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         uint32_t *tv0 = ensure<uint32_t>(ctx->tvals0, nt), *tv1 = ensure<uint32_t>(ctx->tvals1, nt);
         double *d_tpos = ptr<double>(ctx->tpos);
         k_keys<<<(int)std::min<int64_t>(nblocks(nt), (int64_t)ctx->ncu * 16), kTpb, 0, st>>>(d_tpos, nullptr, nt, d_box,
-                                                                                            tk0, tv0);
+                                                                                            tk0, tv0, nullptr);
         uint64_t *tks;
         uint32_t *tvs;
         radix_sort_pairs(ctx, tk0, tk1, tv0, tv1, nt, 0, 64, &tks, &tvs);
