@@ -985,10 +985,11 @@ __device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8
 // child to finish carries on with the parent.  Accumulation follows
 // hackcofm's order: children in octant order, FP64, no FMA (this file is
 // compiled with -ffp-contract=off).
+template <bool QUAD>
 __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__restrict__ cfirst,
                              const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
                              const double4 *__restrict__ bd, double *mo,
-                             float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize, int usequad,
+                             float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1, double rsize,
                              const uint64_t *__restrict__ keys, double4 box, float4 *cF0, float4 *cF1, int lsys,
                              int sysbits, const double4 *__restrict__ sysbox)
 {
@@ -1019,7 +1020,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
         px = px / M;
         py = py / M;
         pz = pz / M;
-        if (usequad) {
+        if (QUAD) {
             for (int t = s; t <= e; t++) {
                 double4 b = bd[t];
                 double dr[3] = {b.x - px, b.y - py, b.z - pz};
@@ -1068,7 +1069,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
         px = px / M;
         py = py / M;
         pz = pz / M;
-        if (usequad) {
+        if (QUAD) {
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 if (r[k] == kNoChild) continue;
@@ -1101,7 +1102,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     me[1] = px;
     me[2] = py;
     me[3] = pz;
-    if (usequad) {
+    if (QUAD) {
 #pragma unroll
         for (int u = 0; u < 6; u++) me[4 + u] = q[u];
     }
@@ -1113,7 +1114,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
     cA[c] = A;
     cB[c] = B;
-    if (usequad) {
+    if (QUAD) {
         cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
         cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
     }
@@ -1231,21 +1232,22 @@ __global__ __launch_bounds__(kTpb) void k_level_place(const uint8_t *__restrict_
     }
 }
 
-__global__ void k_moments_level(const int *__restrict__ list, const uint32_t *__restrict__ loff, int L,
+template <bool QUAD>
+__global__ __launch_bounds__(kTpb) void k_moments_level(const int *__restrict__ list, const uint32_t *__restrict__ loff, int L,
                                 const uint64_t *__restrict__ keys, float4 *cF0, float4 *cF1,
                                 const int *__restrict__ csub,
                                 const int *__restrict__ cfirst, const int *__restrict__ clast,
                                 const uint8_t *__restrict__ clevel, const double4 *__restrict__ bd, double *mo,
                                 float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
-                                const double4 *__restrict__ box, int usequad, int lsys, int sysbits,
+                                const double4 *__restrict__ box, int lsys, int sysbits,
                                 const double4 *__restrict__ sysbox)
 {
     const double rsize = box->w;
     const double4 bx = *box;
     const uint32_t lo = loff[L], hi = loff[L + 1];
     for (uint32_t k = lo + blockIdx.x * blockDim.x + threadIdx.x; k < hi; k += gridDim.x * blockDim.x)
-        cell_moments(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, usequad, keys, bx, cF0,
-                     cF1, lsys, sysbits, sysbox);
+        cell_moments<QUAD>(list[k], csub, cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, rsize, keys, bx, cF0,
+                           cF1, lsys, sysbits, sysbox);
 }
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
@@ -1542,10 +1544,16 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int lsys = ctx->sysbits / 3;
     // the learned features are written only for the learned walk and sampling
     const bool feat = p.learned != 0 || ctx->mlp_sampling;
-    for (int L = kMaxLevel; L >= 0; L--)
-        k_moments_level<<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst, clast,
-                                                clevel, bd, mo, cA,
-                                                cB, cQ0, cQ1, d_box, p.usequad, lsys, ctx->sysbits, sysbox);
+    for (int L = kMaxLevel; L >= 0; L--) {
+        if (p.usequad)
+            k_moments_level<true><<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst,
+                                                          clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box, lsys,
+                                                          ctx->sysbits, sysbox);
+        else
+            k_moments_level<false><<<mgrid * 2, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
+                                                               cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box,
+                                                               lsys, ctx->sysbits, sysbox);
+    }
     T.rc2 = p.criterion != BH_CRIT_HACK ? cell_rcrit(ctx, p, ks, cfirst, clevel, mo, tt.ncellp, cap, d_box, cB)
                                          : nullptr;
     if (nsys)
