@@ -1385,6 +1385,12 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo,
     const int G = p.group_size > 0 ? p.group_size : 64;
     const int64_t nt = ctx->ntest;
     const Tree &T = ctx->tree;
+    // at most G points: one group (a cell only cuts groups for children of a
+    // cell holding more than G points); batches keep one group per system
+    if (ctx->nsys == 0 && nt <= G) {
+        make_groups(ctx, nullptr, nullptr, nt, G);
+        return;
+    }
     if (ctx->self_test && nt == T.nb && tree_topo) {
         make_groups(ctx, tree_topo, ks, nt, G, ctx->sysbits / 3, T.sub, T.fnb);
         return;
