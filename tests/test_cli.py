@@ -249,6 +249,28 @@ def test_hackcode1_default_run_reproduces_reference_kats(golden, tmp_path):
 
 
 @gpu
+def test_hackcode1_fast_small_system(golden, tmp_path):
+    """The default run in fast mode: 128 bodies form one group, so every
+    body sees every other one; the t=0 energy equals the FP64 direct sum's
+    (to the log's 4 decimals) and drifts less than the reference's own run
+    (tree forces, -0.2675 -> -0.2647)."""
+    from oracle import oracle
+
+    src = os.path.join(golden, "hack128.sun4")
+    r = run("hackcode1", "in=" + src, "out=fast", "mode=fast", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    t = [l.split() for l in r.stdout.decode().splitlines()]
+    rows = [x for x in t if len(x) == 7 and x[0].replace(".", "").isdigit()]
+    assert len(rows) == 65
+    s0 = snapshot.read(src)[0]
+    phi, _ = oracle.direct(s0.mass, s0.pos, 0.05)
+    e0 = 0.5 * (s0.mass * (s0.vel ** 2).sum(1)).sum() + 0.5 * (s0.mass * phi).sum()
+    assert abs(float(rows[0][1]) - e0) <= 6e-5
+    assert int(rows[0][3]) == 128 * 127
+    assert max(abs(float(x[1]) - float(rows[0][1])) for x in rows) <= 0.0028
+
+
+@gpu
 def test_hackcode1_save_and_restart_continue_bitwise(golden, tmp_path):
     """A run saved at t=1 and restarted to t=2 ends bit-identical to a
     straight run to t=2 (strict and fast walk)."""
