@@ -283,7 +283,7 @@ def main():
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--nsub", type=int, default=8192)
     ap.add_argument("--tol", type=float, default=None, help="skip the plain calibration and use this tol")
-    ap.add_argument("--group", type=int, default=128)
+    ap.add_argument("--group", type=int, default=0, help="group size (64, 128, 256); 0: autotuned per variant")
     ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
@@ -347,6 +347,15 @@ def main():
     log("direct-sum reference: %d targets x %d sources in %.2f s" % (len(idx), n, time.perf_counter() - t0))
 
     def variant(base, fixed_tol=None):
+        tune = None
+        if base.get("group_size", 0) == 0:
+            # group and leaf size from the autotuner, timed at a nominal tol
+            nominal = fixed_tol or (0.93 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
+            t0 = time.perf_counter()
+            best, ms = eng.autotune(E.default_params(**dict(base, tol=nominal)), reps=2)
+            base = dict(base, group_size=best.group_size, nleaf=best.nleaf)
+            tune = {"nominal_tol": nominal, "candidates": [list(c) for c in E.AUTOTUNE_CANDIDATES],
+                    "ms": [round(float(t), 3) for t in ms], "seconds": time.perf_counter() - t0}
         if fixed_tol is None:
             tol, med, p99 = calibrate_tol(eng, E, base, idx, dacc, args.target)
         else:
@@ -356,6 +365,7 @@ def main():
         sec, stats = time_steps(eng, p, args.steps, args.warmup)
         last = stats[-1]
         return {"usequad": base["usequad"], "learned": base.get("learned", 0), "tol": tol,
+                "group_size": base["group_size"], "nleaf": base.get("nleaf", -1), "autotune": tune,
                 "median_rel_err": med, "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
                 "interactions_per_step": int(last.n2btot + last.nbctot), "n2b": int(last.n2btot),
                 "nbc": int(last.nbctot), "n_mlp": int(last.n_mlp), "ngroups": int(last.ngroups),
@@ -368,15 +378,15 @@ def main():
     plain = []
     for usequad in ((0, 1) if args.quad < 0 else (args.quad,)):
         v = variant(dict(eps=args.eps, usequad=usequad, group_size=args.group), args.tol)
-        log("plain tree usequad=%d: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
-            usequad, v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
+        log("plain tree usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
+            usequad, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
         plain.append(v)
     best_plain = min(plain, key=lambda v: v["ms_per_step"])
     learned = None
     if not args.no_learned:
         learned = variant(dict(eps=args.eps, usequad=1, group_size=args.group, learned=1), args.learned_tol)
-        log("learned walk: tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, %d evaluations)" % (
-            learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
+        log("learned walk: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, "
+            "%d evaluations)" % (learned["group_size"], learned["nleaf"], learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
             learned["t_mlp_kernel_ms"], learned["n_mlp"]))
     head_v = learned if learned is not None else best_plain
     t_walk = head_v["t_walk_kernel_ms"] * 1e-3
@@ -396,7 +406,8 @@ def main():
         "data": "synthetic Plummer (mkplummer recipe, own counter RNG, seed 123)",
         "config": {
             "workload": "plummer_n%d_%s_matched_median_err" % (n, "learned_vs_plain" if learned else "plain_tree"),
-            "n": n, "eps": args.eps, "group_size": args.group, "target_median_err": args.target,
+            "n": n, "eps": args.eps, "group_size": args.group or "autotuned per variant",
+            "target_median_err": args.target,
             "err_subset": len(idx),
             "value_is": "learned" if learned else "plain",
             "learned": learned, "learned_training": learned_train,
@@ -406,7 +417,7 @@ def main():
         "roofline": {
             "bound": "mfma",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
-            "kernel": "k_walk<%d,%s,false,%s,false>" % (2 if args.group == 128 else 1,
+            "kernel": "k_walk<%d,%s,false,%s,false>" % ({64: 1, 128: 2, 256: 4}[head_v["group_size"]],
                                                        "true" if head_v["usequad"] else "false",
                                                        "true" if head_v["learned"] else "false"),
             "achieved": achieved,

@@ -192,6 +192,26 @@ __device__ __forceinline__ void eval_p2c(Lane<NPL> &L, const float4 *src, int cn
     }
 }
 
+// Tiles for the first nq particle slots of each lane (groups of at most
+// 64 nq points leave the other slots idle).
+template <int NPL, bool SOFT0>
+__device__ __forceinline__ void p2p_slots(int nq, Lane<NPL> &L, const float4 *src, int cnt, float eps2)
+{
+    if (nq <= 1) eval_p2p<NPL, 1, SOFT0>(L, src, cnt, eps2);
+    else if (NPL > 2 && nq == 2) eval_p2p<NPL, (NPL > 2 ? 2 : NPL), SOFT0>(L, src, cnt, eps2);
+    else if (NPL > 3 && nq == 3) eval_p2p<NPL, (NPL > 3 ? 3 : NPL), SOFT0>(L, src, cnt, eps2);
+    else eval_p2p<NPL, NPL, SOFT0>(L, src, cnt, eps2);
+}
+
+template <int NPL, bool QUAD, bool SOFT0>
+__device__ __forceinline__ void p2c_slots(int nq, Lane<NPL> &L, const float4 *src, int cnt, float eps2)
+{
+    if (nq <= 1) eval_p2c<NPL, 1, QUAD, SOFT0>(L, src, cnt, eps2);
+    else if (NPL > 2 && nq == 2) eval_p2c<NPL, (NPL > 2 ? 2 : NPL), QUAD, SOFT0>(L, src, cnt, eps2);
+    else if (NPL > 3 && nq == 3) eval_p2c<NPL, (NPL > 3 ? 3 : NPL), QUAD, SOFT0>(L, src, cnt, eps2);
+    else eval_p2c<NPL, NPL, QUAD, SOFT0>(L, src, cnt, eps2);
+}
+
 struct WalkArgs {
     const float4 *thi, *tlo;
     const int *tself;
@@ -260,7 +280,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         if (g >= (int)a.stat[S_NGROUPS]) break;
         const int64_t gbase = a.gfirst[g];
         const int gcount = a.gcount[g];
-        const bool one_slot = NPL > 1 && gcount <= 64;   // wave-uniform
+        const int nq = (gcount + 63) >> 6;             // particle slots in use (wave-uniform)
         // opening and softening of this group's system
         float tolsq = a.tolsq, eps2 = a.eps2;
         int sys = -1;
@@ -478,8 +498,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 }
                 wave_lds_sync();
                 while (np >= 64) {
-                    if (one_slot) eval_p2p<NPL, 1, SOFT0>(L, P, 64, eps2);
-                    else eval_p2p<NPL, NPL, SOFT0>(L, P, 64, eps2);
+                    p2p_slots<NPL, SOFT0>(nq, L, P, 64, eps2);
                     tot_pp += 64;
                     np -= 64;
                     wave_lds_sync();
@@ -489,8 +508,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     wave_lds_sync();
                 }
                 while (nc >= 64) {
-                    if (one_slot) eval_p2c<NPL, 1, QUAD, SOFT0>(L, Cl, 64, eps2);
-                    else eval_p2c<NPL, NPL, QUAD, SOFT0>(L, Cl, 64, eps2);
+                    p2c_slots<NPL, QUAD, SOFT0>(nq, L, Cl, 64, eps2);
                     tot_pc += 64;
                     nc -= 64;
                     wave_lds_sync();
@@ -522,13 +540,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             continue;
         }
         wave_lds_sync();
-        if (one_slot) {
-            if (np > 0) eval_p2p<NPL, 1, SOFT0>(L, P, np, eps2);
-            if (nc > 0) eval_p2c<NPL, 1, QUAD, SOFT0>(L, Cl, nc, eps2);
-        } else {
-            if (np > 0) eval_p2p<NPL, NPL, SOFT0>(L, P, np, eps2);
-            if (nc > 0) eval_p2c<NPL, NPL, QUAD, SOFT0>(L, Cl, nc, eps2);
-        }
+        if (np > 0) p2p_slots<NPL, SOFT0>(nq, L, P, np, eps2);
+        if (nc > 0) p2c_slots<NPL, QUAD, SOFT0>(nq, L, Cl, nc, eps2);
         tot_pp += np;
         tot_pc += nc;
         wave_lds_sync();
@@ -598,7 +611,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
         k_fill_zero<<<std::max<int64_t>(1, (4 * nt + 255) / 256), 256, 0, ctx->stream>>>(phi, nt, acc);
         return;
     }
-    const int npl = (p.group_size == 128) ? 2 : 1;
+    const int npl = p.group_size == 256 ? 4 : (p.group_size == 128 ? 2 : 1);
     WalkArgs a{};
     a.thi = T.thi;
     a.tlo = T.tlo;
@@ -661,7 +674,10 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     const bool crit = p.criterion != BH_CRIT_HACK;
     a.box = T.dbox;
-    if (npl == 2) {
+    if (npl == 4) {
+        if (p.usequad) soft0 ? launch<4, true, true>(ctx, a, grid, crit) : launch<4, true, false>(ctx, a, grid, crit);
+        else soft0 ? launch<4, false, true>(ctx, a, grid, crit) : launch<4, false, false>(ctx, a, grid, crit);
+    } else if (npl == 2) {
         if (p.usequad) soft0 ? launch<2, true, true>(ctx, a, grid, crit) : launch<2, true, false>(ctx, a, grid, crit);
         else soft0 ? launch<2, false, true>(ctx, a, grid, crit) : launch<2, false, false>(ctx, a, grid, crit);
     } else {
