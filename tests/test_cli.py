@@ -79,6 +79,7 @@ def test_hackforce_refuses_existing_output(golden, tmp_path):
     (["in=a", "out=b", "bogus=1"], b"unknown"),
     (["in=a", "out=b", "mode=fastest"], b"mode"),
     (["in=a", "out=b", "tol=abc"], b"not a number"),
+    (["in=a", "out=b", "criterion=bh87"], b"criterion=bh87"),
     (["in=/nonexistent/a", "out=b"], b"cannot open"),
 ])
 def test_hackforce_argument_errors(tmp_path, args, msg):
@@ -131,6 +132,30 @@ def test_hackforce_strict_matches_oracle_and_kat(golden, tmp_path):
     assert int(dg.find("n2btot").data.reshape(-1)[0]) == n2b and int(dg.find("nbctot").data.reshape(-1)[0]) == nbc
     assert n2b + nbc == 3864
     assert np.array_equal(prt.find("PhaseSpace").data[:, 0, :], s.pos)
+
+
+@gpu
+@pytest.mark.parametrize("crit", ["rcrit", "bh86", "sw94"])
+def test_hackforce_criterion_keyword_strict(tmp_path, crit):
+    """criterion= in strict mode: the interaction totals written to
+    Diagnostics equal treecode1's (restated) over the same root box, and the
+    forces agree with it to FP64 rounding."""
+    from nemo_amd import engine as E
+    from oracle import oracle
+
+    m, x, v = E.mkplummer(3000, seed=17)
+    x = x * (1.9 / np.abs(x).max())      # inside the default box [-2, 2)^3
+    write_stream(str(tmp_path / "in"), [(m, x, v, 0.0)])
+    r = run("hackforce", "in", "out", "mode=strict", "tol=0.6", "eps=0.02", "usequad=t", "criterion=" + crit,
+            cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    snap = [it for it in _items(str(tmp_path / "out")) if it.tag == "SnapShot"][0]
+    prt, dg = snap.find("Particles"), snap.find("Diagnostics")
+    t = oracle.tc_force(m, x, [-2.0, -2.0, -2.0], 4.0, theta=0.6, eps=0.02, criterion=crit, usequad=True)
+    assert int(dg.find("n2btot").data.reshape(-1)[0]) == t["n2b"].sum()
+    assert int(dg.find("nbctot").data.reshape(-1)[0]) == t["nbc"].sum()
+    a = prt.find("Acceleration").data
+    assert np.abs(a - t["acc"]).max() <= 1e-11 * np.abs(t["acc"]).max()
 
 
 @gpu
@@ -352,3 +377,12 @@ def test_plugin_matches_bh_force():
     accf = np.zeros((5000, 3), np.float32)
     f(3, 0.0, 5000, vp(mf), vp(xf), None, None, vp(potf), vp(accf), 0, b"f")
     np.testing.assert_allclose(accf, acc0, rtol=1e-3, atol=1e-3 * np.abs(acc0).max())
+    # fourth parameter: treecode1's sw94 criterion (bh_criterion 3)
+    pars4 = (C.c_double * 4)(0.7, 0.02, 1.0, 3.0)
+    lib.iniacceleration(pars4, 4, None, C.byref(f), C.byref(nm), C.byref(nv))
+    eng = E.Engine(0)
+    phi3, acc3, _ = eng.force(m, x, E.default_params(tol=0.7, eps=0.02, usequad=1, criterion="sw94"))
+    eng.close()
+    f(3, 0.0, 5000, vp(m), vp(x), None, None, vp(pot), vp(acc), 0, b"d")
+    assert np.array_equal(pot, phi3) and np.array_equal(acc, acc3)
+    assert not np.array_equal(acc3, acc0)
