@@ -388,7 +388,9 @@ def main():
         log("learned walk: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, "
             "%d evaluations)" % (learned["group_size"], learned["nleaf"], learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
             learned["t_mlp_kernel_ms"], learned["n_mlp"]))
-    head_v = learned if learned is not None else best_plain
+    # value: the faster of the two at the target median error (both are in
+    # config, with their ratio)
+    head_v = learned if learned is not None and learned["evals_per_s"] > best_plain["evals_per_s"] else best_plain
     t_walk = head_v["t_walk_kernel_ms"] * 1e-3
     achieved = FLOP_PER_INTERACTION * head_v["interactions_per_step"] / t_walk / 1e12
     out = {
@@ -409,7 +411,7 @@ def main():
             "n": n, "eps": args.eps, "group_size": args.group or "autotuned per variant",
             "target_median_err": args.target,
             "err_subset": len(idx),
-            "value_is": "learned" if learned else "plain",
+            "value_is": "learned" if head_v is learned else "plain",
             "learned": learned, "learned_training": learned_train,
             "plain": best_plain, "plain_variants": plain,
             "learned_over_plain": (learned["evals_per_s"] / best_plain["evals_per_s"]) if learned else None,
