@@ -8,7 +8,7 @@
  * write_snapshot); the force pass is bh_force of libnemo_bh.  Invoked as
  * hackforce_qp (Makefile:86-91 of the reference) it uses quadrupole cells.
  *
- * Keywords added by this build: usequad= (default: by program name),
+ * Keywords added here: usequad= (default: by program name),
  * mode=fast|strict|direct, group=, nleaf=, criterion=hack|rcrit|bh86|sw94
  * (bh_criterion: treecode1's opening tests, treeload.c:232-252), device=.  fcells= is accepted and
  * ignored: the device tree sizes itself.  Errors print

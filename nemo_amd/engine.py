@@ -111,7 +111,7 @@ def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIBPATH):
-            raise ImportError("libnemo_bh.so not built: run `make -C nemo_amd` or __graft_entry__.build()")
+            raise ImportError("libnemo_bh.so not built: run `make -C nemo_amd`")
         L = C.CDLL(LIBPATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
