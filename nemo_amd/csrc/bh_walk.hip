@@ -561,7 +561,10 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
             double ph = (double)L.ph[q] + (double)L.pl[q];
             if (sSelf[w][k]) {
                 nself++;
-                if (!SOFT0) ph -= (double)(a.bhi[gbase + k].w * rsq(eps2));
+                // the self pair added m rsq(eps^2) unless eps = 0 masked it
+                // (a batch with an unsoftened system runs the masking variant
+                // for every system)
+                if (eps2 > 0.f) ph -= (double)(a.bhi[gbase + k].w * rsq(eps2));
             }
             const int64_t o = a.tperm[gbase + k];
             a.phi[o] = -ph;

@@ -129,3 +129,19 @@ def test_autotune_picks_fastest_and_becomes_default(eng):
     st = eng.eval(E.default_params(usequad=1))      # group_size 0 / nleaf -1 -> tuned
     st2 = eng.eval(E.default_params(usequad=1, group_size=best.group_size, nleaf=best.nleaf))
     assert (st.n2btot, st.nbctot) == (st2.n2btot, st2.nbctot)
+
+
+def test_batch_with_an_unsoftened_system(eng):
+    """One system with eps = 0 switches the batch walk to the variant that
+    masks r = 0; every system still equals its own evaluation bit for bit."""
+    off, m, x, v, eps, tol = ensemble(6, 300, 3000, seed=11)
+    eps[2] = 0.0
+    p = E.default_params(usequad=1, group_size=128, nleaf=2)
+    eng.batch_upload(off, m, x, eps, tol)
+    eng.eval(p)
+    phi, acc = eng.download()
+    for s in range(len(off) - 1):
+        a, b = off[s], off[s + 1]
+        ps, as_, _ = eng.force(m[a:b], x[a:b], E.default_params(eps=eps[s], tol=tol[s], usequad=1,
+                                                                group_size=128, nleaf=2))
+        assert np.array_equal(phi[a:b], ps) and np.array_equal(acc[a:b], as_), s

@@ -124,3 +124,21 @@ def test_sixteen_million_bodies(eng):
     e = _rel(acc[idx], acc_d)
     assert np.median(e) < 3e-3 and np.percentile(e, 99) < 2e-2
     assert st.ngroups > n // 128 and st.ncell > n // 4
+
+
+@pytest.mark.parametrize("usequad", [0, 1])
+def test_zero_softening(eng, usequad):
+    """eps = 0 (unsoftened gravity): strict equals the oracle bit for bit; the
+    fast walk masks the r = 0 self pair and equals the direct sum at tol = 0."""
+    m, x, _ = E.mkplummer(3000, seed=8)
+    phi, acc, st = eng.force(m, x, E.default_params(tol=0.7, eps=0.0, usequad=usequad, mode="strict"))
+    rphi, racc, n2b, nbc, _, _ = oracle.force(m, x, tol=0.7, eps=0.0, usequad=bool(usequad))
+    assert (st.n2btot, st.nbctot) == (n2b, nbc)
+    assert np.array_equal(phi, rphi) and np.array_equal(acc, racc)
+    dphi, dacc = oracle.direct(m, x, 0.0)
+    phi, acc, st = eng.force(m, x, E.default_params(tol=0.0, eps=0.0, usequad=usequad))
+    assert np.isfinite(phi).all() and np.isfinite(acc).all()
+    assert np.max(np.abs(phi - dphi) / np.abs(dphi)) < 1e-5
+    assert _rel(acc, dacc).max() < 1e-4
+    phi, acc, st = eng.force(m, x, E.default_params(tol=0.7, eps=0.0, usequad=usequad))
+    assert np.median(_rel(acc, dacc)) < 5e-3
