@@ -43,12 +43,14 @@ def test_default_run_strict_matches_reference_kats(eng, golden):
         assert abs(a.etot - b.etot) < 1e-14
 
 
-@pytest.mark.parametrize("mode", ["strict", "fast"])
-def test_graph_replay_equals_eager(eng, mode):
+@pytest.mark.parametrize("mode, criterion", [("strict", "hack"), ("fast", "hack"), ("fast", "sw94"),
+                                             ("strict", "rcrit")])
+def test_graph_replay_equals_eager(eng, mode, criterion):
     """Steps captured in a hipGraph and replayed give bit-identical states to
-    eager steps, replay after replay (with eager work in between)."""
+    eager steps, replay after replay (with eager work in between); with
+    treecode1's criteria the per-cell Rcrit2 pass is part of the graph."""
     m, x, v = E.mkplummer(20000, seed=9)
-    kw = dict(tol=0.7, eps=0.02, usequad=1, mode=mode)
+    kw = dict(tol=0.7, eps=0.02, usequad=1, mode=mode, criterion=criterion)
     a = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, **kw)
     da, na, ca = a.run(12, graph=False)
     sa = a.state()

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from nemo_amd import engine as E
-from oracle import mlp_oracle
+from oracle import mlp_oracle, oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +90,25 @@ def test_training_reduces_loss_and_error(eng):
     eng.eval(pl)
     _, a2 = eng.download()
     assert np.array_equal(a1, a2)
+
+
+def test_learned_correction_on_test_points(eng):
+    """The model trained by the previous test applied to a separate test
+    set: the groups are cut from the test points' own topology and the
+    correction still lowers the error against the direct sum."""
+    m, x, _ = E.mkplummer(200000, seed=34)
+    t = x + 1e-4          # as dense as the mass set, so the groups are alike
+    eng.upload(m, x, tpos=t)
+    p = E.default_params(tol=1.0, eps=0.01, usequad=1)
+    eng.eval(p)
+    _, a0 = eng.download()
+    st = eng.eval(E.default_params(tol=1.0, eps=0.01, usequad=1, learned=1))
+    _, a1 = eng.download()
+    allpos = np.concatenate([x, t])
+    allm = np.concatenate([m, np.zeros(len(t))])
+    sel = np.sort(np.random.default_rng(3).permutation(len(t))[:2000])
+    _, dacc = oracle.direct(allm, allpos, 0.01, len(x) + sel)
+    e0 = np.linalg.norm(a0[sel] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    e1 = np.linalg.norm(a1[sel] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+    assert st.n_mlp > 0
+    assert np.median(e1) < 0.85 * np.median(e0), (np.median(e0), np.median(e1))
