@@ -353,7 +353,11 @@ def main():
             nominal = fixed_tol or (0.93 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
             t0 = time.perf_counter()
             best, ms = eng.autotune(E.default_params(**dict(base, tol=nominal)), reps=2)
-            base = dict(base, group_size=best.group_size, nleaf=best.nleaf)
+            # near-ties (within 2 %) go to the larger group: its conservative
+            # opening is more accurate at the same tol
+            near = [k for k, t in enumerate(ms) if t <= 1.02 * min(ms)]
+            g, nl = max((E.AUTOTUNE_CANDIDATES[k] for k in near), key=lambda c: (c[0], -ms[E.AUTOTUNE_CANDIDATES.index(c)]))
+            base = dict(base, group_size=g, nleaf=nl)
             tune = {"nominal_tol": nominal, "candidates": [list(c) for c in E.AUTOTUNE_CANDIDATES],
                     "ms": [round(float(t), 3) for t in ms], "seconds": time.perf_counter() - t0}
         if fixed_tol is None:

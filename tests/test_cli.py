@@ -296,6 +296,24 @@ def test_hackcode1_fast_small_system(golden, tmp_path):
 
 
 @gpu
+def test_hackcode1_criterion_keyword(golden, tmp_path):
+    """criterion= in hackcode1 (strict walk): the t=0 interaction total of the
+    log equals treecode1's (restated) over hackcode1's box."""
+    from oracle import oracle
+
+    src = os.path.join(golden, "hack128.sun4")
+    r = run("hackcode1", "in=" + src, "out=c", "mode=strict", "criterion=sw94", "tstop=0.25", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    rows = [x for x in (l.split() for l in r.stdout.decode().splitlines())
+            if len(x) == 7 and x[0].replace(".", "").isdigit()]
+    assert len(rows) == 9
+    s0 = snapshot.read(src)[0]
+    assert np.abs(s0.pos).max() < 2.0
+    t = oracle.tc_force(s0.mass, s0.pos, [-2.0, -2.0, -2.0], 4.0, theta=1.0, eps=0.05, criterion="sw94")
+    assert int(rows[0][3]) == t["n2b"].sum() + t["nbc"].sum()
+
+
+@gpu
 def test_hackcode1_save_and_restart_continue_bitwise(golden, tmp_path):
     """A run saved at t=1 and restarted to t=2 ends bit-identical to a
     straight run to t=2 (strict and fast walk)."""
