@@ -222,7 +222,7 @@ def run_ensemble(args, E, log):
                                 "ms_per_step": t_lf / args.lf_steps * 1e3,
                                 "interactions_per_s": float((n2b + nbc).sum() / t_lf),
                                 "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}},
-        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false,false>" % (2 if best.group_size == 128 else 1)),
+        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false,false>" % (best.group_size // 64)),
     }
 
 
@@ -230,19 +230,21 @@ def run_diskhalo(args, E, log):
     """Exponential disk (N = 2^21, M = 0.1, scale length 1, sech^2 height
     0.1, Q = 1.5) in a Hernquist halo (N = 2^22, M = 1, a = 4), eps = 0.02:
     force error of the first evaluation, then 256 graph-captured leapfrog
-    steps at dt = 1/64 (tol 0.7, quadrupole cells)."""
+    steps at dt = 1/64 (tol 0.7, quadrupole cells, 192-point groups unless
+    --group names another size)."""
     nd, nh = args.ndisk, args.nhalo
+    group = args.group or 192
     m, x, v = E.mkdiskhalo(nd, nh, seed=args.seed)
     n = len(m)
     eng = E.Engine(0)
     eng.upload(m, x)
     idx = np.sort(np.random.default_rng(args.seed).permutation(n)[: args.nsub])
     _, dacc = eng.direct(0.02, idx)
-    p = E.default_params(tol=args.dh_tol, eps=0.02, usequad=1, rsize=4.0)
+    p = E.default_params(tol=args.dh_tol, eps=0.02, usequad=1, rsize=4.0, group_size=group)
     st = eng.eval(p)
     _, acc = eng.download()
     e = rel_err(acc[idx], dacc)
-    lf = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, tol=args.dh_tol, eps=0.02, usequad=1)
+    lf = E.Leapfrog(eng, m, x, v, 1.0 / 64.0, tol=args.dh_tol, eps=0.02, usequad=1, group_size=group)
     lf.run(1, graph=True)
     t0 = time.perf_counter()
     d, n2b, nbc = lf.run(args.lf_steps, graph=True)
@@ -255,12 +257,14 @@ def run_diskhalo(args, E, log):
         "ms_per_step": t_lf / args.lf_steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic disk + Hernquist halo (own recipe and RNG, seed %d)" % args.seed,
         "config": {"workload": "diskhalo_%d_%d_leapfrog" % (nd, nh), "n": n, "eps": 0.02, "tol": args.dh_tol,
+                   "group_size": group,
                    "usequad": 1, "dt": 1.0 / 64.0, "evals_per_s": n * args.lf_steps / t_lf,
                    "interactions_per_s": float((n2b + nbc).sum() / t_lf),
                    "relative_energy_drift": float(drift), "energy_first_last": [float(d[0, 0]), float(d[-1, 0])],
                    "first_eval": {"median_rel_err": float(np.median(e)), "p99_rel_err": float(np.percentile(e, 99)),
                                   "t_tree_ms": st.t_tree_ms, "t_walk_kernel_ms": st.t_walk_kernel_ms}},
-        "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3, "k_walk<2,true,false,false,false>"),
+        "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3,
+                                  "k_walk<%d,true,false,false,false>" % (group // 64)),
     }
 
 
@@ -283,7 +287,7 @@ def main():
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--nsub", type=int, default=8192)
     ap.add_argument("--tol", type=float, default=None, help="skip the plain calibration and use this tol")
-    ap.add_argument("--group", type=int, default=0, help="group size (64, 128, 256); 0: autotuned per variant")
+    ap.add_argument("--group", type=int, default=0, help="group size (64, 128, 192, 256); 0: autotuned per variant")
     ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
@@ -423,7 +427,7 @@ def main():
         "roofline": {
             "bound": "mfma",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
-            "kernel": "k_walk<%d,%s,false,%s,false>" % ({64: 1, 128: 2, 256: 4}[head_v["group_size"]],
+            "kernel": "k_walk<%d,%s,false,%s,false>" % (head_v["group_size"] // 64,
                                                        "true" if head_v["usequad"] else "false",
                                                        "true" if head_v["learned"] else "false"),
             "achieved": achieved,

@@ -74,7 +74,7 @@ typedef struct {
     double rsize;        /* initial box side (hackforce rsize=4.0) */
     double rmin[3];      /* initial lower corner, used when rmin_given */
     int rmin_given;      /* 0: rmin = -rsize/2 in every dimension (hackforce.c:189-193) */
-    int group_size;      /* fast mode: particles per group (0 = tuned default, else 64, 128 or 256) */
+    int group_size;      /* fast mode: particles per group (0 = tuned default, else 64, 128, 192 or 256) */
     int learned;         /* fast mode: apply the loaded MLP correction to accepted cells */
     int keep_box;        /* bh_eval: start from the box of the previous call (hackcode1 semantics) */
     int nleaf;           /* fast mode: a cell holding <= nleaf bodies interacts body by body (exact);
@@ -186,12 +186,12 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
                            const double *pos, const double *vel, const double *eps, const double *tol, double dt);
 
 /* Autotuner (no reference counterpart): times the fast walk on the uploaded
- * problem (single or batch) for group_size in {64, 128, 256} x nleaf in
+ * problem (single or batch) for group_size in {64, 128, 192, 256} x nleaf in
  * {0, 1, 2, 4, 8}, best of reps evaluations each (device time of tree +
  * walk), and makes the fastest the default used when group_size = 0 and
  * nleaf = -1.  ms (may be NULL) receives BH_AUTOTUNE_NCAND times in that
  * order; best (may be NULL) the chosen parameters. */
-#define BH_AUTOTUNE_NCAND 15
+#define BH_AUTOTUNE_NCAND 20
 bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms);
 
 /* Learned far-field correction (no reference counterpart).  A 12-64-64-64-3

@@ -55,7 +55,7 @@ static gp_keyword keys[] = {
     {"log", "-", "logging output", NULL},
     {"usequad", "", "quadrupole cells (default: true when invoked as hackcode1_qp)", NULL},
     {"mode", "fast", "fast (FP32 group walk) or strict (FP64, reference order)", NULL},
-    {"group", "0", "particles per group in fast mode (0 = default, 64, 128 or 256)", NULL},
+    {"group", "0", "particles per group in fast mode (0 = default, 64, 128, 192 or 256)", NULL},
     {"nleaf", "-1", "cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
     {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},

@@ -40,7 +40,7 @@ static gp_keyword keys[] = {
     {"fcells", "0.75", "Cell/body allocation ratio (accepted, not used: the device tree sizes itself)", NULL},
     {"usequad", "", "Quadrupole cells (default: true when invoked as hackforce_qp)", NULL},
     {"mode", "fast", "fast (FP32 group walk), strict (FP64, reference order), direct (FP64 direct sum)", NULL},
-    {"group", "0", "Particles per group in fast mode (0 = default, 64, 128 or 256)", NULL},
+    {"group", "0", "Particles per group in fast mode (0 = default, 64, 128, 192 or 256)", NULL},
     {"nleaf", "-1", "Cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
     {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},

@@ -52,8 +52,9 @@ void check_params(const bh_params *p)
     if (!(p->eps >= 0.0) || !std::isfinite(p->eps)) throw ApiFail(BH_ERR_ARG, "eps must be finite and >= 0");
     if (!(p->rsize > 0.0) || !std::isfinite(p->rsize)) throw ApiFail(BH_ERR_ARG, "rsize must be finite and > 0");
     if (p->mode < BH_MODE_FAST || p->mode > BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "unknown mode");
-    if (p->group_size != 0 && p->group_size != 64 && p->group_size != 128 && p->group_size != 256)
-        throw ApiFail(BH_ERR_ARG, "group_size must be 0, 64, 128 or 256");
+    if (p->group_size != 0 && p->group_size != 64 && p->group_size != 128 && p->group_size != 192 &&
+        p->group_size != 256)
+        throw ApiFail(BH_ERR_ARG, "group_size must be 0, 64, 128, 192 or 256");
     if (p->rmin_given)
         for (int k = 0; k < 3; k++)
             if (!std::isfinite(p->rmin[k])) throw ApiFail(BH_ERR_ARG, "rmin must be finite");
@@ -665,8 +666,8 @@ bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best
         check_params(p);
         if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_autotune before an upload");
         if (p->mode != BH_MODE_FAST) throw ApiFail(BH_ERR_ARG, "the autotuner tunes the fast walk");
-        static const int groups[3] = {64, 128, 256}, leaves[5] = {0, 1, 2, 4, 8};
-        static_assert(3 * 5 == BH_AUTOTUNE_NCAND, "candidate count");
+        static const int groups[4] = {64, 128, 192, 256}, leaves[5] = {0, 1, 2, 4, 8};
+        static_assert(4 * 5 == BH_AUTOTUNE_NCAND, "candidate count");
         double tbest = 1e300;
         bh_params qbest = *p;
         int k = 0;

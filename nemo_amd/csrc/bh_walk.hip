@@ -614,7 +614,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
         k_fill_zero<<<std::max<int64_t>(1, (4 * nt + 255) / 256), 256, 0, ctx->stream>>>(phi, nt, acc);
         return;
     }
-    const int npl = p.group_size == 256 ? 4 : (p.group_size == 128 ? 2 : 1);
+    const int npl = p.group_size / 64;
     WalkArgs a{};
     a.thi = T.thi;
     a.tlo = T.tlo;
@@ -677,7 +677,10 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     const bool crit = p.criterion != BH_CRIT_HACK;
     a.box = T.dbox;
-    if (npl == 4) {
+    if (npl == 3) {
+        if (p.usequad) soft0 ? launch<3, true, true>(ctx, a, grid, crit) : launch<3, true, false>(ctx, a, grid, crit);
+        else soft0 ? launch<3, false, true>(ctx, a, grid, crit) : launch<3, false, false>(ctx, a, grid, crit);
+    } else if (npl == 4) {
         if (p.usequad) soft0 ? launch<4, true, true>(ctx, a, grid, crit) : launch<4, true, false>(ctx, a, grid, crit);
         else soft0 ? launch<4, false, true>(ctx, a, grid, crit) : launch<4, false, false>(ctx, a, grid, crit);
     } else if (npl == 2) {

@@ -103,7 +103,7 @@ SIGNATURES = {
 # bh_criterion: hackcode1's opening test, treecode1's default / bh86 / sw94
 CRITERIA = {"hack": 0, "rcrit": 1, "bh86": 2, "sw94": 3}
 
-AUTOTUNE_CANDIDATES = [(g, nl) for g in (64, 128, 256) for nl in (0, 1, 2, 4, 8)]
+AUTOTUNE_CANDIDATES = [(g, nl) for g in (64, 128, 192, 256) for nl in (0, 1, 2, 4, 8)]
 
 
 def lib():

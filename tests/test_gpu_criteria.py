@@ -63,7 +63,7 @@ def test_strict_criterion_test_points(eng, plum, crit):
     assert np.abs(acc - r["acc"][n:]).max() <= 1e-11 * np.linalg.norm(r["acc"][n:], axis=1).max()
 
 
-@pytest.mark.parametrize("group", [64, 128, 256])
+@pytest.mark.parametrize("group", [64, 128, 192, 256])
 @pytest.mark.parametrize("crit", ["rcrit", "bh86", "sw94"])
 def test_fast_criterion_is_conservative(eng, plum, crit, group):
     """The group test accepts a cell only when every body of the group

@@ -49,7 +49,7 @@ def test_one_two_three_bodies(eng, mode):
 
 
 @pytest.mark.parametrize("n", [3, 64, 100, 128, 129, 200, 256, 257, 700])
-@pytest.mark.parametrize("group", [64, 128, 256])
+@pytest.mark.parametrize("group", [64, 128, 192, 256])
 def test_small_systems_fast(eng, n, group):
     """Systems of at most one group (and just above): every body gets its
     force (a single group when N <= G)."""

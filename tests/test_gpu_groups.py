@@ -83,7 +83,7 @@ def eng():
     e.close()
 
 
-@pytest.mark.parametrize("G", [64, 128, 256])
+@pytest.mark.parametrize("G", [64, 128, 192, 256])
 def test_groups_follow_the_packing_rule(eng, G):
     m, x, _ = E.mkplummer(20000, seed=77)
     p = E.default_params(tol=0.7, eps=0.02, group_size=G, nleaf=0)
