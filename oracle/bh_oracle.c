@@ -556,28 +556,77 @@ static double now_s(void)
 }
 
 /* Build the full tree (timed), then evaluate the bodies idx[0..ns) of the
- * mass set, each skipping itself (timed).  The caller extrapolates
- * per-body walk time to the whole set. */
-int orc_force_sample(int n, const double *mass, const double *pos, int ns, const int64_t *idx,
-                     double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
-                     double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, double *t_tree, double *t_walk)
+ * mass set, each skipping itself (timed), on nthreads host threads sharing
+ * the read-only tree (each thread walks a contiguous slice of idx).  The
+ * caller extrapolates per-body walk time to the whole set. */
+#include <pthread.h>
+
+typedef struct {
+    const orc_tree *t;
+    const double *pos;
+    const int64_t *idx;
+    int k0, k1;
+    double tol, eps;
+    double *phi, *acc;
+    int64_t n2b, nbc;
+} orc_slice;
+
+static void *orc_walk_slice(void *arg)
 {
+    orc_slice *s = (orc_slice *)arg;
+    s->n2b = s->nbc = 0;
+    for (int k = s->k0; k < s->k1; k++) {
+        int a, b;
+        int64_t i = s->idx[k];
+        orc_tree_grav(s->t, s->pos + 3 * i, (int)i, s->tol, s->eps, s->phi + k, s->acc + 3 * (size_t)k, &a, &b);
+        s->n2b += a;
+        s->nbc += b;
+    }
+    return NULL;
+}
+
+int orc_force_sample_mt(int n, const double *mass, const double *pos, int ns, const int64_t *idx,
+                        double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
+                        int nthreads, double *phi, double *acc, int64_t *n2btot, int64_t *nbctot,
+                        double *t_tree, double *t_walk)
+{
+    enum { MAXT = 256 };
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > MAXT) nthreads = MAXT;
     orc_tree *t;
     double t0 = now_s();
     int err = orc_tree_build(&t, n, mass, pos, rmin, rsize, (int)(fcells * n), usequad);
     double t1 = now_s();
     *n2btot = *nbctot = 0;
     if (err) { orc_tree_free(t); return err; }
-    for (int k = 0; k < ns; k++) {
-        int a, b;
-        int64_t i = idx[k];
-        orc_tree_grav(t, pos + 3 * i, (int)i, tol, eps, phi + k, acc + 3 * (size_t)k, &a, &b);
-        *n2btot += a;
-        *nbctot += b;
+    orc_slice sl[MAXT];
+    pthread_t th[MAXT];
+    int started[MAXT] = {0};
+    for (int j = 0; j < nthreads; j++) {
+        sl[j] = (orc_slice){t, pos, idx, (int)((int64_t)ns * j / nthreads), (int)((int64_t)ns * (j + 1) / nthreads),
+                            tol, eps, phi, acc, 0, 0};
+        if (j > 0) started[j] = pthread_create(&th[j], NULL, orc_walk_slice, &sl[j]) == 0;
     }
+    /* slice 0, and any slice whose thread could not start, runs on this thread */
+    for (int j = 0; j < nthreads; j++)
+        if (!started[j]) orc_walk_slice(&sl[j]);
+    for (int j = 1; j < nthreads; j++)
+        if (started[j]) pthread_join(th[j], NULL);
     double t2 = now_s();
+    for (int j = 0; j < nthreads; j++) {
+        *n2btot += sl[j].n2b;
+        *nbctot += sl[j].nbc;
+    }
     *t_tree = t1 - t0;
     *t_walk = t2 - t1;
     orc_tree_free(t);
     return 0;
+}
+
+int orc_force_sample(int n, const double *mass, const double *pos, int ns, const int64_t *idx,
+                     double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
+                     double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, double *t_tree, double *t_walk)
+{
+    return orc_force_sample_mt(n, mass, pos, ns, idx, tol, eps, usequad, rmin, rsize, fcells, 1, phi, acc,
+                               n2btot, nbctot, t_tree, t_walk);
 }

@@ -45,6 +45,9 @@ def lib():
         L.orc_force_sample.argtypes = [C.c_int, _d, _d, C.c_int, _i64, C.c_double, C.c_double, C.c_int,
                                        _d, _d, C.c_double, _d, _d, _i64, _i64, _d, _d]
         L.orc_force_sample.restype = C.c_int
+        L.orc_force_sample_mt.argtypes = [C.c_int, _d, _d, C.c_int, _i64, C.c_double, C.c_double, C.c_int,
+                                          _d, _d, C.c_double, C.c_int, _d, _d, _i64, _i64, _d, _d]
+        L.orc_force_sample_mt.restype = C.c_int
         L.orc_diagnostics.argtypes = [C.c_int, _d, _d, _d, _d, _d, _d]
         L.orc_morton.argtypes = [C.c_int, _d, _d, C.c_double, _u64]
         L.orc_tree_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _d, _d, _d, _d, C.c_int, C.c_int]
@@ -89,9 +92,10 @@ def force(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fce
     return phi, acc, int(a[0]), int(b[0]), rmin, float(rs[0])
 
 
-def force_sample(mass, pos, idx, tol, eps, usequad, rsize=4.0, fcells=2.0):
+def force_sample(mass, pos, idx, tol, eps, usequad, rsize=4.0, fcells=2.0, nthreads=1):
     """Full tree build plus the walk of the bodies idx (self skipped), timed
-    in C.  Returns (phi, acc, n2b, nbc, t_tree_s, t_walk_s)."""
+    in C; the walk runs on nthreads host threads over one shared tree.
+    Returns (phi, acc, n2b, nbc, t_tree_s, t_walk_s)."""
     mass = np.ascontiguousarray(mass, np.float64)
     pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
     idx = np.ascontiguousarray(idx, np.int64)
@@ -103,8 +107,9 @@ def force_sample(mass, pos, idx, tol, eps, usequad, rsize=4.0, fcells=2.0):
     b = np.zeros(1, np.int64)
     tt = np.zeros(1)
     tw = np.zeros(1)
-    err = lib().orc_force_sample(len(mass), _p(mass), _p(pos), len(idx), _p(idx, _i64), tol, eps, int(usequad),
-                                 _p(rmin), _p(rs), fcells, _p(phi), _p(acc), _p(a, _i64), _p(b, _i64), _p(tt), _p(tw))
+    err = lib().orc_force_sample_mt(len(mass), _p(mass), _p(pos), len(idx), _p(idx, _i64), tol, eps, int(usequad),
+                                    _p(rmin), _p(rs), fcells, int(nthreads), _p(phi), _p(acc), _p(a, _i64),
+                                    _p(b, _i64), _p(tt), _p(tw))
     if err:
         raise RuntimeError("oracle tree error %d" % err)
     return phi, acc, int(a[0]), int(b[0]), float(tt[0]), float(tw[0])

@@ -176,3 +176,16 @@ def test_zero_mass_bodies_feel_but_do_not_pull():
     assert np.allclose(phi, dphi, rtol=1e-13)
     assert np.allclose(acc, dacc, rtol=1e-12, atol=1e-15)
     assert keep.sum() < 50
+
+
+def test_threaded_sample_equals_single_thread_and_force():
+    """The timed sample of the CPU baseline: the walk split over host threads
+    (one shared tree) gives the single-thread results bit for bit, and both
+    equal the full hackforce evaluation at those bodies."""
+    mass, pos = _rand_bodies(3000, 11)
+    idx = np.sort(np.random.default_rng(2).permutation(3000)[:700])
+    r1 = oracle.force_sample(mass, pos, idx, 0.8, 0.03, True, nthreads=1)
+    r4 = oracle.force_sample(mass, pos, idx, 0.8, 0.03, True, nthreads=5)
+    assert np.array_equal(r1[0], r4[0]) and np.array_equal(r1[1], r4[1]) and r1[2:4] == r4[2:4]
+    phi, acc, *_ = oracle.force(mass, pos, tol=0.8, eps=0.03, usequad=True)
+    assert np.array_equal(r1[0], phi[idx]) and np.array_equal(r1[1], acc[idx])
