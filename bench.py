@@ -15,8 +15,9 @@ trained on the device from residual samples of three other Plummer
 snapshots (seeds 41, 43 and 47, tol 0.8/1.0/1.2, cells with size/r in
 [0.45, 1.3]), then the learned walk (quadrupole cells + correction) is
 calibrated to the same target on the seed-123 snapshot.
-`value` is the learned variant's evals/s (the plain figure and the ratio are
-in `config`); with --no-learned it is the plain figure.
+`value` is the evals/s of the faster of the two (`config.value_is` names it;
+both variants and their ratio `learned_over_plain` are in `config`); with
+--no-learned it is the plain figure.
 
 Prints one JSON line.  `roofline` is for the dominant kernel, the walk: 20
 flop per interaction (P2P and P2C, counted by the kernel) over the walk
@@ -149,6 +150,34 @@ def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stri
                 n, list(seeds), list(tols), stride), "size_over_r_range": [rlo, rhi],
             "epochs": epochs, "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
             "t_collect_s": t1 - t0, "t_train_s": t2 - t1}
+
+
+def profiled_traffic(kernel, profile="walk_plummer4M_plain_mono.md"):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a device copy of known
+    size), when that summary's dominant kernel is this kernel; else None.
+    Counters cannot be read from inside the timed process, so the figure is
+    the one measured on the same workload by the profile's command."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", profile)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None, None
+    want = kernel.replace(" ", "")
+    dom, in_traffic = None, False
+    for line in text.splitlines():
+        if line.startswith("Dominant kernel:") and "`" in line:
+            dom = line.split("`")[1].replace(" ", "")
+        if line.startswith("## "):
+            in_traffic = line.startswith("## HBM traffic")
+        if (in_traffic and dom == want and line.startswith("| `")
+                and line.split("`")[1].replace(" ", "") == want):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[-1]), "profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; scaled)" % profile
+            except ValueError:
+                return None, None
+    return None, None
 
 
 def walk_roofline(n_int, t_walk_s, kernel):
@@ -449,6 +478,10 @@ def main():
             "flop_convention": "20 flop per interaction (P2P and P2C), interactions counted by the kernel",
         },
     }
+    traffic, src = profiled_traffic(out["roofline"]["kernel"])
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_source"] = src
     if learned and learned["t_mlp_kernel_ms"] > 0:
         a_mlp = FLOP_PER_MLP * learned["n_mlp"] / (learned["t_mlp_kernel_ms"] * 1e-3) / 1e12
         out["roofline_mlp"] = {"bound": "mfma", "kernel": "k_learned_reg", "achieved": a_mlp,
