@@ -292,6 +292,12 @@ bh_status bh_create(bh_ctx **out, int device)
         delete ctx;
         return BH_ERR_DEVICE;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return BH_ERR_DEVICE;
+    }
+    for (auto &e : ctx->fork_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->ncu = prop.multiProcessorCount;
@@ -305,6 +311,7 @@ void bh_destroy(bh_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
     lf_drop_graph(ctx);
     bh::DevBuf *bufs[] = {&ctx->mass,   &ctx->pos,    &ctx->tpos,   &ctx->dbox,    &ctx->dboxi,  &ctx->keys0,
                           &ctx->keys1,  &ctx->vals0,  &ctx->vals1,  &ctx->hist,    &ctx->scan_tmp, &ctx->tkeys0,
@@ -329,6 +336,9 @@ void bh_destroy(bh_ctx *ctx)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->fork_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -24,6 +24,10 @@ struct bh_ctx {
     int device = 0;
     int ncu = 256;
     hipStream_t stream = nullptr;
+    // second stream of the tree build (same priority): branches that no
+    // kernel of the main stream needs until the matching join
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev[6] = {};
     std::string err;
     int inject = 0;
 

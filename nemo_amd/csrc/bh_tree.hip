@@ -1406,6 +1406,38 @@ static void build_groups(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo,
     make_groups(ctx, &t, tks, nt, G);
 }
 
+// The build's second stream.  side_fork(k): ctx->side waits for the work
+// queued so far on the main stream.  side_mark(k): marks the work queued so
+// far on ctx->side; main_wait(k): the main stream waits for that mark.
+// Under graph capture the side stream joins the capture through the forked
+// event and every branch is joined back before the build returns.
+enum { kForkSort = 0, kMarkGather = 1, kForkTopo = 2, kMarkLevels = 3, kForkKids = 4, kMarkGroups = 5 };
+
+static void side_fork(bh_ctx *ctx, int k)
+{
+    BH_HIP(hipEventRecord(ctx->fork_ev[k], ctx->stream));
+    BH_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev[k], 0));
+}
+
+static void side_mark(bh_ctx *ctx, int k) { BH_HIP(hipEventRecord(ctx->fork_ev[k], ctx->side)); }
+
+static void main_wait(bh_ctx *ctx, int k) { BH_HIP(hipStreamWaitEvent(ctx->stream, ctx->fork_ev[k], 0)); }
+
+// build_groups on the side stream (its launches and scans use ctx->stream).
+static void build_groups_side(bh_ctx *ctx, const bh_params &p, const Topo *tree_topo, const uint64_t *ks,
+                              const uint64_t *tks)
+{
+    hipStream_t main = ctx->stream;
+    ctx->stream = ctx->side;
+    try {
+        build_groups(ctx, p, tree_topo, ks, tks);
+    } catch (...) {
+        ctx->stream = main;
+        throw;
+    }
+    ctx->stream = main;
+}
+
 // Builds the tree of the uploaded mass set and the sorted test set.  No host
 // synchronisation: every count stays on the device and capacities come from
 // the host (the caller checks stat[S_FLAGS] afterwards).
@@ -1466,7 +1498,11 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
     float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
-    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, st>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo);
+    // the gather (HBM-bound) runs beside the topology (latency-bound); only
+    // the moments and the walk read its records
+    side_fork(ctx, kForkSort);
+    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, ctx->side>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo);
+    side_mark(ctx, kMarkGather);
     T.bd = bd;
     T.bhi = bhi;
     T.blo = blo;
@@ -1518,6 +1554,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cap = 0;
     T.ncellp = stat + S_NCELL;
     if (nb <= 1) {
+        main_wait(ctx, kMarkGather);
         if (nb == 1) k_set_int<<<1, 1, 0, st>>>(pre, 0);
         if (p.mode == BH_MODE_FAST) build_groups(ctx, p, nullptr, ks, tks_for_groups);
         return;
@@ -1533,19 +1570,40 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.pre = ptr<int>(ctx->pre);
     int *csub = ensure<int>(ctx->csub, 8 * (size_t)cap);
     int *cnkids = ensure<int>(ctx->cnkids, cap);
-    k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids);
     double *mo = ensure<double>(ctx->cmo, 10 * (size_t)cap);
     float4 *cA = ensure<float4>(ctx->cA, cap), *cB = ensure<float4>(ctx->cB, cap);
     float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
     float4 *cF0 = ensure<float4>(ctx->cF0, cap), *cF1 = ensure<float4>(ctx->cF1, cap);
-    // cells bucketed by level, then the moments level by level, deepest first
+    // cells bucketed by level (side stream) beside the child slots (main)
     uint32_t *lwork = ensure<uint32_t>(ctx->lhist, 3 * (kLevels + 1));
     uint32_t *lcount = lwork, *loffd = lwork + (kLevels + 1), *lcursor = lwork + 2 * (kLevels + 1);
     int *llist = ensure<int>(ctx->llist, cap);
-    k_level_zero<<<1, 64, 0, st>>>(lcount);
-    k_level_count<<<capgrid(ctx, cap), kTpb, 0, st>>>(clevel, tt.ncellp, lcount);
-    k_level_scan<<<1, 64, 0, st>>>(lcount, loffd, lcursor);
-    k_level_place<<<capgrid(ctx, cap), kTpb, 0, st>>>(clevel, tt.ncellp, lcursor, llist);
+    side_fork(ctx, kForkTopo);
+    hipStream_t sd = ctx->side;
+    k_level_zero<<<1, 64, 0, sd>>>(lcount);
+    k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
+    k_level_scan<<<1, 64, 0, sd>>>(lcount, loffd, lcursor);
+    k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcursor, llist);
+    side_mark(ctx, kMarkLevels);
+    k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids);
+    T.first = cfirst;
+    T.last = clast;
+    T.fnb = ptr<int2>(ctx->cfnb);
+    T.level = clevel;
+    T.skip = ptr<int>(ctx->cskip);
+    T.sub = csub;
+    T.nkids = cnkids;
+    // the groups (side stream, after the level lists there) need the child
+    // slots and the topology only; the moments need the gathered bodies and
+    // the level lists
+    const bool groups = p.mode == BH_MODE_FAST;
+    if (groups) {
+        side_fork(ctx, kForkKids);
+        build_groups_side(ctx, p, &tt, ks, tks_for_groups);
+        side_mark(ctx, kMarkGroups);
+    }
+    main_wait(ctx, kMarkGather);
+    main_wait(ctx, kMarkLevels);
     const int mgrid = ctx->ncu * 4;
     const int lsys = ctx->sysbits / 3;
     // the learned features are written only for the learned walk and sampling
@@ -1566,13 +1624,6 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         k_sysroot_cells<<<nblocks(cap), kTpb, 0, st>>>(clevel, cfirst, ks, lsys, ctx->sysbits, tt.ncellp,
                                                        ptr<int>(ctx->sysroot));
     BH_HIP(hipGetLastError());
-    T.first = cfirst;
-    T.last = clast;
-    T.fnb = ptr<int2>(ctx->cfnb);
-    T.level = clevel;
-    T.skip = ptr<int>(ctx->cskip);
-    T.sub = csub;
-    T.nkids = cnkids;
     T.mo = mo;
     T.cA = cA;
     T.cB = cB;
@@ -1580,7 +1631,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cQ1 = cQ1;
     T.cF0 = cF0;
     T.cF1 = cF1;
-    if (p.mode == BH_MODE_FAST) build_groups(ctx, p, &tt, ks, tks_for_groups);
+    if (groups) main_wait(ctx, kMarkGroups);
 }
 
 }  // namespace bh
