@@ -615,40 +615,68 @@ __global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, ui
     ccnt[s] = (uint32_t)max(0, c - cp);
 }
 
-// Block minima of cpl over 64 and 4096 bodies, for the range-end search.
-__global__ void k_cmin(const int8_t *__restrict__ cpl, int nb, int8_t *m1, int n1)
+// Block minima over 64 entries of a byte array in the cq domain (cpl + 1,
+// values 0..22): level 1 over cq, levels 2 and 3 over the level below, for
+// the range-end search.
+__global__ void k_cmin(const uint8_t *__restrict__ in, int nin, uint8_t *out, int nout)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n1) return;
-    int lo = b * 64, hi = min(nb, lo + 64);
-    int m = 127;
-    for (int j = lo; j < hi; j++) m = min(m, (int)cpl[j]);
-    m1[b] = (int8_t)m;
+    if (b >= nout) return;
+    int lo = b * 64, hi = min(nin, lo + 64);
+    int m = 255;
+    for (int j = lo; j < hi; j++) m = min(m, (int)in[j]);
+    out[b] = (uint8_t)m;
 }
 
-// first j >= s with cpl[j] < L (exists: cpl[nb-1] = -1)
-// The scan reads cq = cpl + 1 eight entries at a time (8-byte aligned words;
-// cq holds nb + 8 bytes): a byte below n = L + 1 <= 22 in a word of bytes
-// below 128 is flagged by (x - n*0x01..01) & ~x & 0x80..80, exactly for the
-// lowest such byte (borrows only reach higher bytes).  Bytes before j are
-// set to 0x7f so that they never match.
-__device__ int range_end(const uint8_t *__restrict__ cq, const int8_t *__restrict__ m1,
-                         const int8_t *__restrict__ m2, const int8_t *__restrict__ m3, int s, int L)
+// Flags (the high bit of a byte) of the entries k, k+1, .. of the aligned
+// 8-entry word of a that holds entry k whose value is below n (per byte:
+// n = L + 1 <= 22).  A byte below n is flagged by (x - n*0x01..01) & ~x &
+// 0x80..80 exactly for the lowest such byte (borrows only reach higher
+// bytes; bytes of 128 and above never flag); the entries before k are set
+// to 0x7f so that they never match.
+__device__ __forceinline__ uint64_t below(const uint8_t *__restrict__ a, int k, uint64_t nl)
 {
-    const uint64_t ones = 0x0101010101010101ULL, highs = 0x8080808080808080ULL;
-    const uint64_t nl = ones * (uint64_t)(L + 1);
+    const int w = k & ~7;
+    uint64_t x = *reinterpret_cast<const uint64_t *>(a + w);
+    const int sh = (k - w) * 8;
+    if (sh) x |= 0x7f7f7f7f7f7f7f7fULL & ((1ULL << sh) - 1);
+    return (x - nl) & ~x & 0x8080808080808080ULL;
+}
+
+// First j >= s with cpl[j] < L (exists: cpl[nb-1] = -1), i.e. cq[j] < L + 1.
+// Each level of the minima pyramid (m1: 64 bodies, m2: 4096, m3: 262144 per
+// entry; all in the cq domain, 8-byte aligned, one word of padding) is read
+// eight entries at a time: at an aligned position the word either lets the
+// search skip up to eight blocks whose minimum is >= L, or names the first
+// block holding an entry below L, which the next finer level then searches.
+// At most ~8 words per level going up and down.
+__device__ int range_end(const uint8_t *__restrict__ cq, const uint8_t *__restrict__ m1,
+                         const uint8_t *__restrict__ m2, const uint8_t *__restrict__ m3, int s, int L)
+{
+    const uint64_t nl = 0x0101010101010101ULL * (uint64_t)(L + 1);
     int j = s;
     for (;;) {
-        if ((j & 262143) == 0 && m3[j >> 18] >= L) { j += 262144; continue; }
-        if ((j & 4095) == 0 && m2[j >> 12] >= L) { j += 4096; continue; }
-        if ((j & 63) == 0 && m1[j >> 6] >= L) { j += 64; continue; }
-        const int w = j & ~7;
-        uint64_t x = *reinterpret_cast<const uint64_t *>(cq + w);
-        const int sh = (j - w) * 8;
-        if (sh) x |= 0x7f7f7f7f7f7f7f7fULL & ((1ULL << sh) - 1);
-        const uint64_t t = (x - nl) & ~x & highs;
-        if (t) return w + (__builtin_ctzll(t) >> 3);
-        j = w + 8;
+        if ((j & 262143) == 0) {
+            const int k = j >> 18;
+            const uint64_t t = below(m3, k, nl);
+            if (!t) { j = ((k | 7) + 1) << 18; continue; }
+            j = ((k & ~7) + (__builtin_ctzll(t) >> 3)) << 18;
+        }
+        if ((j & 4095) == 0) {
+            const int k = j >> 12;
+            const uint64_t t = below(m2, k, nl);
+            if (!t) { j = ((k | 7) + 1) << 12; continue; }
+            j = ((k & ~7) + (__builtin_ctzll(t) >> 3)) << 12;
+        }
+        if ((j & 63) == 0) {
+            const int k = j >> 6;
+            const uint64_t t = below(m1, k, nl);
+            if (!t) { j = ((k | 7) + 1) << 6; continue; }
+            j = ((k & ~7) + (__builtin_ctzll(t) >> 3)) << 6;
+        }
+        const uint64_t t = below(cq, j, nl);
+        if (t) return (j & ~7) + (__builtin_ctzll(t) >> 3);
+        j = (j | 7) + 1;
     }
 }
 
@@ -667,7 +695,7 @@ __global__ void k_owner(const uint32_t *__restrict__ cbase, int nb, const unsign
 // cpl[j] < L.
 __global__ void k_cells(const int8_t *__restrict__ cpl, const uint8_t *__restrict__ cq, const uint32_t *__restrict__ cbase,
                         const int *__restrict__ owner,
-                        const int8_t *__restrict__ m1, const int8_t *__restrict__ m2, const int8_t *__restrict__ m3,
+                        const uint8_t *__restrict__ m1, const uint8_t *__restrict__ m2, const uint8_t *__restrict__ m3,
                         const unsigned long long *ncellp, int *cfirst, int *clast, uint8_t *clevel, int *cskip,
                         int *pre, int2 *fnb, unsigned long long *deep)
 {
@@ -1319,9 +1347,12 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     exclusive_scan_u32(ctx, ccnt, t.cbase, n);
     k_clamp_cells<<<1, 1, 0, st>>>(t.cbase, n, (unsigned long long)cap, stat, slot, slot_raw, flag);
     int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64, n3 = (n2 + 63) / 64;
-    int8_t *m1 = ensure<int8_t>(bm1, n1 + n2 + n3 + 3);
-    int8_t *m2 = m1 + n1 + 1, *m3 = m2 + n2 + 1;
-    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(t.cpl, n, m1, n1);
+    // each level 8-byte aligned with at least one word of padding (the
+    // search reads whole words; it stops at the entry holding cpl[n-1] = -1)
+    auto words = [](int v) { return (v + 15) & ~7; };
+    uint8_t *m1 = reinterpret_cast<uint8_t *>(ensure<int8_t>(bm1, words(n1) + words(n2) + words(n3)));
+    uint8_t *m2 = m1 + words(n1), *m3 = m2 + words(n2);
+    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(cq, n, m1, n1);
     k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
     k_cmin<<<nblocks(n3), kTpb, 0, st>>>(m2, n2, m3, n3);
     t.first = ensure<int>(bfirst, cap);
