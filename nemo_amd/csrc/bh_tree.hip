@@ -108,9 +108,11 @@ __device__ __forceinline__ bool fits(const double4 c, const BoxTest &t)
 __device__ __forceinline__ bool outside_massive(const double *__restrict__ pos, const double *__restrict__ mass,
                                                 int64_t i, int64_t n, const BoxTest &b, int *bad)
 {
-    if (i >= n || mass[i] == 0.0) return false;
-    const double *x = pos + 3 * i;
-    if (fits(x, b)) return false;
+    if (i >= n) return false;
+    // mass and position loads issued together (one memory round trip)
+    const double m = mass[i];
+    const double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    if (m == 0.0 || fits(x, b)) return false;
     if (!(isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]))) *bad = 1;
     return true;
 }
@@ -235,7 +237,13 @@ __global__ void k_box_check(const int *boxi, unsigned long long *stat)
 // box, or the system's previous box, take the first body in input order that
 // lies outside, double the box towards it (load.c:68-95), and go on from the
 // next body; every body of a batch is massive.  All threads compute the same
-// box.  At most `rounds` expansions; a box still open is flagged.
+// box.  A round scans forward from its start in chunks of kTpb * kSysBoxPer
+// bodies (all loads of a chunk in flight) and stops at the first chunk that
+// holds a body outside; the next round starts after that body, so all rounds
+// together read each body about once.  At most `rounds` expansions; a box
+// still open is flagged.
+constexpr int kSysBoxPer = 8;
+
 __global__ __launch_bounds__(kTpb) void k_sys_box(const double *__restrict__ pos, const int64_t *__restrict__ off,
                                                   double4 *sysbox, double4 *box0, double rx, double ry, double rz,
                                                   double rsize, int keep, int rounds, unsigned long long *stat)
@@ -247,20 +255,24 @@ __global__ __launch_bounds__(kTpb) void k_sys_box(const double *__restrict__ pos
     int start = lo;
     bool closed = false;
     for (int round = 0; round <= rounds; round++) {
-        int first = hi;
-        for (int i = start + (int)threadIdx.x; i < hi; i += blockDim.x)
-            if (!fits(pos + 3 * (int64_t)i, b)) {
-                first = i;
-                break;
+        const BoxTest bt = box_test(b);
+        int f = hi;
+        for (int c0 = start; c0 < hi && f == hi; c0 += kTpb * kSysBoxPer) {
+            int first = hi;
+#pragma unroll
+            for (int k = 0; k < kSysBoxPer; k++) {
+                const int i = c0 + k * kTpb + (int)threadIdx.x;
+                if (i < hi && !fits(pos + 3 * (int64_t)i, bt)) first = min(first, i);
             }
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) first = min(first, __shfl_xor(first, d, 64));
-        if (lane_id() == 0) red[threadIdx.x >> 6] = first;
-        __syncthreads();
-        int f = red[0];
+            for (int d = 32; d >= 1; d >>= 1) first = min(first, __shfl_xor(first, d, 64));
+            if (lane_id() == 0) red[threadIdx.x >> 6] = first;
+            __syncthreads();
+            f = red[0];
 #pragma unroll
-        for (int k = 1; k < kTpb / 64; k++) f = min(f, red[k]);
-        __syncthreads();
+            for (int k = 1; k < kTpb / 64; k++) f = min(f, red[k]);
+            __syncthreads();
+        }
         if (f == hi) {
             closed = true;
             break;
