@@ -856,6 +856,7 @@ bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo
             ctx->box_valid = true;
             const unsigned long long f = stat[bh::S_FLAGS];
             if (f & bh::F_NONFINITE) throw ApiFail(BH_ERR_NONFINITE, "non-finite position");
+            ctx->ngroups = (int)stat[bh::S_NGROUPS];   // the sampled groups (mlp_collect orders by group)
             if (!f) break;
             if (attempt > 12) throw ApiFail(BH_ERR_CAPACITY, "sample collection did not converge");
             if (f & bh::F_CELL_OVF) ctx->cell_cap = (int)std::min<uint64_t>(stat[bh::S_NCELL_RAW] * 5 / 4 + 64, INT32_MAX / 2);

@@ -645,6 +645,23 @@ __global__ void k_zero_f4(float4 *p, int n)
     if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Cells recorded per group (sample mode): every walked group has one record
+// in the region of the wave that walked it; groups without one keep 0.
+__global__ void k_group_cells(const int *__restrict__ lrec, int lcap, int nreg, int ng, uint32_t *cnt)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreg) return;
+    const int *reg = lrec + (int64_t)r * lcap;
+    for (int p = 1; p < reg[0]; p += 5 + reg[p + 1])
+        if ((unsigned)reg[p] < (unsigned)ng) cnt[reg[p]] = (uint32_t)reg[p + 1];
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Cells recorded per region (sample mode), for the sample offsets.
 __global__ void k_region_cells(const int *__restrict__ lrec, int lcap, int nreg, uint32_t *cnt)
 {
@@ -729,7 +746,10 @@ __global__ __launch_bounds__(64) void k_exact_corr(const int *__restrict__ lrec,
 // Training samples: for every recorded (group centre, cell), the features and
 // the exact residual (FP64 softened direct sum over the cell's bodies minus
 // the FP64 multipole force of the walk's formula) in units of M / r^2.
-__global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ off,
+// Group g's samples go to goff[g] onwards, so the set is in group order
+// whichever wave walked which group (training is then reproducible).
+__global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ goff,
+                                                int ng,
                                                 const float4 *__restrict__ cA, const float4 *__restrict__ cB,
                                                 const float4 *__restrict__ cF0, const float4 *__restrict__ cF1,
                                                 const int2 *__restrict__ fnb, const double *__restrict__ mo,
@@ -739,10 +759,14 @@ __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, in
     const int l = lane_id();
     const int *reg = lrec + (int64_t)blockIdx.x * lcap;
     const int used = reg[0];
-    int64_t j = off[blockIdx.x];
     const double e2 = eps * eps;
     for (int p = 1; p < used;) {
         const int n = reg[p + 1];
+        if ((unsigned)reg[p] >= (unsigned)ng) {   // not counted by k_group_cells
+            p += 5 + n;
+            continue;
+        }
+        int64_t j = goff[reg[p]];
         const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
         for (int k = 0; k < n; k++, j++) {
             const int c = reg[p + 5 + k];
@@ -886,17 +910,21 @@ int64_t mlp_collect(bh_ctx *ctx, const bh_params &p)
     const Tree &T = ctx->tree;
     hipStream_t st = ctx->stream;
     const int nreg = ctx->lnwave;
-    uint32_t *cnt = ensure<uint32_t>(M.soff, 2 * (size_t)nreg + 2);
-    uint32_t *off = cnt + nreg + 1;
-    k_region_cells<<<(nreg + 255) / 256, 256, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, nreg, cnt);
-    exclusive_scan_u32(ctx, cnt, off, nreg);
+    const int ng = ctx->ngroups;
+    if (ng <= 0) return 0;
+    // samples in group order: per-group counts, then their exclusive scan
+    uint32_t *cnt = ensure<uint32_t>(M.soff, 2 * (size_t)ng + 2);
+    uint32_t *goff = cnt + ng + 1;
+    k_fill_u32<<<(ng + 255) / 256, 256, 0, st>>>(cnt, ng, 0u);
+    k_group_cells<<<(nreg + 255) / 256, 256, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, nreg, ng, cnt);
+    exclusive_scan_u32(ctx, cnt, goff, ng);
     uint32_t total = 0;
-    BH_HIP(hipMemcpyAsync(&total, off + nreg, sizeof total, hipMemcpyDeviceToHost, st));
+    BH_HIP(hipMemcpyAsync(&total, goff + ng, sizeof total, hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
     if (total == 0) return 0;
     float *nx = ensure<float>(M.snew_x, (size_t)total * kMlpIn), *ny = ensure<float>(M.snew_y, (size_t)total * 3);
-    k_samples<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, off, T.cA, T.cB, T.cF0, T.cF1, T.fnb, T.mo, T.bd,
-                                   p.eps, p.usequad, nx, ny);
+    k_samples<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, goff, ng, T.cA, T.cB, T.cF0, T.cF1, T.fnb, T.mo,
+                                   T.bd, p.eps, p.usequad, nx, ny);
     BH_HIP(hipGetLastError());
     // append to the training set
     const int64_t old = M.nsamples, nn = old + total;

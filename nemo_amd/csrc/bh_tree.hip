@@ -440,21 +440,30 @@ __global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n, const 
 // ---- radix sort (LSD, 8-bit digits, stable) -----------------------------------
 
 constexpr int kSortItems = 8;
+static_assert(kTpb == 256, "the radix kernels use one thread per 8-bit digit");
 constexpr int kSortTile = kTpb * kSortItems;
 
-__global__ void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, uint32_t *hist, int nblk)
+// One sub-histogram per wave (skewed digits, e.g. the top byte of clustered
+// keys, would otherwise serialise the block on a few LDS counters).
+__global__ __launch_bounds__(kTpb) void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
+                                                     uint32_t *hist, int nblk)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[kTpb / 64][256];
+#pragma unroll
+    for (int w = 0; w < kTpb / 64; w++) h[w][threadIdx.x] = 0;
     __syncthreads();
+    uint32_t *hw = h[threadIdx.x >> 6];
     const int64_t base = (int64_t)blockIdx.x * kSortTile;
 #pragma unroll
     for (int r = 0; r < kSortItems; r++) {
         int64_t i = base + (int64_t)r * kTpb + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&hw[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < kTpb / 64; w++) c += h[w][threadIdx.x];
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = c;
 }
 
 // Exclusive scan of each digit's row of the digit-major block histogram
