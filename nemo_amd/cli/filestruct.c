@@ -302,3 +302,97 @@ int fs_put_item(FILE *f, const fs_item *it)
     }
     return fs_put_data(f, it->tag, it->type, it->data, it->ndim, it->dims);
 }
+
+/* ---- typed reads and conversions ------------------------------------------ */
+
+int fs_get_data(const fs_item *it, const char *type, void *dat, int ndim, const int *dims, int coerce, char *err,
+                size_t errlen)
+{
+    if (!it || !strcmp(it->type, "(")) {
+        snprintf(err, errlen, "get_data: item %s: not a data item", it ? it->tag : "?");
+        return -1;
+    }
+    const int same = !strcmp(it->type, type);
+    const int f2d = !strcmp(it->type, "f") && !strcmp(type, "d");
+    const int d2f = !strcmp(it->type, "d") && !strcmp(type, "f");
+    if (!same && !(coerce && (f2d || d2f))) {
+        snprintf(err, errlen, coerce ? "get_data_sub: item %s: types %s, %s don't convert"
+                                     : "get_data_sub: item %s: types %s, %s don't match",
+                 it->tag, it->type, type);
+        return -1;
+    }
+    if (ndim > 0 && it->ndim > 0) {
+        int ok = ndim == it->ndim;
+        for (int k = 0; ok && k < ndim; k++) ok = dims[k] == it->dims[k];
+        if (!ok) {
+            snprintf(err, errlen, "get_data_sub: item %s: dimensions don't match", it->tag);
+            return -1;
+        }
+    } else if (ndim == 0 && it->ndim > 0) {
+        snprintf(err, errlen, "get_data_sub: item %s: can't copy plural to scalar", it->tag);
+        return -1;
+    } else if (ndim > 0 && it->ndim == 0) {
+        snprintf(err, errlen, "get_data_sub: item %s: can't copy scalar to plural", it->tag);
+        return -1;
+    }
+    if (same) {
+        memcpy(dat, it->data, it->count * it->elsize);
+    } else if (f2d) {
+        const float *s = (const float *)it->data;
+        double *d = (double *)dat;
+        for (size_t i = 0; i < it->count; i++) d[i] = (double)s[i];
+    } else {
+        const double *s = (const double *)it->data;
+        float *d = (float *)dat;
+        for (size_t i = 0; i < it->count; i++) d[i] = (float)s[i];
+    }
+    return 0;
+}
+
+/* First code that applies to an item of type t (findtype). */
+static const char *find_code(const char *const *codes, int ncodes, const char *t)
+{
+    for (int i = 0; i < ncodes; i++) {
+        const char c = codes[i][0];
+        if ((!strcmp(t, "d") && c == 'd') || (!strcmp(t, "f") && c == 'f') || (!strcmp(t, "h") && c == 'h') ||
+            (!strcmp(t, "i") && (c == 'i' || c == 's')))
+            return codes[i];
+    }
+    return NULL;
+}
+
+int fs_convert(fs_item *it, const char *const *codes, int ncodes, void (*warn)(const char *code, const char *tag))
+{
+    if (!strcmp(it->type, "(")) {
+        int n = 0;
+        for (int k = 0; k < it->nkids; k++) {
+            const int c = fs_convert(&it->kids[k], codes, ncodes, warn);
+            if (c < 0) return -1;
+            n += c;
+        }
+        return n;
+    }
+    const char *cp = find_code(codes, ncodes, it->type);
+    if (!cp) return 0;
+    const int d2f = !strcmp(it->type, "d") && !strcmp(cp, "d2f");
+    const int f2d = !strcmp(it->type, "f") && !strcmp(cp, "f2d");
+    if (!d2f && !f2d) {
+        if (warn) warn(cp, it->tag);
+        return 0;
+    }
+    const size_t es = d2f ? sizeof(float) : sizeof(double);
+    void *nd = malloc(it->count * es + 1);
+    if (!nd) return -1;
+    if (d2f) {
+        const double *s = (const double *)it->data;
+        for (size_t i = 0; i < it->count; i++) ((float *)nd)[i] = (float)s[i];
+    } else {
+        const float *s = (const float *)it->data;
+        for (size_t i = 0; i < it->count; i++) ((double *)nd)[i] = (double)s[i];
+    }
+    free(it->data);
+    it->data = nd;
+    it->elsize = es;
+    snprintf(it->type, sizeof it->type, "%s", d2f ? "f" : "d");
+    return 1;
+}

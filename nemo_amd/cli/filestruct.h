@@ -75,6 +75,24 @@ int fs_put_history(FILE *f, const char *text);
 /* Re-emit one parsed item (and its children) unchanged. */
 int fs_put_item(FILE *f, const fs_item *it);
 
+/* Typed read of one item (get_data / get_data_coerced, filesecret.c:532-608):
+ * the stored type must equal `type`, or, with coerce, float and double
+ * convert into each other (copyfun, filesecret.c:1204-1213; no other pair).
+ * The shape must equal dims[0..ndim) (ndim 0: a singular item); plural to
+ * scalar and scalar to plural are errors.  Returns 0, or -1 with the reason
+ * in err. */
+int fs_get_data(const fs_item *it, const char *type, void *dat, int ndim, const int *dims, int coerce, char *err,
+                size_t errlen);
+
+/* csf's convert= (copy_item_cvt, filesecret.c:127-240): every data item of
+ * `it` (sets are recursed) whose type matches the first letter of one of the
+ * codes (findtype, filesecret.c:1454-1472: d, f, h; i for codes starting
+ * with i or s) is converted by the first such code: d2f (double to float),
+ * f2d (float to double).  Any other matching code leaves the item as it is
+ * and is reported through warn(code, tag).  Returns the number of items
+ * converted, or -1 when memory runs out. */
+int fs_convert(fs_item *it, const char *const *codes, int ncodes, void (*warn)(const char *code, const char *tag));
+
 #ifdef __cplusplus
 }
 #endif

@@ -404,3 +404,48 @@ def test_plugin_matches_bh_force():
     f(3, 0.0, 5000, vp(m), vp(x), None, None, vp(pot), vp(acc), 0, b"d")
     assert np.array_equal(pot, phi3) and np.array_equal(acc, acc3)
     assert not np.array_equal(acc3, acc0)
+
+
+def test_csf_convert_and_coerced_reads(golden, tmp_path):
+    """csf convert=d2f / f2d (copy_item_cvt) and the codec's typed reads:
+    get_data is rigid (a float PhaseSpace is refused, as hackforce refuses
+    it), get_data_coerced converts float to double; shapes are checked."""
+    src = os.path.join(golden, "plum10")
+    f, d = str(tmp_path / "f"), str(tmp_path / "d")
+    assert run("csf", "in=" + src, "out=" + f, "convert=d2f").returncode == 0
+    r = run("csf", "in=" + f, "out=" + d, "convert=f2d,i2f")
+    assert r.returncode == 0
+    assert b"### Warning [csf]: Cannot convert i2f yet in Nobj" in r.stderr
+    s0 = snapshot.read(src)[0]
+
+    def particles(path):
+        for it in snapshot.parse(open(path, "rb").read()):
+            if it.tag == "SnapShot":
+                return it.find("Particles")
+
+    pf, pd = particles(f), particles(d)
+    assert pf.find("PhaseSpace").type == "f" and pf.find("Mass").type == "f"
+    assert pd.find("PhaseSpace").type == "d" and pd.find("CoordSystem").type == "i"
+    x32 = np.asarray(pf.find("PhaseSpace").data, np.float32).reshape(-1, 2, 3)
+    assert np.array_equal(x32[:, 0], s0.pos.astype(np.float32)) and np.array_equal(x32[:, 1], s0.vel.astype(np.float32))
+    x64 = np.asarray(pd.find("PhaseSpace").data, np.float64).reshape(-1, 2, 3)
+    assert np.array_equal(x64, x32.astype(np.float64))
+    # float snapshots are refused by the rigid reader of hackforce
+    r = run("hackforce", "in=" + f, "out=" + str(tmp_path / "o"))
+    assert r.returncode == 255 and b"### Fatal error [hackforce]" in r.stderr and b"has type f, d expected" in r.stderr
+    # typed reads through the C codec
+    exe = str(tmp_path / "codec_check")
+    cli = os.path.join(ROOT, "nemo_amd", "cli")
+    subprocess.run(["gcc", "-O1", "-std=gnu11", "-I", cli, "-o", exe, os.path.join(ROOT, "tests", "helpers", "codec_check.c"),
+                    os.path.join(cli, "filestruct.c")], check=True)
+    out = subprocess.run([exe, f], capture_output=True, text=True, check=True).stdout.splitlines()
+    res = {ln.split()[0]: ln for ln in out}
+    assert res["rigid"].startswith("rigid -1") and "types f, d don't match" in res["rigid"]
+    assert res["coerced"] == "coerced 0 ok" and res["mass"] == "mass 0 ok"
+    vals = np.array([float(v) for v in res["phase"].split()[1:]]).reshape(-1, 2, 3)
+    assert np.array_equal(vals, x32.astype(np.float64))
+    assert "dimensions don't match" in res["baddims"]
+    assert "can't copy plural to scalar" in res["scalar"]
+    assert "types f, i don't convert" in res["toint"]
+    out = subprocess.run([exe, d], capture_output=True, text=True, check=True).stdout.splitlines()
+    assert "rigid 0 ok" in out
