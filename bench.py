@@ -266,6 +266,49 @@ def run_ensemble(args, E, log):
     }
 
 
+def run_plummer1m(args, E, log):
+    """Plummer N = 2^20, eps 0.01, monopole cells, tol in {0.5, 0.7, 1.0}:
+    one force evaluation per tol (group and leaf size from the autotuner at
+    that tol), its median and 99th-percentile |da|/|a| against the FP64
+    direct sum on a seeded 8192-body subset, interactions/s and the walk's
+    fraction of the FP32 roofline."""
+    n = 1 << 20
+    m, x, _ = E.mkplummer(n, seed=args.seed, with_vel=False)
+    eng = E.Engine(0)
+    eng.upload(m, x)
+    idx = np.sort(np.random.default_rng(args.seed).permutation(n)[: args.nsub])
+    _, dacc = eng.direct(0.01, idx)
+    rows = []
+    for tol in (0.5, 0.7, 1.0):
+        best, _ = eng.autotune(E.default_params(tol=tol, eps=0.01, usequad=0), reps=2)
+        p = E.default_params(tol=tol, eps=0.01, usequad=0, group_size=best.group_size, nleaf=best.nleaf)
+        med, p99 = measure_err(eng, E, p, idx, dacc)
+        sec, stats = time_steps(eng, p, args.steps, args.warmup)
+        st = stats[-1]
+        n_int = int(st.n2btot + st.nbctot)
+        t_walk = float(np.mean([s.t_walk_kernel_ms for s in stats])) * 1e-3
+        rows.append({"tol": tol, "group_size": best.group_size, "nleaf": best.nleaf, "median_rel_err": med,
+                     "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
+                     "interactions_per_step": n_int, "interactions_per_s": n_int / sec,
+                     "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])), "t_walk_kernel_ms": t_walk * 1e3,
+                     "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,false,false,false,false>" %
+                                               (best.group_size // 64))})
+        log("plummer 2^20 tol %.1f: group %d nleaf %d median %.3g p99 %.3g  %.3f ms/step  %.3g interactions/s" % (
+            tol, best.group_size, best.nleaf, med, p99, sec * 1e3, n_int / sec))
+    eng.close()
+    head = rows[1]
+    return {
+        "metric": "interactions/s, Plummer N=2^20, eps 0.01, tol 0.7 (0.5 and 1.0 in config)",
+        "value": head["interactions_per_s"], "unit": "interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic Plummer (mkplummer recipe, own counter RNG, seed %d)" % args.seed,
+        "config": {"workload": "plummer_n1048576_tol_0.5_0.7_1.0", "n": n, "eps": 0.01, "usequad": 0,
+                   "err_subset": len(idx), "per_tol": rows},
+        "roofline": head["roofline"],
+    }
+
+
 def run_diskhalo(args, E, log):
     """Exponential disk (N = 2^21, M = 0.1, scale length 1, sech^2 height
     0.1, Q = 1.5) in a Hernquist halo (N = 2^22, M = 1, a = 4), eps = 0.02:
@@ -310,9 +353,9 @@ def run_diskhalo(args, E, log):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="plummer4m", choices=["plummer4m", "ensemble", "diskhalo"],
-                    help="plummer4m: the headline metric (default); ensemble / diskhalo: the batch and "
-                         "time-integration workloads")
+    ap.add_argument("--workload", default="plummer4m", choices=["plummer4m", "plummer1m", "ensemble", "diskhalo"],
+                    help="plummer4m: the headline metric (default); plummer1m: interactions/s and roofline at "
+                         "N=2^20 for tol 0.5/0.7/1.0; ensemble / diskhalo: the batch and time-integration workloads")
     ap.add_argument("--seed", type=int, default=123)
     ap.add_argument("--nsys", type=int, default=512)
     ap.add_argument("--lf-steps", type=int, default=0, help="leapfrog steps (default 32 ensemble, 256 disk+halo)")
@@ -350,6 +393,9 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.workload == "plummer1m":
+        print(json.dumps(run_plummer1m(args, E, log)))
+        return
     if args.workload != "plummer4m":
         if args.lf_steps <= 0:
             args.lf_steps = 32 if args.workload == "ensemble" else 256
