@@ -68,6 +68,22 @@ def test_cells_match_oracle_tree(eng, plum10k):
     assert np.array_equal(c["quad6"], q9[:, [0, 1, 2, 4, 5, 8]])
 
 
+def test_cells_match_oracle_tree_large(eng):
+    """600,000 bodies: cell ranges past the coarsest block of the range
+    search (262,144 bodies per minima entry), levels, body counts and FP64
+    moments equal the oracle's tree."""
+    m, x, _ = E.mkplummer(600000, seed=5, with_vel=False)
+    eng.upload(m, x)
+    st = eng.eval(E.default_params(tol=0.7, eps=0.01, usequad=1, mode="strict"))
+    assert st.deep_buckets == 0
+    c = eng.debug_cells()
+    t = oracle.tree_cells(m, x, np.full(3, -2.0), 4.0, usequad=True)
+    assert len(c["level"]) > 200000
+    assert np.array_equal(c["level"], t["level"]) and np.array_equal(c["nbody"], t["nbody"])
+    assert np.array_equal(c["mass"], t["mass"]) and np.array_equal(c["pos"], t["pos"])
+    assert np.array_equal(c["quad6"], t["quad"][:, [0, 1, 2, 4, 5, 8]])
+
+
 def test_hack128_fixture_strict(eng, golden):
     """The reference's own t=0 frame: tree forces at tol=1 (box -2..2) give the
     stored FP64 potential energy and PE tensor exactly."""
