@@ -111,6 +111,9 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     if (ctx->cell_cap == 0) ctx->cell_cap = (int)std::min<int64_t>(2 * ctx->nb + 64, INT32_MAX / 2);
     unsigned long long stat[bh::S_NSLOTS];
     double box[4];
+    // plain fast walk: the tree-end event also starts the walk kernel and the
+    // walk kernel's end event also ends the phase (fewer event records)
+    const bool plain_fast = q.mode == BH_MODE_FAST && !q.learned;
     for (int attempt = 0;; attempt++) {
         BH_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         bh::build_tree(ctx, q);
@@ -125,7 +128,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         } else {
             bh::walk_fast(ctx, q);
         }
-        BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+        if (!plain_fast) BH_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
         BH_HIP(hipMemcpyAsync(stat, ctx->counters.p, sizeof stat, hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipMemcpyAsync(box, ctx->dbox.p, sizeof box, hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
@@ -166,9 +169,10 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     s.rsize = box[3];
     float t1 = 0, t2 = 0, tw = 0, ts = 0;
     BH_HIP(hipEventElapsedTime(&t1, ctx->ev[0], ctx->ev[1]));
-    BH_HIP(hipEventElapsedTime(&t2, ctx->ev[1], ctx->ev[2]));
+    BH_HIP(hipEventElapsedTime(&t2, ctx->ev[1], plain_fast ? ctx->ev[5] : ctx->ev[2]));
     BH_HIP(hipEventElapsedTime(&ts, ctx->ev[6], ctx->ev[7]));
-    if (q.mode == BH_MODE_FAST) BH_HIP(hipEventElapsedTime(&tw, ctx->ev[4], ctx->ev[5]));
+    if (plain_fast) tw = t2;
+    else if (q.mode == BH_MODE_FAST) BH_HIP(hipEventElapsedTime(&tw, ctx->ev[4], ctx->ev[5]));
     else tw = t2;
     s.t_tree_ms = t1;
     s.t_walk_ms = t2;

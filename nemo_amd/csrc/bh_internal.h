@@ -75,8 +75,9 @@ enum StatSlot {
     S_NGROUPS = 6,
     S_FLAGS = 7,
     S_TNCELL = 8, S_TNCELL_RAW = 9,   // topology of a separate test set (grouping only)
-    S_ZERO = 10,        // slots below are cleared at the start of every evaluation
-    S_STICKY = 10,      // flags or-ed over the steps of a leapfrog batch
+    S_GCTR = 10,        // the fast walk's group counter (low 32 bits; one walk per tree build)
+    S_ZERO = 11,        // slots below are cleared at the start of every evaluation
+    S_STICKY = 11,      // flags or-ed over the steps of a leapfrog batch
     S_NSLOTS = 16
 };
 enum StatFlag : unsigned long long {

@@ -213,8 +213,13 @@ __global__ __launch_bounds__(kTpb) void k_box_round(const double *__restrict__ p
 // Starting box: the caller's (rmin, rsize), or the previous box when keep.
 // boxi: [0] first outside candidate (min), [1] start index, [2] expansions,
 // [3] round ticket, [4] candidate count, [5] done.
-__global__ void k_box_init(double4 *box, int *boxi, double rx, double ry, double rz, double rsize, int keep)
+// Start of a build: clear the per-evaluation status words and, when box is
+// given, set the start box and the expansion bookkeeping (one launch).
+__global__ void k_build_start(unsigned long long *stat, int nzero, double4 *box, int *boxi, double rx, double ry,
+                              double rz, double rsize, int keep)
 {
+    if ((int)threadIdx.x < nzero) stat[threadIdx.x] = 0;
+    if (threadIdx.x != 0 || !box) return;
     if (!keep) *box = make_double4(rx, ry, rz, rsize);
     boxi[0] = INT32_MAX;
     boxi[1] = 0;
@@ -1503,7 +1508,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *d_boxi = ensure<int>(ctx->dboxi, 8);
     unsigned long long *stat = ensure<unsigned long long>(ctx->counters, S_NSLOTS);
     const bool timing = !ctx->capturing;
-    k_zero_u64<<<1, 64, 0, st>>>(stat, S_ZERO);
+    static_assert(S_ZERO <= 64, "k_build_start clears the status words with one 64-thread block");
 
     // -- box: start from the caller's box (or the previous one) and expand in
     //    input order; a fixed number of rounds, checked afterwards
@@ -1515,10 +1520,11 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int64_t *sysoff = nsys ? ptr<int64_t>(ctx->boff) : nullptr;
     double4 *sysbox = nsys ? ensure<double4>(ctx->sysbox, nsys) : nullptr;
     if (nsys) {
+        k_build_start<<<1, 64, 0, st>>>(stat, S_ZERO, nullptr, nullptr, 0.0, 0.0, 0.0, 0.0, 0);
         k_sys_box<<<nsys, kTpb, 0, st>>>(d_pos, sysoff, sysbox, d_box, r0, r1, r2, p.rsize, keep ? 1 : 0,
                                          16 * ctx->box_rounds, stat);
     } else {
-    k_box_init<<<1, 1, 0, st>>>(d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
+    k_build_start<<<1, 64, 0, st>>>(stat, S_ZERO, d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
     double4 *cand = ensure<double4>(ctx->bcand, n);
     k_box_compact<<<(int)((n + kBoxChunk - 1) / kBoxChunk), kTpb, 0, st>>>(d_pos, d_mass, n, d_box, cand, d_boxi,
                                                                             stat);

@@ -660,8 +660,9 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     if (ctx->fcap == 0) ctx->fcap = 1 << 15;
     a.fcap = (int)ctx->fcap;
     a.frontier = ensure<int>(ctx->frontier, (size_t)grid * kWaves * 2 * ctx->fcap);
-    a.gctr = ensure<int>(ctx->gctr, 1);
-    k_zero_i32<<<1, 1, 0, ctx->stream>>>(a.gctr);
+    // the group counter is a status word cleared at the start of the tree
+    // build (every tree build is followed by exactly one fast walk)
+    a.gctr = reinterpret_cast<int *>(ptr<unsigned long long>(ctx->counters) + S_GCTR);
     const int nwave = grid * kWaves;
     if (lmode) {
         if (ctx->lcap == 0) ctx->lcap = 1 << 16;
@@ -670,11 +671,13 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
         k_clear_regions<<<nwave, 64, 0, ctx->stream>>>(a.lrec, a.lcap);
     } else {
         a.lcap = 0;
-        a.lrec = ptr<int>(ctx->gctr);
+        a.lrec = a.gctr;   // not written without records
     }
     ctx->lnwave = nwave;
     const bool soft0 = ctx->nsys ? ctx->sys_eps0 : p.eps == 0.0;
-    if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
+    // walk-kernel start: without records nothing follows the caller's
+    // tree-end event, which then stands for it (an event record costs a few us)
+    if (!ctx->capturing && lmode) BH_HIP(hipEventRecord(ctx->ev[4], ctx->stream));
     const bool crit = p.criterion != BH_CRIT_HACK;
     a.box = T.dbox;
     if (npl == 3) {
