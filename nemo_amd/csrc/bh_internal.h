@@ -75,10 +75,13 @@ enum StatSlot {
     S_NGROUPS = 6,
     S_FLAGS = 7,
     S_TNCELL = 8, S_TNCELL_RAW = 9,   // topology of a separate test set (grouping only)
-    S_GCTR = 10,        // the fast walk's group counter (low 32 bits; one walk per tree build)
-    S_ZERO = 11,        // slots below are cleared at the start of every evaluation
-    S_STICKY = 11,      // flags or-ed over the steps of a leapfrog batch
-    S_NSLOTS = 16
+    S_ZERO = 10,        // slots below are cleared at the start of every evaluation
+    S_STICKY = 10,      // flags or-ed over the steps of a leapfrog batch
+    // the fast walk's group counter (low 32 bits; cleared at the start of
+    // every tree build, one walk per build), on a cache line of its own: its
+    // atomics would otherwise serialise the walk's reads of S_NGROUPS
+    S_GCTR = 32,
+    S_NSLOTS = 48
 };
 enum StatFlag : unsigned long long {
     F_CELL_OVF = 1, F_BOX_OPEN = 2, F_NONFINITE = 4, F_FRONTIER = 8, F_TCELL_OVF = 16, F_LREC_OVF = 32, F_ECAP_OVF = 64

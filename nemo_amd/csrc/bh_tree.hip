@@ -215,10 +215,12 @@ __global__ __launch_bounds__(kTpb) void k_box_round(const double *__restrict__ p
 // [3] round ticket, [4] candidate count, [5] done.
 // Start of a build: clear the per-evaluation status words and, when box is
 // given, set the start box and the expansion bookkeeping (one launch).
-__global__ void k_build_start(unsigned long long *stat, int nzero, double4 *box, int *boxi, double rx, double ry,
-                              double rz, double rsize, int keep)
+__global__ void k_build_start(unsigned long long *stat, int nzero, uint32_t *lwork, int nlwork, double4 *box,
+                              int *boxi, double rx, double ry, double rz, double rsize, int keep)
 {
     if ((int)threadIdx.x < nzero) stat[threadIdx.x] = 0;
+    if (threadIdx.x == 0) stat[S_GCTR] = 0;
+    if ((int)threadIdx.x < nlwork) lwork[threadIdx.x] = 0;
     if (threadIdx.x != 0 || !box) return;
     if (!keep) *box = make_double4(rx, ry, rz, rsize);
     boxi[0] = INT32_MAX;
@@ -227,13 +229,6 @@ __global__ void k_build_start(unsigned long long *stat, int nzero, double4 *box,
     boxi[3] = 0;
     boxi[4] = 0;
     boxi[5] = 0;
-}
-
-// After the expansion rounds: flag a box that may still be open (no round
-// found every candidate inside).
-__global__ void k_box_check(const int *boxi, unsigned long long *stat)
-{
-    if (!boxi[5]) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
 }
 
 // ---- batched ensembles: one box per system -----------------------------------
@@ -357,9 +352,14 @@ __global__ void k_keys_batch(const double *__restrict__ pos, int64_t n, const in
 // Mass set: massless bodies get bit 63 so they sort after every massive body.
 // pk (optional): the body packed as (x, y, z, m) in input order, so that the
 // gather into sorted order reads one aligned 32-byte record per body.
+// boxi (mass-set keys after the expansion rounds, else nullptr): the rounds'
+// done flag; a box that no round found closed (every candidate inside) is
+// flagged, and the host grows the number of rounds
 __global__ void k_keys(const double *__restrict__ pos, const double *__restrict__ mass, int64_t n,
-                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals, double4 *pk)
+                       const double4 *__restrict__ box, uint64_t *keys, uint32_t *vals, double4 *pk,
+                       const int *__restrict__ boxi = nullptr, unsigned long long *stat = nullptr)
 {
+    if (boxi && blockIdx.x == 0 && threadIdx.x == 0 && !boxi[5]) atomicOr(&stat[S_FLAGS], (unsigned long long)F_BOX_OPEN);
     const double4 b = *box;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double *x = pos + 3 * i;
@@ -765,12 +765,6 @@ __global__ void k_clamp_cells(const uint32_t *__restrict__ cbase, int n, unsigne
     stat[slot_raw] = raw;
     stat[slot] = raw < cap ? raw : cap;
     if (raw > cap) atomicOr(&stat[S_FLAGS], flag);
-}
-
-__global__ void k_zero_u64(unsigned long long *p, int n)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -1198,11 +1192,6 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
 // is arbitrary; each cell's moments depend only on its children.
 constexpr int kLevels = kMaxLevel + 1;
 
-__global__ void k_level_zero(uint32_t *lcount)
-{
-    if (threadIdx.x < kLevels) lcount[threadIdx.x] = 0;
-}
-
 // Lanes of the wave holding the same level (levels < 32): a 5-ballot match.
 __device__ __forceinline__ uint64_t level_peers(int L, bool ok)
 {
@@ -1235,27 +1224,27 @@ __global__ __launch_bounds__(kTpb) void k_level_count(const uint8_t *__restrict_
     if (threadIdx.x < kLevels && h[threadIdx.x]) atomicAdd(&lcount[threadIdx.x], h[threadIdx.x]);
 }
 
-// Offsets of the levels (loff[0..kLevels]) and the running cursors.
-__global__ void k_level_scan(const uint32_t *__restrict__ lcount, uint32_t *loff, uint32_t *lcursor)
-{
-    if (threadIdx.x != 0) return;
-    uint32_t acc = 0;
-    for (int L = 0; L < kLevels; L++) {
-        loff[L] = acc;
-        lcursor[L] = acc;
-        acc += lcount[L];
-    }
-    loff[kLevels] = acc;
-}
-
-// Each block counts its cells per level, reserves ranges in the level lists,
-// then places its cells (same grid-stride order in both passes).
+// Each block computes the level offsets (the scan of the 22 counts; block 0
+// also stores them in loff[0..kLevels] for the moments), counts its cells per
+// level, reserves ranges in the level lists (cursors start at 0), then places
+// its cells (same grid-stride order in both passes).
 __global__ __launch_bounds__(kTpb) void k_level_place(const uint8_t *__restrict__ clevel,
-                                                      const unsigned long long *ncellp, uint32_t *lcursor, int *list)
+                                                      const unsigned long long *ncellp,
+                                                      const uint32_t *__restrict__ lcount, uint32_t *loff,
+                                                      uint32_t *lcursor, int *list)
 {
-    __shared__ uint32_t h[kLevels], base[kLevels];
+    __shared__ uint32_t h[kLevels], base[kLevels], off[kLevels + 1];
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int L = 0; L < kLevels; L++) {
+            off[L] = acc;
+            acc += lcount[L];
+        }
+        off[kLevels] = acc;
+    }
     if (threadIdx.x < kLevels) h[threadIdx.x] = 0;
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x <= kLevels) loff[threadIdx.x] = off[threadIdx.x];
     const int ncell = (int)*ncellp;
     const int l = lane_id();
     const uint64_t lt = l ? (~0ULL >> (64 - l)) : 0ULL;
@@ -1268,7 +1257,7 @@ __global__ __launch_bounds__(kTpb) void k_level_place(const uint8_t *__restrict_
     }
     __syncthreads();
     if (threadIdx.x < kLevels) {
-        base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&lcursor[threadIdx.x], h[threadIdx.x]) : 0u;
+        base[threadIdx.x] = off[threadIdx.x] + (h[threadIdx.x] ? atomicAdd(&lcursor[threadIdx.x], h[threadIdx.x]) : 0u);
         h[threadIdx.x] = 0;
     }
     __syncthreads();
@@ -1508,7 +1497,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *d_boxi = ensure<int>(ctx->dboxi, 8);
     unsigned long long *stat = ensure<unsigned long long>(ctx->counters, S_NSLOTS);
     const bool timing = !ctx->capturing;
-    static_assert(S_ZERO <= 64, "k_build_start clears the status words with one 64-thread block");
+    static_assert(S_ZERO <= 128, "k_build_start clears the status words with one 128-thread block");
 
     // -- box: start from the caller's box (or the previous one) and expand in
     //    input order; a fixed number of rounds, checked afterwards
@@ -1519,18 +1508,22 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int nsys = ctx->nsys;
     const int64_t *sysoff = nsys ? ptr<int64_t>(ctx->boff) : nullptr;
     double4 *sysbox = nsys ? ensure<double4>(ctx->sysbox, nsys) : nullptr;
+    // level-list counters and cursors (k_level_count / k_level_place)
+    uint32_t *lwork = ensure<uint32_t>(ctx->lhist, 3 * (kLevels + 1));
+    static_assert(3 * (kLevels + 1) <= 128, "k_build_start clears the level counters with one 128-thread block");
     if (nsys) {
-        k_build_start<<<1, 64, 0, st>>>(stat, S_ZERO, nullptr, nullptr, 0.0, 0.0, 0.0, 0.0, 0);
+        k_build_start<<<1, 128, 0, st>>>(stat, S_ZERO, lwork, 3 * (kLevels + 1), nullptr, nullptr, 0.0, 0.0, 0.0, 0.0,
+                                        0);
         k_sys_box<<<nsys, kTpb, 0, st>>>(d_pos, sysoff, sysbox, d_box, r0, r1, r2, p.rsize, keep ? 1 : 0,
                                          16 * ctx->box_rounds, stat);
     } else {
-    k_build_start<<<1, 64, 0, st>>>(stat, S_ZERO, d_box, d_boxi, r0, r1, r2, p.rsize, keep ? 1 : 0);
+    k_build_start<<<1, 128, 0, st>>>(stat, S_ZERO, lwork, 3 * (kLevels + 1), d_box, d_boxi, r0, r1, r2, p.rsize,
+                                    keep ? 1 : 0);
     double4 *cand = ensure<double4>(ctx->bcand, n);
     k_box_compact<<<(int)((n + kBoxChunk - 1) / kBoxChunk), kTpb, 0, st>>>(d_pos, d_mass, n, d_box, cand, d_boxi,
                                                                             stat);
     const int rounds = std::max(1, keep ? ctx->box_rounds_keep : ctx->box_rounds);
     for (int r = 0; r <= rounds; r++) k_box_round<<<ctx->ncu, kTpb, 0, st>>>(d_pos, cand, d_box, d_boxi);
-    k_box_check<<<1, 1, 0, st>>>(d_boxi, stat);
     }
 
     // -- keys and sort of the mass set (massless bodies sort last)
@@ -1543,7 +1536,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         k_sysroot_init<<<nblocks(nsys), kTpb, 0, st>>>(sysoff, nsys, ensure<int>(ctx->sysroot, nsys));
     } else {
         pk = ensure<double4>(ctx->bpk, n);
-        k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, pk);
+        k_keys<<<grid_n, kTpb, 0, st>>>(d_pos, d_mass, n, d_box, k0, v0, pk, d_boxi, stat);
     }
     uint64_t *ks;
     uint32_t *vs;
@@ -1633,15 +1626,12 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     float4 *cQ0 = ensure<float4>(ctx->cQ0, cap), *cQ1 = ensure<float4>(ctx->cQ1, cap);
     float4 *cF0 = ensure<float4>(ctx->cF0, cap), *cF1 = ensure<float4>(ctx->cF1, cap);
     // cells bucketed by level (side stream) beside the child slots (main)
-    uint32_t *lwork = ensure<uint32_t>(ctx->lhist, 3 * (kLevels + 1));
     uint32_t *lcount = lwork, *loffd = lwork + (kLevels + 1), *lcursor = lwork + 2 * (kLevels + 1);
     int *llist = ensure<int>(ctx->llist, cap);
     side_fork(ctx, kForkTopo);
     hipStream_t sd = ctx->side;
-    k_level_zero<<<1, 64, 0, sd>>>(lcount);
     k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
-    k_level_scan<<<1, 64, 0, sd>>>(lcount, loffd, lcursor);
-    k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcursor, llist);
+    k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount, loffd, lcursor, llist);
     side_mark(ctx, kMarkLevels);
     k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids);
     T.first = cfirst;

@@ -588,8 +588,6 @@ __global__ void k_fill_zero(double *phi, int64_t n, double *acc)
     if (i < 3 * n) acc[i] = 0.0;
 }
 
-__global__ void k_zero_i32(int *p) { *p = 0; }
-
 __global__ void k_clear_regions(int *lrec, int cap)
 {
     if (threadIdx.x == 0) lrec[(int64_t)blockIdx.x * cap] = 1;
