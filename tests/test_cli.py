@@ -449,3 +449,30 @@ def test_csf_convert_and_coerced_reads(golden, tmp_path):
     assert "types f, i don't convert" in res["toint"]
     out = subprocess.run([exe, d], capture_output=True, text=True, check=True).stdout.splitlines()
     assert "rigid 0 ok" in out
+
+
+def test_csf_item_select_headline_blocksize(tmp_path):
+    """csf's other keywords (csf.c:27-34): item= keeps the named top-level
+    items, select= the given positions among them, headline= writes a
+    Headline item first, blocksize= copies raw bytes."""
+    rng = np.random.default_rng(4)
+    snaps = [(np.full(20, 0.05), rng.normal(size=(20, 3)), rng.normal(size=(20, 3)), float(t)) for t in range(3)]
+    src = str(tmp_path / "s")
+    write_stream(src, snaps)
+    out = str(tmp_path / "o2")
+    r = run("csf", "in=" + src, "out=" + out, "item=SnapShot", "select=2", "headline=picked")
+    assert r.returncode == 0 and b"On top-level: 2 items  read, 1 items written" in r.stderr
+    items = snapshot.parse(open(out, "rb").read())
+    assert [it.tag for it in items] == ["Headline", "SnapShot"]
+    assert bytes(items[0].data).rstrip(b"\x00") == b"picked"
+    s = list(snapshot.snapshots(items))[0]
+    assert s.time == 1.0 and np.array_equal(s.pos, snaps[1][1]) and np.array_equal(s.vel, snaps[1][2])
+    r = run("csf", "in=" + src, "out=" + str(tmp_path / "h"), "item=History")
+    assert r.returncode == 0 and [it.tag for it in snapshot.parse(open(tmp_path / "h", "rb").read())] == ["History"]
+    r = run("csf", "in=" + src, "out=" + str(tmp_path / "all"), "select=1,3", "item=SnapShot")
+    assert [s.time for s in snapshot.read(str(tmp_path / "all"))] == [0.0, 2.0]
+    for bad in ("select=0", "select=3,2"):
+        r = run("csf", "in=" + src, "out=" + str(tmp_path / ("b" + bad[-1])), bad)
+        assert r.returncode == 255 and b"### Fatal error [csf]" in r.stderr
+    r = run("csf", "in=" + src, "out=" + str(tmp_path / "raw"), "blocksize=7")
+    assert r.returncode == 0 and open(tmp_path / "raw", "rb").read() == open(src, "rb").read()
