@@ -14,13 +14,14 @@
  * parameters, tnow, tout, minor_tout, nstep, rmin, rsize, nbody, bodytab)
  * with bodytab as the reference's 96-byte body records (type, mass, pos,
  * vel, acc, phi on an LP64 host), so a saved run resumes with continue= or
- * restart=.  Without in=, restart= and continue=, the initial system is a
- * Plummer sphere of nbody bodies from seed (the reference draws its own
- * test model with its generator; cencon= is accepted).  Keywords added:
+ * restart=.  Without in=, restart= and continue=, the initial system is the
+ * reference's test model (uniform ball, or M(r) = r with cencon=) drawn
+ * from this program's own generator seeded by seed=.  Keywords added:
  * usequad= (default: true when invoked as hackcode1_qp), mode=fast|strict,
  * group=, nleaf=, criterion=hack|rcrit|bh86|sw94 (treecode1's opening tests),
  * device=.  fcells= is accepted and ignored.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -43,7 +44,7 @@ static gp_keyword keys[] = {
     {"save", "", "output state as code runs", NULL},
     {"nbody", "128", "number of particles to generate", NULL},
     {"seed", "123", "random number generator seed", NULL},
-    {"cencon", "false", "centrally concentrated system (accepted; a Plummer sphere is generated)", NULL},
+    {"cencon", "false", "centrally concentrated system (M(r) = r) instead of a uniform ball", NULL},
     {"freq", "32.0", "fundamental integration frequency", NULL},
     {"eps", "0.05", "usual potential softening", NULL},
     {"tol", "1.0", "cell subdivision tolerence", NULL},
@@ -119,12 +120,70 @@ static void inputdata(const char *file)
     fs_free_snapshot(&s);
 }
 
+/* Uniform deviates in [0, 1) from a counter-based generator (splitmix64 of
+ * seed and a running counter): the test model's own RNG. */
+static uint64_t td_state;
+
+static double td_uniform(void)
+{
+    uint64_t z = (td_state += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+/* A point uniform in the ball of radius r (rejection from the cube). */
+static void td_ball(double *x, double r)
+{
+    double s;
+    do {
+        for (int k = 0; k < 3; k++) x[k] = 2.0 * td_uniform() - 1.0;
+        s = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    } while (s > 1.0 || s == 0.0);
+    for (int k = 0; k < 3; k++) x[k] *= r;
+}
+
+/* A point on the sphere of radius r (uniform direction). */
+static void td_shell(double *x, double r)
+{
+    td_ball(x, 1.0);
+    const double n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (int k = 0; k < 3; k++) x[k] *= r / n;
+}
+
+/* The reference's test model (make_testdata, code.c:129-153, and pickvec,
+ * util.c:15-27): equal masses 1/nbody; positions uniform in the unit ball,
+ * or with cencon on shells of radius uniform in [0, 1) (M(r) = r);
+ * velocities uniform in the unit ball; centre-of-mass position and velocity
+ * removed.  Drawn from this program's own generator seeded by seed=. */
 static void make_testdata(void)
 {
     headline = dupstr("Hack code: test data");
     tnow = 0.0;
     alloc_bodies(nbody);
-    bh_mkplummer(nbody, (uint64_t)gp_iparam("seed"), mass, pos, vel);
+    td_state = (uint64_t)gp_iparam("seed") * 0xd1342543de82ef95ULL + 1;
+    const int cencon = gp_bparam("cencon", 0);
+    double cmr[3] = {0, 0, 0}, cmv[3] = {0, 0, 0};
+    for (int i = 0; i < nbody; i++) {
+        mass[i] = 1.0 / nbody;
+        if (cencon) td_shell(pos + 3 * i, td_uniform());
+        else td_ball(pos + 3 * i, 1.0);
+        td_ball(vel + 3 * i, 1.0);
+        for (int k = 0; k < 3; k++) {
+            cmr[k] += pos[3 * i + k];
+            cmv[k] += vel[3 * i + k];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        cmr[k] /= nbody;
+        cmv[k] /= nbody;
+    }
+    for (int i = 0; i < nbody; i++)
+        for (int k = 0; k < 3; k++) {
+            pos[3 * i + k] -= cmr[k];
+            vel[3 * i + k] -= cmv[k];
+        }
 }
 
 /* ---- state files (savestate, restorestate) ---- */

@@ -476,3 +476,29 @@ def test_csf_item_select_headline_blocksize(tmp_path):
         assert r.returncode == 255 and b"### Fatal error [csf]" in r.stderr
     r = run("csf", "in=" + src, "out=" + str(tmp_path / "raw"), "blocksize=7")
     assert r.returncode == 0 and open(tmp_path / "raw", "rb").read() == open(src, "rb").read()
+
+
+@gpu
+@pytest.mark.parametrize("cencon", ["false", "true"])
+def test_hackcode1_test_model(tmp_path, cencon):
+    """Without in=, hackcode1 draws the reference's test model (make_testdata,
+    code.c:129-153): equal masses 1/nbody, positions uniform in the unit ball
+    (M(r) = r^3) or with cencon on shells of radius uniform in [0, 1)
+    (M(r) = r), velocities uniform in the unit ball, centre of mass at rest
+    at the origin; the same seed gives the same model."""
+    n = 4000
+    outs = []
+    for k in range(2):
+        o = "m%d" % k
+        r = run("hackcode1", "out=" + o, "nbody=%d" % n, "seed=7", "cencon=" + cencon, "tstop=0", cwd=str(tmp_path))
+        assert r.returncode == 0, r.stderr
+        outs.append(snapshot.read(str(tmp_path / o))[0])
+    s = outs[0]
+    assert np.array_equal(s.pos, outs[1].pos) and np.array_equal(s.vel, outs[1].vel)
+    assert np.all(s.mass == 1.0 / n)
+    assert np.abs(s.pos.mean(0)).max() < 1e-12 and np.abs(s.vel.mean(0)).max() < 1e-12
+    r = np.linalg.norm(s.pos, axis=1)
+    v = np.linalg.norm(s.vel, axis=1)
+    assert r.max() < 1.1 and v.max() < 1.1
+    frac = (r < 0.5).mean()
+    assert abs(frac - (0.5 if cencon == "true" else 0.125)) < 0.03, frac
