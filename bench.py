@@ -10,11 +10,13 @@ interactions and, for the learned variant, the MLP correction.  Inputs are
 resident in HBM before timing starts; results stay on the device.
 
 Plain tree: monopole and quadrupole cells are both calibrated to the target
-median error and the faster is the plain figure.  Learned: the MLP is
-trained on the device from residual samples of three other Plummer
-snapshots (seeds 41, 43 and 47, tol 0.8/1.0/1.2, cells with size/r in
-[0.45, 1.3]), then the learned walk (quadrupole cells + correction) is
-calibrated to the same target on the seed-123 snapshot.
+median error and the faster is the plain figure.  Learned: for walks with
+quadrupole and with monopole cells, an MLP is trained on the device from
+residual samples of three other Plummer snapshots (seeds 41, 43 and 47, tol
+0.8/1.0/1.2, cells with size/r in [0.45, 1.3]; the residual is the exact
+cell force minus that walk's multipole force), then that learned walk is
+calibrated to the same target on the seed-123 snapshot; the faster of the
+two is the learned figure (both are in `config.learned_variants`).
 `value` is the evals/s of the faster of the two (`config.value_is` names it;
 both variants and their ratio `learned_over_plain` are in `config`); with
 --no-learned it is the plain figure.
@@ -128,7 +130,8 @@ def cpu_baseline(m, x, idx_err, dacc, tol, eps, usequad, budget_s, nthreads=1):
     }
 
 
-def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stride=2, epochs=10, rlo=0.45, rhi=1.3):
+def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stride=2, epochs=10, rlo=0.45, rhi=1.3,
+                  usequad=1):
     """Residual samples of other snapshots, then training on the device.
     The correction (and the samples) cover accepted cells with size/r in
     [rlo, rhi]: below 0.45 the residual is small and correcting it does not
@@ -140,7 +143,7 @@ def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stri
         m, x, _ = E.mkplummer(n, seed=s, with_vel=False)
         eng.upload(m, x)
         for tol in tols:
-            eng.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=1), stride=stride, rlo=rlo, rhi=rhi)
+            eng.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=usequad), stride=stride, rlo=rlo, rhi=rhi)
     nsamp = eng.mlp_nsamples()
     t1 = time.perf_counter()
     loss = eng.mlp_train(epochs=epochs, batch=8192, lr=1e-3)
@@ -375,6 +378,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
                     help="skip the learned calibration and use this tol (profiling runs)")
+    ap.add_argument("--learned-quad", type=int, default=-1,
+                    help="learned walk cells: 1 quadrupole, 0 monopole, -1 both (each with its own model; the faster "
+                         "is the learned figure)")
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
@@ -415,19 +421,25 @@ def main():
         del a, b
     n = args.n
     eng = E.Engine(0)
-    learned_train = None
-    if not args.no_learned and args.model:
-        with np.load(args.model) as z:
-            eng.mlp_set(z["params"], z["norm"])
-        learned_train = {"loaded_from": args.model}
-    elif not args.no_learned:
-        learned_train = train_learned(eng, E, n, args.eps, epochs=args.epochs)
-        log("learned: %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
-            learned_train["samples"], learned_train["t_collect_s"], learned_train["t_train_s"],
-            learned_train["loss_first"], learned_train["loss_last"]))
+
+    def model_path(path, lq):
+        return path[:-4] + "_q%d.npz" % lq if path.endswith(".npz") else path + "_q%d" % lq
+
+    def learned_model(lq):
+        """Load (--model) or train the MLP for learned walks with quadrupole
+        (lq = 1) or monopole (lq = 0) cells."""
+        if args.model:
+            with np.load(model_path(args.model, lq)) as z:
+                eng.mlp_set(z["params"], z["norm"])
+            return {"loaded_from": model_path(args.model, lq)}
+        tr = train_learned(eng, E, n, args.eps, epochs=args.epochs, usequad=lq)
+        log("learned (usequad=%d): %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
+            lq, tr["samples"], tr["t_collect_s"], tr["t_train_s"], tr["loss_first"], tr["loss_last"]))
         if args.save_model:
             params, norm = eng.mlp_get()
-            np.savez(args.save_model, params=params, norm=norm)
+            np.savez(model_path(args.save_model, lq), params=params, norm=norm)
+        return tr
+
     m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
     eng.upload(m, x)
     rng = np.random.default_rng(123)
@@ -476,12 +488,19 @@ def main():
             usequad, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
         plain.append(v)
     best_plain = min(plain, key=lambda v: v["ms_per_step"])
-    learned = None
+    learned_variants = []
     if not args.no_learned:
-        learned = variant(dict(eps=args.eps, usequad=1, group_size=args.group, learned=1), args.learned_tol)
-        log("learned walk: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel %.3f ms, "
-            "%d evaluations)" % (learned["group_size"], learned["nleaf"], learned["tol"], learned["median_rel_err"], learned["p99_rel_err"], learned["ms_per_step"],
-            learned["t_mlp_kernel_ms"], learned["n_mlp"]))
+        for lq in ((1, 0) if args.learned_quad < 0 else (args.learned_quad,)):
+            tr = learned_model(lq)
+            eng.upload(m, x)
+            v = variant(dict(eps=args.eps, usequad=lq, group_size=args.group, learned=1), args.learned_tol)
+            v["training"] = tr
+            log("learned walk usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel "
+                "%.3f ms, %d evaluations)" % (lq, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"],
+                                              v["p99_rel_err"], v["ms_per_step"], v["t_mlp_kernel_ms"], v["n_mlp"]))
+            learned_variants.append(v)
+    learned = min(learned_variants, key=lambda v: v["ms_per_step"]) if learned_variants else None
+    learned_train = learned["training"] if learned else None
     # value: the faster of the two at the target median error (both are in
     # config, with their ratio)
     head_v = learned if learned is not None and learned["evals_per_s"] > best_plain["evals_per_s"] else best_plain
@@ -506,7 +525,7 @@ def main():
             "target_median_err": args.target,
             "err_subset": len(idx),
             "value_is": "learned" if head_v is learned else "plain",
-            "learned": learned, "learned_training": learned_train,
+            "learned": learned, "learned_training": learned_train, "learned_variants": learned_variants,
             "plain": best_plain, "plain_variants": plain,
             "learned_over_plain": (learned["evals_per_s"] / best_plain["evals_per_s"]) if learned else None,
         },

@@ -1656,8 +1656,11 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int lsys = ctx->sysbits / 3;
     // the learned features are written only for the learned walk and sampling
     const bool feat = p.learned != 0 || ctx->mlp_sampling;
+    // the learned features carry the quadrupole (the leading error of a
+    // monopole cell), so a learned or sampling build computes it whatever
+    // the walk's cells are
     for (int L = kMaxLevel; L >= 0; L--) {
-        if (p.usequad)
+        if (p.usequad || feat)
             k_moments_level<true><<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst,
                                                           clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box, lsys,
                                                           ctx->sysbits, sysbox);
