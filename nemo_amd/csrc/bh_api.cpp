@@ -330,7 +330,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
                           &ctx->lvals0, &ctx->lvals1, &ctx->cfnb, &ctx->cowner, &ctx->gowner,
                           &ctx->vel,    &ctx->acc_old, &ctx->lfstep, &ctx->diag_partial, &ctx->diag_buf,
-                          &ctx->n2b_buf, &ctx->nbc_buf, &ctx->lrec,   &ctx->cF0,    &ctx->cF1,
+                          &ctx->n2b_buf, &ctx->nbc_buf, &ctx->lrec, &ctx->sacc,   &ctx->cF0,    &ctx->cF1,
                           &ctx->mlp.params, &ctx->mlp.wb, &ctx->mlp.bias, &ctx->mlp.gpart, &ctx->mlp.lpart,
                           &ctx->mlp.adam_m, &ctx->mlp.adam_v, &ctx->mlp.grad, &ctx->mlp.loss, &ctx->mlp.sx,
                           &ctx->mlp.sy, &ctx->mlp.snew_x, &ctx->mlp.snew_y, &ctx->mlp.soff, &ctx->mlp.corr,

@@ -80,6 +80,7 @@ struct bh_ctx {
 
     bh::Mlp mlp;
     bh::DevBuf lrec;             // per-wave learned-correction records of the walk
+    bh::DevBuf sacc;             // recording walks: (a, phi) per test point in sorted order
     int64_t lcap = 0;            // ints per wave region
     int lnwave = 0;              // waves (regions) of the last recording walk
     bh::DevBuf cF0, cF1;         // per-cell learned features

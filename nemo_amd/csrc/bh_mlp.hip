@@ -591,16 +591,21 @@ __global__ __launch_bounds__(64) void k_learned_reg(const int *__restrict__ ecel
 }
 
 // Adds the correction c to the accelerations of group g's members.
+// The recording walk left each member's (a, phi) in sorted order in sacc;
+// the member's final values (plus the group's correction c) go to the
+// input-order arrays in one write.
 __device__ __forceinline__ void apply_group(int g, float4 c, const int *__restrict__ gfirst,
                                             const int *__restrict__ gcount, const uint32_t *__restrict__ tperm,
-                                            double *acc)
+                                            const double4 *__restrict__ sacc, double *phi, double *acc)
 {
     const int f = gfirst[g], n = gcount[g];
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const int64_t o = tperm[f + k];
-        acc[3 * o] += (double)c.x;
-        acc[3 * o + 1] += (double)c.y;
-        acc[3 * o + 2] += (double)c.z;
+        const double4 r = sacc[f + k];
+        acc[3 * o] = r.x + (double)c.x;
+        acc[3 * o + 1] = r.y + (double)c.y;
+        acc[3 * o + 2] = r.z + (double)c.z;
+        phi[o] = r.w;
     }
 }
 
@@ -610,7 +615,8 @@ __global__ __launch_bounds__(64) void k_group_apply(const int *__restrict__ gsta
                                                     const unsigned long long *__restrict__ stat,
                                                     const float4 *__restrict__ res, const int *__restrict__ gfirst,
                                                     const int *__restrict__ gcount,
-                                                    const uint32_t *__restrict__ tperm, double *acc)
+                                                    const uint32_t *__restrict__ tperm,
+                                                    const double4 *__restrict__ sacc, double *phi, double *acc)
 {
     const int ng = (int)stat[S_NGROUPS];
     const int l = lane_id();
@@ -626,17 +632,18 @@ __global__ __launch_bounds__(64) void k_group_apply(const int *__restrict__ gsta
         ax = wave_sum(ax);
         ay = wave_sum(ay);
         az = wave_sum(az);
-        apply_group(g, make_float4(ax, ay, az, 0.f), gfirst, gcount, tperm, acc);
+        apply_group(g, make_float4(ax, ay, az, 0.f), gfirst, gcount, tperm, sacc, phi, acc);
     }
 }
 
 // Adds the per-group corrections corr[] to the group members' accelerations.
 __global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
                              const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
-                             const float4 *__restrict__ corr, double *acc)
+                             const float4 *__restrict__ corr, const double4 *__restrict__ sacc, double *phi,
+                             double *acc)
 {
     const int ng = (int)stat[S_NGROUPS];
-    for (int g = blockIdx.x; g < ng; g += gridDim.x) apply_group(g, corr[g], gfirst, gcount, tperm, acc);
+    for (int g = blockIdx.x; g < ng; g += gridDim.x) apply_group(g, corr[g], gfirst, gcount, tperm, sacc, phi, acc);
 }
 
 __global__ void k_zero_f4(float4 *p, int n)
@@ -862,6 +869,7 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
     const bool exact = p.learned == 2;   // diagnostic: exact residual at the group centre
     Mlp &M = ctx->mlp;
     const Tree &T = ctx->tree;
+    if (T.nb == 0) return;   // the walk wrote zero forces directly
     hipStream_t st = ctx->stream;
     const int64_t ngcap = ctx->ntest;
     unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
@@ -872,7 +880,7 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         k_exact_corr<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.fnb, T.mo, T.bd, p.eps,
                                                  p.usequad, corr);
         k_apply_corr<<<gridg, 64, 0, st>>>(ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), stat, T.tperm, corr,
-                                           ptr<double>(ctx->acc));
+                                           ptr<double4>(ctx->sacc), ptr<double>(ctx->phi), ptr<double>(ctx->acc));
     } else {
         // flatten the records, one MLP tile per wave, per-group sums
         const int nreg = ctx->lnwave;
@@ -898,7 +906,8 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
             M.norm[0], res);
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
         k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
-                                            T.tperm, ptr<double>(ctx->acc));
+                                            T.tperm, ptr<double4>(ctx->sacc), ptr<double>(ctx->phi),
+                                            ptr<double>(ctx->acc));
     }
     BH_HIP(hipGetLastError());
 }

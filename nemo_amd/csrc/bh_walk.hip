@@ -234,6 +234,7 @@ struct WalkArgs {
     int *gctr;
     unsigned long long *counters;
     double *phi, *acc;
+    double4 *sacc;        // lmode 1: (a, phi) per test point in sorted order (the correction pass writes phi, acc)
     // learned correction: record, per group, the accepted cells whose
     // size/distance lies in [lr_lo, lr_hi] (squared bounds) into a per-wave
     // region: [used][g, n, cx, cy, cz, cell ids ...] ...
@@ -566,11 +567,21 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 // for every system)
                 if (eps2 > 0.f) ph -= (double)(a.bhi[gbase + k].w * rsq(eps2));
             }
-            const int64_t o = a.tperm[gbase + k];
-            a.phi[o] = -ph;
-            a.acc[3 * o] = (double)L.axh[q] + (double)L.axl[q];
-            a.acc[3 * o + 1] = (double)L.ayh[q] + (double)L.ayl[q];
-            a.acc[3 * o + 2] = (double)L.azh[q] + (double)L.azl[q];
+            const double ax = (double)L.axh[q] + (double)L.axl[q];
+            const double ay = (double)L.ayh[q] + (double)L.ayl[q];
+            const double az = (double)L.azh[q] + (double)L.azl[q];
+            if (LREC && a.lmode == 1) {
+                // coalesced; the correction pass adds the group's residual and
+                // writes the input-order arrays once (sample walks, lmode 2,
+                // have no such pass)
+                a.sacc[gbase + k] = make_double4(ax, ay, az, -ph);
+            } else {
+                const int64_t o = a.tperm[gbase + k];
+                a.phi[o] = -ph;
+                a.acc[3 * o] = ax;
+                a.acc[3 * o + 1] = ay;
+                a.acc[3 * o + 2] = az;
+            }
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) nself += __shfl_xor(nself, d, 64);
@@ -641,6 +652,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     a.counters = ptr<unsigned long long>(ctx->counters);
     a.phi = phi;
     a.acc = acc;
+    a.sacc = lmode == 1 ? ensure<double4>(ctx->sacc, std::max<int64_t>(nt, 1)) : nullptr;
     a.lmode = lmode;
     a.lstride = std::max(1, lstride);
     a.lr_lo2 = (float)(ctx->mlp.norm[1] * ctx->mlp.norm[1]);
