@@ -1162,7 +1162,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
     cA[c] = A;
     cB[c] = B;
-    if (QUAD) {
+    if (QUAD && cQ0) {
         cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
         cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
     }
@@ -1661,9 +1661,10 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     // the walk's cells are
     for (int L = kMaxLevel; L >= 0; L--) {
         if (p.usequad || feat)
-            k_moments_level<true><<<mgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub, cfirst,
-                                                          clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box, lsys,
-                                                          ctx->sysbits, sysbox);
+            k_moments_level<true><<<mgrid * 2, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
+                                                              cfirst, clast, clevel, bd, mo, cA, cB,
+                                                              p.usequad ? cQ0 : nullptr, cQ1, d_box, lsys,
+                                                              ctx->sysbits, sysbox);
         else
             k_moments_level<false><<<mgrid * 2, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
                                                                cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box,
