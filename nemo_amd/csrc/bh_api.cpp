@@ -63,6 +63,39 @@ void check_params(const bh_params *p)
         throw ApiFail(BH_ERR_ARG, "the learned correction is trained for the hackcode1 criterion only");
 }
 
+void lf_drop_graph(bh_ctx *ctx)
+{
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_cap = 0;
+}
+
+// Device temporary of one API call, freed on every exit path.
+struct TmpDev {
+    void *p = nullptr;
+    explicit TmpDev(size_t bytes)
+    {
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw ApiFail(BH_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+    }
+    ~TmpDev()
+    {
+        if (p) (void)hipFree(p);
+    }
+    TmpDev(const TmpDev &) = delete;
+    TmpDev &operator=(const TmpDev &) = delete;
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// A new upload replaces the bodies a leapfrog integrates: its state (vel,
+// acc_old sized for the old set) and captured step graph are void, and the
+// bh_lf_* calls report BH_ERR_STATE until bh_lf_init runs again.
 void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, int64_t ntest, const double *tpos)
 {
     if (nmass < 1) throw ApiFail(BH_ERR_ARG, "nmass must be >= 1");
@@ -70,6 +103,8 @@ void upload(bh_ctx *ctx, int64_t nmass, const double *mass, const double *pos, i
     if (!mass || !pos) throw ApiFail(BH_ERR_ARG, "mass/pos is NULL");
     const bool self = (tpos == nullptr || ntest == 0);
     if (!self && (ntest < 0 || ntest > (int64_t)INT32_MAX / 2)) throw ApiFail(BH_ERR_ARG, "bad ntest");
+    ctx->lf_ready = false;
+    lf_drop_graph(ctx);
     ctx->nsys = 0;
     ctx->sysbits = 0;
     ctx->nmass = nmass;
@@ -238,15 +273,6 @@ void lf_upload_state(bh_ctx *ctx, int64_t n, const double *vel)
     BH_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-void lf_drop_graph(bh_ctx *ctx)
-{
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph_exec = nullptr;
-    ctx->graph = nullptr;
-    ctx->graph_cap = 0;
-}
-
 // Parameters of one leapfrog force pass: after the first step the box is kept.
 bh_params lf_params(const bh_ctx *ctx)
 {
@@ -406,17 +432,14 @@ bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, d
         if (!(eps >= 0.0)) throw ApiFail(BH_ERR_ARG, "eps must be >= 0");
         for (int64_t i = 0; i < nidx; i++)
             if (idx[i] < 0 || idx[i] >= ctx->nmass) throw ApiFail(BH_ERR_ARG, "direct-sum index out of range");
-        int64_t *d_idx;
-        double *d_out;
-        BH_HIP(hipMalloc(&d_idx, nidx * sizeof(int64_t)));
-        BH_HIP(hipMalloc(&d_out, 4 * nidx * sizeof(double)));
+        TmpDev t_idx(nidx * sizeof(int64_t)), t_out(4 * nidx * sizeof(double));
+        int64_t *d_idx = t_idx.as<int64_t>();
+        double *d_out = t_out.as<double>();
         BH_HIP(hipMemcpyAsync(d_idx, idx, nidx * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
         bh::direct_sum(ctx, eps, nidx, d_idx, d_out, d_out + nidx);
         BH_HIP(hipMemcpyAsync(phi, d_out, nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipMemcpyAsync(acc, d_out + nidx, 3 * nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
-        BH_HIP(hipFree(d_idx));
-        BH_HIP(hipFree(d_out));
     });
 }
 
@@ -425,11 +448,9 @@ bh_status bh_debug_keys(bh_ctx *ctx, uint64_t *keys)
     return guarded(ctx, [&] {
         if (!ctx->uploaded || !ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
         if (!keys) throw ApiFail(BH_ERR_ARG, "keys is NULL");
-        uint64_t *d;
-        BH_HIP(hipMalloc(&d, ctx->nmass * sizeof(uint64_t)));
-        bh::debug_keys(ctx, d);
-        BH_HIP(hipMemcpy(keys, d, ctx->nmass * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        BH_HIP(hipFree(d));
+        TmpDev t(ctx->nmass * sizeof(uint64_t));
+        bh::debug_keys(ctx, t.as<uint64_t>());
+        BH_HIP(hipMemcpy(keys, t.p, ctx->nmass * sizeof(uint64_t), hipMemcpyDeviceToHost));
     });
 }
 
@@ -645,6 +666,7 @@ bh_status bh_batch_force(bh_ctx *ctx, const bh_params *p, int nsys, const int64_
                          bh_stats *st)
 {
     return guarded(ctx, [&] {
+        check_params(p);
         if (!phi || !acc) throw ApiFail(BH_ERR_ARG, "phi/acc is NULL");
         batch_upload(ctx, nsys, off, mass, pos, eps, tol);
         bh_params q = *p;
@@ -798,15 +820,12 @@ bh_status bh_mlp_forward(bh_ctx *ctx, int64_t n, const float *x, float *y)
     return guarded(ctx, [&] {
         if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
         if (n < 1 || !x || !y) throw ApiFail(BH_ERR_ARG, "bad arguments");
-        float *dx, *dy;
-        BH_HIP(hipMalloc(&dx, n * bh::kMlpIn * sizeof(float)));
-        BH_HIP(hipMalloc(&dy, n * 3 * sizeof(float)));
+        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * 3 * sizeof(float));
+        float *dx = tx.as<float>(), *dy = ty.as<float>();
         BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
         bh::mlp_forward(ctx, dx, n, dy);
         BH_HIP(hipMemcpyAsync(y, dy, n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
-        BH_HIP(hipFree(dx));
-        BH_HIP(hipFree(dy));
     });
 }
 
@@ -815,10 +834,8 @@ bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, fl
     return guarded(ctx, [&] {
         if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
         if (n < 1 || !x || !y || !grad) throw ApiFail(BH_ERR_ARG, "bad arguments");
-        float *dx, *dy, *dg;
-        BH_HIP(hipMalloc(&dx, n * bh::kMlpIn * sizeof(float)));
-        BH_HIP(hipMalloc(&dy, n * 3 * sizeof(float)));
-        BH_HIP(hipMalloc(&dg, (bh::kMlpParams + 1) * sizeof(float)));
+        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * 3 * sizeof(float)), tg((bh::kMlpParams + 1) * sizeof(float));
+        float *dx = tx.as<float>(), *dy = ty.as<float>(), *dg = tg.as<float>();
         BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
         BH_HIP(hipMemcpy(dy, y, n * 3 * sizeof(float), hipMemcpyHostToDevice));
         bh::mlp_grad(ctx, dx, dy, n, dg, dg + bh::kMlpParams);
@@ -827,9 +844,6 @@ bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, fl
         BH_HIP(hipMemcpyAsync(&l, dg + bh::kMlpParams, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
         if (loss) *loss = l;
-        BH_HIP(hipFree(dx));
-        BH_HIP(hipFree(dy));
-        BH_HIP(hipFree(dg));
     });
 }
 

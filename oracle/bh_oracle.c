@@ -53,6 +53,7 @@ typedef struct {
     double pos[3];
     int sub[8];
     double quad[3][3];
+    int bhead, btail, nbk;   /* depth-capped trees: bodies of a bucket cell, in insertion order */
 } orc_cell;
 
 typedef struct {
@@ -65,6 +66,9 @@ typedef struct {
     const double *bpos;  /* [n][3] */
     int usequad;
     int err;             /* 0 ok, 1 cell pool exhausted, 2 out of bits, 3 non-finite */
+    int depth_cap;       /* 0: the reference's tree; > 0: cells stop at this level (buckets) */
+    int *bnext;          /* bucket lists: next body, -1 at the end */
+    int nbucket;         /* bucket cells (cells at level depth_cap) */
 } orc_tree;
 
 static int orc_intcoord(const orc_tree *t, int xp[3], const double rp[3])
@@ -93,6 +97,8 @@ static int orc_makecell(orc_tree *t)
     if (t->ncell >= t->maxcell) { t->err = 1; return -1; }
     int c = t->ncell++;
     for (int i = 0; i < 8; i++) t->cells[c].sub[i] = ORC_NULL;
+    t->cells[c].bhead = t->cells[c].btail = -1;
+    t->cells[c].nbk = 0;
     return c;
 }
 
@@ -142,6 +148,52 @@ static void orc_loadtree(orc_tree *t, int b)
     *qptr = b;
 }
 
+/* Depth-capped insertion: the device's tree (63-bit keys, 21 levels below
+ * the root).  The same descent as loadtree, except that a collision in a slot
+ * of level depth_cap makes that slot a bucket cell, and a body reaching a
+ * bucket is appended to it, so the bodies of a bucket are its children in
+ * insertion (input) order.  Levels count from the root cell (level 0), so
+ * the caller inserts with the final box (no expandbox grafts afterwards). */
+static void orc_loadtree_capped(orc_tree *t, int b)
+{
+    int xp[3], xq[3];
+    orc_intcoord(t, xp, t->bpos + 3 * (size_t)b);
+    int l = ORC_IMAX >> 1, level = 0;
+    int *qptr = &t->root;
+    while (*qptr != ORC_NULL) {
+        if (IS_CELL(*qptr) && t->cells[CELL_OF(*qptr)].nbk > 0) {
+            orc_cell *c = &t->cells[CELL_OF(*qptr)];
+            t->bnext[c->btail] = b;
+            t->bnext[b] = -1;
+            c->btail = b;
+            c->nbk++;
+            return;
+        }
+        if (IS_BODY(*qptr)) {
+            int c = orc_makecell(t);
+            if (c < 0) return;
+            if (level == t->depth_cap) {
+                orc_cell *cc = &t->cells[c];
+                cc->bhead = *qptr;
+                t->bnext[*qptr] = b;
+                t->bnext[b] = -1;
+                cc->btail = b;
+                cc->nbk = 2;
+                t->nbucket++;
+                *qptr = CELL_REF(c);
+                return;
+            }
+            orc_intcoord(t, xq, t->bpos + 3 * (size_t)(*qptr));
+            t->cells[c].sub[orc_subindex(xq, l)] = *qptr;
+            *qptr = CELL_REF(c);
+        }
+        qptr = &t->cells[CELL_OF(*qptr)].sub[orc_subindex(xp, l)];
+        l = l >> 1;
+        level++;
+    }
+    *qptr = b;
+}
+
 static double node_mass(const orc_tree *t, int r)
 {
     return IS_BODY(r) ? t->bmass[r] : t->cells[CELL_OF(r)].mass;
@@ -152,10 +204,51 @@ static const double *node_pos(const orc_tree *t, int r)
     return IS_BODY(r) ? t->bpos + 3 * (size_t)r : t->cells[CELL_OF(r)].pos;
 }
 
+/* hackcofm's sums over a bucket's bodies in list order (the device sums a
+ * bucket's bodies in sorted order, which for equal keys is input order). */
+static void bucket_cofm(orc_tree *t, orc_cell *c)
+{
+    c->mass = 0.0;
+    c->pos[0] = c->pos[1] = c->pos[2] = 0.0;
+    for (int r = c->bhead; r >= 0; r = t->bnext[r]) {
+        double mr = t->bmass[r];
+        const double *pr = t->bpos + 3 * (size_t)r;
+        c->mass += mr;
+        double tmpv[3] = {pr[0] * mr, pr[1] * mr, pr[2] * mr};
+        c->pos[0] = c->pos[0] + tmpv[0];
+        c->pos[1] = c->pos[1] + tmpv[1];
+        c->pos[2] = c->pos[2] + tmpv[2];
+    }
+    for (int k = 0; k < 3; k++) c->pos[k] = c->pos[k] / c->mass;
+    if (!t->usequad) return;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) c->quad[i][j] = 0.0;
+    for (int r = c->bhead; r >= 0; r = t->bnext[r]) {
+        const double *pr = t->bpos + 3 * (size_t)r;
+        double dr[3] = {pr[0] - c->pos[0], pr[1] - c->pos[1], pr[2] - c->pos[2]};
+        double drsq = dr[0] * dr[0];
+        drsq += dr[1] * dr[1];
+        drsq += dr[2] * dr[2];
+        double mr = t->bmass[r];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double idr = (a == b ? 1.0 : 0.0) * drsq;
+                double tm = (dr[a] * dr[b]) * 3.0;
+                tm = tm - idr;
+                tm = tm * mr;
+                c->quad[a][b] = c->quad[a][b] + tm;
+            }
+    }
+}
+
 static void orc_cofm(orc_tree *t, int q)
 {
     if (!IS_CELL(q)) return;
     orc_cell *c = &t->cells[CELL_OF(q)];
+    if (c->nbk > 0) {
+        bucket_cofm(t, c);
+        return;
+    }
     c->mass = 0.0;
     c->pos[0] = c->pos[1] = c->pos[2] = 0.0;
     for (int i = 0; i < 8; i++) {
@@ -199,9 +292,13 @@ static void orc_cofm(orc_tree *t, int q)
 
 /* Build the tree over the massive bodies, in input order (load.c:32-62).
  * rmin[3]/rsize are in/out: the box only grows.  maxcell is the cell pool
- * size (fcells*nbody in the reference).  Returns 0 or an error code. */
-int orc_tree_build(orc_tree **out, int n, const double *mass, const double *pos,
-                   double *rmin, double *rsize, int maxcell, int usequad)
+ * size (fcells*nbody in the reference).  depth_cap 0 builds the reference's
+ * tree; depth_cap > 0 first runs expandbox over every massive body (the box
+ * sequence does not depend on the tree) and then inserts with the final box,
+ * capping cells at that level (see orc_loadtree_capped).  Returns 0 or an
+ * error code. */
+static int tree_build(orc_tree **out, int n, const double *mass, const double *pos, double *rmin, double *rsize,
+                      int maxcell, int usequad, int depth_cap)
 {
     orc_tree *t = (orc_tree *)calloc(1, sizeof(orc_tree));
     t->maxcell = maxcell;
@@ -212,11 +309,20 @@ int orc_tree_build(orc_tree **out, int n, const double *mass, const double *pos,
     t->bmass = mass;
     t->bpos = pos;
     t->usequad = usequad;
-    for (int b = 0; b < n && !t->err; b++) {
-        if (mass[b] != 0.0) {
-            orc_expandbox(t, pos + 3 * (size_t)b);
-            if (t->err) break;
-            orc_loadtree(t, b);
+    t->depth_cap = depth_cap;
+    if (depth_cap > 0) {
+        t->bnext = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+        for (int b = 0; b < n && !t->err; b++)
+            if (mass[b] != 0.0) orc_expandbox(t, pos + 3 * (size_t)b);
+        for (int b = 0; b < n && !t->err; b++)
+            if (mass[b] != 0.0) orc_loadtree_capped(t, b);
+    } else {
+        for (int b = 0; b < n && !t->err; b++) {
+            if (mass[b] != 0.0) {
+                orc_expandbox(t, pos + 3 * (size_t)b);
+                if (t->err) break;
+                orc_loadtree(t, b);
+            }
         }
     }
     if (!t->err && t->root != ORC_NULL) orc_cofm(t, t->root);
@@ -226,15 +332,29 @@ int orc_tree_build(orc_tree **out, int n, const double *mass, const double *pos,
     return t->err;
 }
 
+int orc_tree_build(orc_tree **out, int n, const double *mass, const double *pos,
+                   double *rmin, double *rsize, int maxcell, int usequad)
+{
+    return tree_build(out, n, mass, pos, rmin, rsize, maxcell, usequad, 0);
+}
+
+int orc_tree_build_capped(orc_tree **out, int n, const double *mass, const double *pos,
+                          double *rmin, double *rsize, int maxcell, int usequad, int depth_cap)
+{
+    return tree_build(out, n, mass, pos, rmin, rsize, maxcell, usequad, depth_cap);
+}
+
 void orc_tree_free(orc_tree *t)
 {
     if (!t) return;
     free(t->cells);
+    free(t->bnext);
     free(t);
 }
 
 int orc_tree_ncell(const orc_tree *t) { return t->ncell; }
 int orc_tree_root(const orc_tree *t) { return t->root; }
+int orc_tree_nbucket(const orc_tree *t) { return t->nbucket; }
 
 /* Export cells in DFS pre-order (children 0..7): level, mass, com, quad[9],
  * and the body count under each cell.  Returns the number written. */
@@ -249,7 +369,7 @@ static int dump_rec(const orc_tree *t, int r, int level, orc_dump *d)
     for (int k = 0; k < 3; k++) d->pos[3 * me + k] = c->pos[k];
     for (int a = 0; a < 9; a++) d->quad[9 * me + a] = t->usequad ? c->quad[a / 3][a % 3] : 0.0;
     d->level[me] = level;
-    int nb = 0;
+    int nb = c->nbk;
     for (int i = 0; i < 8; i++)
         if (c->sub[i] != ORC_NULL) nb += dump_rec(t, c->sub[i], level + 1, d);
     d->nbody[me] = nb;
@@ -330,6 +450,7 @@ static void orc_walksub(orc_walk *w, int r, double dsq)
         drsq += dr[2] * dr[2];
         if (w->tolsq * drsq < dsq) {
             const orc_cell *c = &t->cells[CELL_OF(r)];
+            for (int b = c->bhead; b >= 0; b = t->bnext[b]) orc_walksub(w, b, dsq / 4.0);
             for (int k = 0; k < 8; k++)
                 if (c->sub[k] != ORC_NULL) orc_walksub(w, c->sub[k], dsq / 4.0);
             return;
@@ -365,15 +486,18 @@ void orc_tree_grav(const orc_tree *t, const double *pos0, int skip, double tol, 
 
 /* hackforce force_calc (hackforce.c:177-215).  ntest/tpos NULL => the mass
  * set is the test set and each body skips itself.  Counters are 64-bit
- * totals (the reference's are 32-bit ints).  Returns the tree error. */
-int orc_force(int n, const double *mass, const double *pos, int ntest, const double *tpos,
-              double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
-              double *phi, double *acc, int64_t *n2btot, int64_t *nbctot)
+ * totals (the reference's are 32-bit ints).  depth_cap as tree_build; the
+ * bucket count goes to *nbucket when it is not NULL.  Returns the tree
+ * error. */
+int orc_force_capped(int n, const double *mass, const double *pos, int ntest, const double *tpos,
+                     double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
+                     int depth_cap, double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, int64_t *nbucket)
 {
     orc_tree *t;
     int maxcell = (int)(fcells * n);
-    int err = orc_tree_build(&t, n, mass, pos, rmin, rsize, maxcell, usequad);
+    int err = tree_build(&t, n, mass, pos, rmin, rsize, maxcell, usequad, depth_cap);
     *n2btot = *nbctot = 0;
+    if (nbucket) *nbucket = t->nbucket;
     if (err) { orc_tree_free(t); return err; }
     int self = (tpos == NULL);
     if (self) { ntest = n; tpos = pos; }
@@ -385,6 +509,14 @@ int orc_force(int n, const double *mass, const double *pos, int ntest, const dou
     }
     orc_tree_free(t);
     return 0;
+}
+
+int orc_force(int n, const double *mass, const double *pos, int ntest, const double *tpos,
+              double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
+              double *phi, double *acc, int64_t *n2btot, int64_t *nbctot)
+{
+    return orc_force_capped(n, mass, pos, ntest, tpos, tol, eps, usequad, rmin, rsize, fcells, 0, phi, acc,
+                            n2btot, nbctot, NULL);
 }
 
 /* FP64 direct sum (directcode/grav.c:23-65, eps >= 0 branch, gravc = 1).

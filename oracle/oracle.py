@@ -35,6 +35,9 @@ def lib():
         L.orc_force.argtypes = [C.c_int, _d, _d, C.c_int, _d, C.c_double, C.c_double, C.c_int,
                                 _d, _d, C.c_double, _d, _d, _i64, _i64]
         L.orc_force.restype = C.c_int
+        L.orc_force_capped.argtypes = [C.c_int, _d, _d, C.c_int, _d, C.c_double, C.c_double, C.c_int,
+                                       _d, _d, C.c_double, C.c_int, _d, _d, _i64, _i64, _i64]
+        L.orc_force_capped.restype = C.c_int
         L.orc_direct.argtypes = [C.c_int, _d, _d, C.c_int, _i64, C.c_double, _d, _d]
         L.orc_direct.restype = None
         L.orc_step_force.argtypes = [C.c_int, _d, _d, _d, _d, _d, C.c_int, C.c_double, C.c_double,
@@ -52,6 +55,10 @@ def lib():
         L.orc_morton.argtypes = [C.c_int, _d, _d, C.c_double, _u64]
         L.orc_tree_build.argtypes = [C.POINTER(C.c_void_p), C.c_int, _d, _d, _d, _d, C.c_int, C.c_int]
         L.orc_tree_build.restype = C.c_int
+        L.orc_tree_build_capped.argtypes = [C.POINTER(C.c_void_p), C.c_int, _d, _d, _d, _d, C.c_int, C.c_int,
+                                            C.c_int]
+        L.orc_tree_build_capped.restype = C.c_int
+        L.orc_tree_nbucket.argtypes = [C.c_void_p]
         L.orc_tree_free.argtypes = [C.c_void_p]
         L.orc_tree_ncell.argtypes = [C.c_void_p]
         L.orc_tree_dump.argtypes = [C.c_void_p, _d, _d, _d, _i32, _i32]
@@ -67,8 +74,22 @@ def _p(a, t=_d):
     return a.ctypes.data_as(t)
 
 
-def force(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fcells=2.0, tpos=None):
-    """hackforce force_calc: returns (phi, acc, n2btot, nbctot, rmin, rsize)."""
+def force(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fcells=2.0, tpos=None, depth_cap=0):
+    """hackforce force_calc: returns (phi, acc, n2btot, nbctot, rmin, rsize).
+
+    depth_cap=0 is the reference's tree (load.c:101-128 splits down to 2^30
+    and fails on coincident bodies); depth_cap=21 stops cells at level 21, a
+    colliding slot there becoming a bucket whose bodies are its children in
+    input order: the device's 63-bit-key tree.  See force_capped."""
+    out = force_capped(mass, pos, tol, eps, usequad, rsize, rmin, fcells, tpos, depth_cap)
+    return out[:6]
+
+
+def force_capped(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fcells=2.0, tpos=None,
+                 depth_cap=21):
+    """As force(), with the tree's depth capped (default: the device's 21
+    levels).  Returns (phi, acc, n2btot, nbctot, rmin, rsize, nbucket), where
+    nbucket counts the bucket cells (cells at level depth_cap)."""
     mass = np.ascontiguousarray(mass, np.float64)
     pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
     n = len(mass)
@@ -81,15 +102,16 @@ def force(mass, pos, tol=1.0, eps=0.05, usequad=False, rsize=4.0, rmin=None, fce
     acc = np.zeros((nt, 3))
     a = np.zeros(1, np.int64)
     b = np.zeros(1, np.int64)
+    nbk = np.zeros(1, np.int64)
     tp = None
     if tpos is not None:
         tpos = np.ascontiguousarray(tpos, np.float64).reshape(-1, 3)
         tp = _p(tpos)
-    err = lib().orc_force(n, _p(mass), _p(pos), nt, tp, tol, eps, int(usequad), _p(rmin), _p(rs),
-                          fcells, _p(phi), _p(acc), _p(a, _i64), _p(b, _i64))
+    err = lib().orc_force_capped(n, _p(mass), _p(pos), nt, tp, tol, eps, int(usequad), _p(rmin), _p(rs),
+                                 fcells, int(depth_cap), _p(phi), _p(acc), _p(a, _i64), _p(b, _i64), _p(nbk, _i64))
     if err:
         raise RuntimeError("oracle tree error %d" % err)
-    return phi, acc, int(a[0]), int(b[0]), rmin, float(rs[0])
+    return phi, acc, int(a[0]), int(b[0]), rmin, float(rs[0]), int(nbk[0])
 
 
 def force_sample(mass, pos, idx, tol, eps, usequad, rsize=4.0, fcells=2.0, nthreads=1):
@@ -138,8 +160,9 @@ def morton(pos, rmin, rsize):
     return keys
 
 
-def tree_cells(mass, pos, rmin, rsize, usequad=True, maxcell=None):
-    """Cells of the reference tree in DFS pre-order: dict of arrays."""
+def tree_cells(mass, pos, rmin, rsize, usequad=True, maxcell=None, depth_cap=0):
+    """Cells of the reference tree in DFS pre-order: dict of arrays
+    (depth_cap as in force(); nbucket counts the bucket cells)."""
     mass = np.ascontiguousarray(mass, np.float64)
     pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
     n = len(mass)
@@ -147,7 +170,8 @@ def tree_cells(mass, pos, rmin, rsize, usequad=True, maxcell=None):
     rs = np.array([rsize], np.float64)
     maxcell = maxcell or 2 * n + 64
     h = C.c_void_p()
-    err = lib().orc_tree_build(C.byref(h), n, _p(mass), _p(pos), _p(rmin), _p(rs), maxcell, int(usequad))
+    err = lib().orc_tree_build_capped(C.byref(h), n, _p(mass), _p(pos), _p(rmin), _p(rs), maxcell, int(usequad),
+                                      int(depth_cap))
     try:
         if err:
             raise RuntimeError("oracle tree error %d" % err)
@@ -158,7 +182,8 @@ def tree_cells(mass, pos, rmin, rsize, usequad=True, maxcell=None):
         lv = np.zeros(nc, np.int32)
         nb = np.zeros(nc, np.int32)
         k = lib().orc_tree_dump(h, _p(m), _p(p), _p(q), _p(lv, _i32), _p(nb, _i32))
-        return dict(mass=m[:k], pos=p[:k], quad=q[:k], level=lv[:k], nbody=nb[:k], rmin=rmin, rsize=float(rs[0]))
+        return dict(mass=m[:k], pos=p[:k], quad=q[:k], level=lv[:k], nbody=nb[:k], rmin=rmin, rsize=float(rs[0]),
+                    nbucket=int(lib().orc_tree_nbucket(h)))
     finally:
         lib().orc_tree_free(h)
 

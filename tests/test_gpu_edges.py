@@ -89,6 +89,12 @@ def test_coincident_bodies(eng):
         assert st.deep_buckets == 2
         e = _rel(acc[idx], acc_d)
         assert (e.max() if tol == 0.0 else np.median(e)) < band, (mode, tol)
+        if mode == "strict":
+            # the oracle's depth-capped tree (the reference itself fails on
+            # coincident bodies) gives the same bits and counts
+            rphi, racc, n2b, nbc, _, _, nbk = oracle.force_capped(m, x, tol=tol, eps=0.02, usequad=True)
+            assert nbk == 2 and (st.n2btot, st.nbctot) == (n2b, nbc)
+            assert np.array_equal(phi, rphi) and np.array_equal(acc, racc)
 
 
 def test_box_boundary_bodies(eng):

@@ -80,3 +80,43 @@ def test_fast_leapfrog_energy_and_oracle(eng):
     # different force errors (FP32 group walk vs FP64 per-particle walk) in a
     # chaotic system: the trajectories agree to the force-error level
     assert np.median(np.linalg.norm(pos - ref.pos, axis=1)) < 5e-3
+
+
+def test_upload_ends_the_leapfrog_state():
+    """A force evaluation on other bodies between leapfrog steps replaces the
+    bodies the integrator holds: every bh_lf_* call then reports
+    BH_ERR_STATE (no kick or drift over arrays sized for the old set) until
+    bh_lf_init runs again, and the new integration is the oracle's."""
+    e = E.Engine(0)
+    try:
+        m, x, v = E.mkplummer(500, seed=3)
+        lf = E.Leapfrog(e, m, x, v, 1.0 / 32.0, tol=0.7, eps=0.05, mode="strict")
+        lf.run(3, graph=True)
+        m2, x2, _ = E.mkplummer(2000, seed=4, with_vel=False)
+        e.force(m2, x2, E.default_params(tol=0.7, eps=0.05))
+        for call in (lambda: lf.run(2, graph=True), lambda: lf.run(1, graph=False), lf.force, lf.drift, lf.state):
+            with pytest.raises(E.BHError) as ei:
+                call()
+            assert ei.value.code == 6, ei.value
+        lf = E.Leapfrog(e, m, x, v, 1.0 / 32.0, tol=0.7, eps=0.05, mode="strict")
+        d, n2b, nbc = lf.run(3, graph=True)
+        ref = oracle.Leapfrog(m, x, v, tol=0.7, eps=0.05, fcells=2.0)
+        for k in range(3):
+            rd, rn2b, rnbc = ref.step()
+            assert (n2b[k], nbc[k]) == (rn2b, rnbc)
+            np.testing.assert_allclose(d[k], rd, rtol=1e-12, atol=1e-15)
+    finally:
+        e.close()
+
+
+def test_batch_force_null_params():
+    e = E.Engine(0)
+    try:
+        off = np.array([0, 10], np.int64)
+        m, x, _ = E.mkplummer(10, seed=1, with_vel=False)
+        phi, acc = np.zeros(10), np.zeros((10, 3))
+        st = E.lib().bh_batch_force(e._h, None, 1, E._p(off, E._i64), E._p(m), E._p(x), E._p(np.full(1, 0.05)),
+                                    E._p(np.full(1, 0.7)), E._p(phi), E._p(acc), None)
+        assert st == 1
+    finally:
+        e.close()

@@ -1,17 +1,20 @@
-"""Seeded random cases over the whole force path: sizes from 2 to 6000
-(around the radix tile of 2048 keys and the pyramid blocks of 64 and 4096),
-uniform, Plummer, clustered (deep trees) and shell distributions, equal and
-unequal masses with massless bodies, tol from 0 to 1.3, eps 0 to 0.05,
-monopole and quadrupole cells, with and without separate test points (some
-outside the tree's box).
+"""Seeded random cases over the whole force path (tests/helpers/random_cases.py:
+sizes 2 to 6000 around the radix tile and pyramid blocks, uniform, Plummer,
+clustered and shell distributions, equal and unequal masses with massless
+bodies, tol 0 to 1.3, eps 0 to 0.05, monopole and quadrupole cells, separate
+test points, and tight clusters whose bodies share 21-level keys).
 
 Strict mode equals the oracle's hackforce restatement bit for bit (phi, a,
-interaction counts, final box) whenever no depth-21 bucket forms (the
-reference would split coincident bodies without end); the fast walk at tol
-= 0 equals the FP64 direct sum to FP32 rounding."""
+interaction counts, final box) on every seed: the oracle's depth-capped tree
+(cells stop at level 21; a level-21 cell's bodies are its children in input
+order) is the device's tree, and with no bucket it is the reference's tree
+itself (both oracle modes are compared then).  The device's bucket count
+equals the oracle's.  The fast walk at tol = 0 equals the FP64 direct sum to
+FP32 rounding."""
 import numpy as np
 import pytest
 
+from helpers import random_cases
 from nemo_amd import engine as E
 from oracle import oracle
 
@@ -25,45 +28,24 @@ def eng():
     e.close()
 
 
-def _case(seed):
-    rng = np.random.default_rng(1000 + seed)
-    n = int(rng.choice([2, 3, 17, 63, 64, 65, 200, 1000, 2047, 2049, 4097, 6000]))
-    dist = ["uniform", "plummer", "clustered", "shell"][seed % 4]
-    if dist == "uniform":
-        x = rng.uniform(-1.5, 1.5, (n, 3))
-    elif dist == "plummer":
-        _, x, _ = E.mkplummer(n, seed=seed + 7, with_vel=False)
-        x = x * 0.8
-    elif dist == "clustered":
-        k = max(1, n // 200)
-        c = rng.uniform(-1.0, 1.0, (k, 3))
-        x = c[rng.integers(0, k, n)] + rng.normal(scale=10.0 ** rng.uniform(-5, -2), size=(n, 3))
-    else:
-        v = rng.normal(size=(n, 3))
-        x = 1.2 * v / np.linalg.norm(v, axis=1, keepdims=True)
-    m = np.full(n, 1.0 / n) if seed % 3 else rng.lognormal(size=n) / n
-    if seed % 5 == 1 and n > 3:
-        m[rng.random(n) < 0.1] = 0.0
-    tol = float(rng.choice([0.0, 0.3, 0.7, 1.0, 1.3]))
-    eps = float(rng.choice([0.0, 0.001, 0.05]))
-    usequad = int(seed % 2)
-    tpos = rng.uniform(-2.5, 2.5, (int(rng.integers(1, 300)), 3)) if seed % 4 == 3 else None
-    return m, x, tol, eps, usequad, tpos
-
-
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(random_cases.NSEEDS))
 def test_random_case_strict_and_fast(eng, seed):
-    m, x, tol, eps, usequad, tpos = _case(seed)
+    m, x, tol, eps, usequad, tpos = random_cases.case(seed)
     print("seed", seed, "n", len(m), "tol", tol, "eps", eps, "quad", usequad, "test", None if tpos is None else len(tpos),
           flush=True)
     p = E.default_params(tol=tol, eps=eps, usequad=usequad, mode="strict")
     phi, acc, st = eng.force(m, x, p, tpos=tpos)
     assert np.isfinite(phi).all() and np.isfinite(acc).all()
-    if st.deep_buckets == 0:
-        rphi, racc, n2b, nbc, rmin, rsize = oracle.force(m, x, tol=tol, eps=eps, usequad=bool(usequad), tpos=tpos)
-        assert (st.rsize, list(st.rmin)) == (rsize, list(rmin))
-        assert (st.n2btot, st.nbctot) == (n2b, nbc)
-        assert np.array_equal(phi, rphi) and np.array_equal(acc, racc)
+    rphi, racc, n2b, nbc, rmin, rsize, nbk = oracle.force_capped(m, x, tol=tol, eps=eps, usequad=bool(usequad),
+                                                                 tpos=tpos, depth_cap=21)
+    print("seed", seed, "buckets", st.deep_buckets, "oracle", nbk, flush=True)
+    assert st.deep_buckets == nbk
+    assert (st.rsize, list(st.rmin)) == (rsize, list(rmin))
+    assert (st.n2btot, st.nbctot) == (n2b, nbc)
+    assert np.array_equal(phi, rphi) and np.array_equal(acc, racc)
+    if nbk == 0:
+        r0 = oracle.force(m, x, tol=tol, eps=eps, usequad=bool(usequad), tpos=tpos)
+        assert np.array_equal(phi, r0[0]) and np.array_equal(acc, r0[1]) and (n2b, nbc) == r0[2:4]
     if eps == 0.0:
         return     # coincident or very close pairs make the unsoftened direct sum a poor reference
     pts = x if tpos is None else tpos

@@ -189,3 +189,112 @@ def test_threaded_sample_equals_single_thread_and_force():
     assert np.array_equal(r1[0], r4[0]) and np.array_equal(r1[1], r4[1]) and r1[2:4] == r4[2:4]
     phi, acc, *_ = oracle.force(mass, pos, tol=0.8, eps=0.03, usequad=True)
     assert np.array_equal(r1[0], phi[idx]) and np.array_equal(r1[1], acc[idx])
+
+
+def _clustered(n, seed, scale):
+    """Tight clusters: many pairs share a 21-level key (2^-21 of the box)."""
+    rng = np.random.default_rng(seed)
+    k = max(1, n // 50)
+    c = rng.uniform(-1.0, 1.0, (k, 3))
+    pos = c[rng.integers(0, k, n)] + rng.normal(scale=scale, size=(n, 3))
+    mass = rng.uniform(0.5, 1.5, n) / n
+    return mass, pos
+
+
+def test_capped_tree_equals_reference_without_buckets():
+    """Depth cap 21 changes nothing when no two bodies share a 21-level key:
+    same box, cells, counts, phi and a bit for bit (inserting with the final
+    box builds the same tree as expandbox's grafts)."""
+    for seed, (n, spread) in enumerate([(500, 1.0), (2000, 0.3), (64, 3.0)]):
+        mass, pos = _rand_bodies(n, 40 + seed, spread)
+        pos[0] = [5.0, -7.0, 2.5]          # forces box expansion with grafts
+        for tol, quad in ((0.0, False), (0.7, True), (1.2, False)):
+            r0 = oracle.force(mass, pos, tol=tol, eps=0.02, usequad=quad)
+            r1 = oracle.force_capped(mass, pos, tol=tol, eps=0.02, usequad=quad)
+            assert r1[6] == 0
+            assert np.array_equal(r0[0], r1[0]) and np.array_equal(r0[1], r1[1])
+            assert r0[2:4] == r1[2:4] and r0[5] == r1[5] and np.array_equal(r0[4], r1[4])
+
+
+def test_capped_tree_cells_are_21_level_prefixes():
+    """With the cap, the cells are exactly the 21-level key prefixes shared by
+    >= 2 bodies (brute force from the keys); the level-21 cells are the
+    buckets, and each holds the bodies of one full key."""
+    mass, pos = _clustered(800, 5, 2e-7)
+    rmin = np.full(3, -2.0)
+    keys = _keys_brute(pos, rmin, 4.0)
+    t = oracle.tree_cells(mass, pos, rmin, 4.0, usequad=True, depth_cap=21)
+    assert t["rsize"] == 4.0 and t["level"].max() == 21
+    want = {}
+    for L in range(0, 22):
+        pre = keys >> np.uint64(63 - 3 * L) if L else np.zeros_like(keys)
+        u, cnt = np.unique(pre, return_counts=True)
+        for v, c in zip(u, cnt):
+            if c >= 2:
+                want.setdefault(L, []).append(int(c))
+    got = {}
+    for lv, nb in zip(t["level"], t["nbody"]):
+        got.setdefault(int(lv), []).append(int(nb))
+    assert {L: sorted(v) for L, v in got.items()} == {L: sorted(v) for L, v in want.items()}
+    assert t["nbucket"] == len(want[21]) > 5
+
+
+def test_capped_bucket_moments_and_walk_closed_form():
+    """A bucket's c.o.m. and quadrupole are the point-set sums over its
+    bodies; opened, its bodies are visited one by one (tol 0 sums every
+    pair: equal to the direct sum to rounding, n2b = n(n-1))."""
+    h = 4.0 / 2 ** 30
+    cell = 512 * h                                                # side of a level-21 cell
+    base = np.floor((np.array([0.1234567, -0.3456789, 0.2345678]) + 2.0) / cell) * cell - 2.0 + 16 * h
+    trio = np.array([base + 320 * h, base + 64 * h, base])      # one 21-level key, reverse octant order
+    rng = np.random.default_rng(9)
+    pos = np.vstack([trio, rng.uniform(-1, 1, (60, 3))])
+    mass = rng.uniform(0.5, 1.5, len(pos)) / len(pos)
+    t = oracle.tree_cells(mass, pos, np.full(3, -2.0), 4.0, usequad=True, depth_cap=21)
+    k = int(np.nonzero(t["level"] == 21)[0][0])
+    assert t["nbucket"] == 1 and t["nbody"][k] == 3
+    m3 = mass[:3]
+    com = (m3[:, None] * trio).sum(0) / m3.sum()
+    d = trio - com
+    q = np.einsum("i,ij,ik->jk", m3, d, d) * 3 - np.eye(3) * (m3 * (d * d).sum(1)).sum()
+    assert np.allclose(t["pos"][k], com, rtol=1e-15, atol=0)
+    assert np.allclose(t["quad"][k].reshape(3, 3), q, rtol=1e-9, atol=1e-9 * np.abs(q).max())
+    phi, acc, n2b, nbc, _, _, nbk = oracle.force_capped(mass, pos, tol=0.0, eps=0.01, usequad=True)
+    dphi, dacc = oracle.direct(mass, pos, 0.01)
+    n = len(mass)
+    assert (n2b, nbc, nbk) == (n * (n - 1), 0, 1)
+    assert np.allclose(phi, dphi, rtol=1e-13) and np.allclose(acc, dacc, rtol=1e-11, atol=1e-15)
+    # the reference tree splits the trio below level 21 and walks it in octant order
+    r0 = oracle.force(mass, pos, tol=1.0, eps=0.01, usequad=True)
+    r1 = oracle.force_capped(mass, pos, tol=1.0, eps=0.01, usequad=True)
+    assert r0[2:4] != r1[2:4]
+
+
+def test_capped_tree_takes_coincident_bodies():
+    """Coincident bodies: the reference fails (out of bits, load.c:113); the
+    capped tree puts them in one bucket, and tol 0 equals the direct sum."""
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-1, 1, (40, 3))
+    pos[10] = pos[3]
+    pos[20] = pos[3]
+    mass = np.full(40, 1.0 / 40)
+    with pytest.raises(RuntimeError):
+        oracle.force(mass, pos, tol=0.5, eps=0.05)
+    phi, acc, n2b, nbc, _, _, nbk = oracle.force_capped(mass, pos, tol=0.0, eps=0.05)
+    dphi, dacc = oracle.direct(mass, pos, 0.05)
+    assert nbk == 1 and n2b == 40 * 39
+    assert np.allclose(phi, dphi, rtol=1e-13) and np.allclose(acc, dacc, rtol=1e-11, atol=1e-15)
+
+
+def test_random_parity_cases_cover_buckets():
+    """The GPU random parity cases include seeds whose device tree forms
+    depth-21 buckets and seeds where it does not (counted by the capped
+    oracle), so both tree forms are compared bit for bit."""
+    from helpers import random_cases
+
+    nb = []
+    for s in range(random_cases.NSEEDS):
+        m, x, tol, eps, uq, tp = random_cases.case(s)
+        nb.append(oracle.force_capped(m, x, tol=tol, eps=eps, usequad=bool(uq), tpos=tp)[6])
+    nb = np.array(nb)
+    assert (nb > 0).sum() >= 6 and (nb == 0).sum() >= 20
