@@ -23,12 +23,14 @@ both variants and their ratio `learned_over_plain` are in `config`); with
 
 Prints one JSON line.  `roofline` is for the dominant kernel, the walk: 20
 flop per interaction (P2P and P2C, counted by the kernel) over the walk
-kernel's own event-timed duration, against MI355X's FP32 peak (the walk is
-FP32 vector-ALU bound; "mfma" is the compute-bound label of the schema).
+kernel's own event-timed duration, against MI355X's FP32 vector peak (the
+walk issues no MFMA: its bound is labelled "valu", the vector ALU).
 `roofline_mlp` is the MLP inference kernel: 18,304 flop per evaluation
 against the dense BF16 MFMA peak.  `cpu_baseline` times the CPU restatement
-of hackforce (FP64, single thread) on a bounded sample of the same snapshot:
-one full tree build plus the walk of a subset of bodies, extrapolated to N.
+of hackforce (FP64, single thread) on a bounded sample of the same snapshot
+at the oracle's own tol for the same median error (bisected on the error
+subset, monopole and quadrupole cells, the faster reported): one full tree
+build plus the walk of a subset of bodies, extrapolated to N.
 `cpu_baseline_allcores` is the same with the walk on every host thread
 (at most 16) over the one tree.
 """
@@ -97,37 +99,73 @@ def host_threads():
     return max(1, min(len(os.sched_getaffinity(0)), 16))
 
 
-def cpu_baseline(m, x, idx_err, dacc, tol, eps, usequad, budget_s, nthreads=1):
+def cpu_tol_at_target(tree, idx_err, dacc, eps, target, nthreads, lo=0.2, hi=1.6, iters=9):
+    """Largest tol at which the oracle's own walk (per-body opening test, FP64)
+    has median |da|/|a| <= target on the error subset (bisection over one tree)."""
+    best = None
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        _, acc, _, _, _ = tree.walk(idx_err, mid, eps, nthreads=nthreads)
+        med = float(np.median(rel_err(acc, dacc)))
+        if med <= target:
+            best, lo = (mid, med), mid
+        else:
+            hi = mid
+    return best if best is not None else (lo, float("nan"))
+
+
+def cpu_baseline(m, x, idx_err, dacc, eps, target, budget_s, nthreads_list=(1,)):
+    """The oracle (hackforce restated, FP64) on the host: for monopole and
+    quadrupole cells, tol bisected on the error subset to median |da|/|a| =
+    target, then one tree build and the walk of a bounded sample at that tol,
+    extrapolated to N; the faster cell type is reported, per thread count."""
     from oracle import oracle
 
-    # pilot: size the sample for the time budget
-    pilot = idx_err[:64]
-    _, _, _, _, t_tree, t_walk = oracle.force_sample(m, x, pilot, tol, eps, usequad)
-    per = t_walk / len(pilot)
     n = len(m)
-    ns = int(max(64, min(n, (budget_s - 2 * t_tree) / max(per, 1e-9) * nthreads)))
-    # the error subset first, then further seeded random bodies up to the budget
-    rest = np.setdiff1d(np.random.default_rng(7).permutation(n)[: ns + len(idx_err)], idx_err)
-    sample = np.concatenate([idx_err, rest])[:ns]
-    sample = np.sort(sample)
-    _, acc, n2b, nbc, t_tree, t_walk = oracle.force_sample(m, x, sample, tol, eps, usequad, nthreads=nthreads)
-    per = t_walk / ns
-    t_full = t_tree + per * n
-    have = np.isin(idx_err, sample)
-    e = rel_err(acc[np.searchsorted(sample, idx_err[have])], dacc[have])
-    how = ("1 thread" if nthreads == 1 else
-           "%d threads walking slices of the sample over one tree, as %d processes that each build the tree and "
-           "walk 1/%d of the bodies would" % (nthreads, nthreads, nthreads))
-    return {
-        "value": n / t_full,
-        "unit": "evals/s",
-        "cores": nthreads,
-        "kind": "port",
-        "sample": "oracle hackforce restatement (FP64, %s): full tree build of N=%d (%.2f s) + walk of %d "
-                  "bodies (%.2f s, %.0f interactions/body), extrapolated to N; same tol=%.4f eps=%g quad=%d; "
-                  "median |da|/|a| on the error subset %.3g" % (how, n, t_tree, ns, t_walk, (n2b + nbc) / ns, tol,
-                                                                eps, usequad, float(np.median(e))),
-    }
+    nt_cal = host_threads()
+    runs = {}
+    for usequad in (0, 1):
+        tree = oracle.Tree(m, x, usequad)
+        try:
+            tol, med = cpu_tol_at_target(tree, idx_err, dacc, eps, target, nt_cal)
+            for nthreads in nthreads_list:
+                # pilot: size the sample for the time budget
+                pilot = idx_err[:64]
+                per = tree.walk(pilot, tol, eps)[4] / len(pilot)
+                ns = int(max(64, min(n, (budget_s / 4 - tree.t_build) / max(per, 1e-9) * nthreads)))
+                rest = np.setdiff1d(np.random.default_rng(7).permutation(n)[: ns + len(idx_err)], idx_err)
+                sample = np.sort(np.concatenate([idx_err, rest])[:ns])
+                _, acc, n2b, nbc, t_walk = tree.walk(sample, tol, eps, nthreads=nthreads)
+                t_full = tree.t_build + t_walk / ns * n
+                have = np.isin(idx_err, sample)
+                e = rel_err(acc[np.searchsorted(sample, idx_err[have])], dacc[have])
+                runs[(usequad, nthreads)] = dict(value=n / t_full, tol=tol, med_cal=med, t_build=tree.t_build,
+                                                 ns=ns, t_walk=t_walk, ipb=(n2b + nbc) / ns,
+                                                 med=float(np.median(e)), p99=float(np.percentile(e, 99)))
+        finally:
+            tree.close()
+    out = {}
+    for nthreads in nthreads_list:
+        uq = max((0, 1), key=lambda q: runs[(q, nthreads)]["value"])
+        r, o = runs[(uq, nthreads)], runs[(1 - uq, nthreads)]
+        how = ("1 thread" if nthreads == 1 else
+               "%d threads walking slices of the sample over one tree, as %d processes that each build the tree "
+               "and walk 1/%d of the bodies would" % (nthreads, nthreads, nthreads))
+        out[nthreads] = {
+            "value": r["value"],
+            "unit": "evals/s",
+            "cores": nthreads,
+            "kind": "port",
+            "sample": "oracle hackforce restatement (FP64, %s), %s cells at its own tol=%.4f, bisected on the "
+                      "%d-body error subset to median |da|/|a| %.3g (target %g; p99 %.3g); full tree build of "
+                      "N=%d (%.2f s) + walk of %d bodies (%.2f s, %.0f interactions/body), extrapolated to N; "
+                      "%s cells at tol=%.4f: %.3g evals/s" % (
+                          how, "quadrupole" if uq else "monopole", r["tol"], len(idx_err), r["med"], target,
+                          r["p99"], n,
+                          r["t_build"], r["ns"], r["t_walk"], r["ipb"], "monopole" if uq else "quadrupole",
+                          o["tol"], o["value"]),
+        }
+    return out
 
 
 def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stride=2, epochs=10, rlo=0.45, rhi=1.3,
@@ -185,7 +223,7 @@ def profiled_traffic(kernel, profile="walk_plummer4M_plain_mono.md"):
 
 def walk_roofline(n_int, t_walk_s, kernel):
     achieved = FLOP_PER_INTERACTION * n_int / t_walk_s / 1e12
-    return {"bound": "mfma", "compute_unit": "FP32 vector ALU (the walk issues no MFMA)", "kernel": kernel,
+    return {"bound": "valu", "compute_unit": "FP32 vector ALU (the walk issues no MFMA)", "kernel": kernel,
             "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
             "flop_convention": "20 flop per interaction (P2P and P2C), interactions counted by the kernel"}
@@ -530,7 +568,7 @@ def main():
             "learned_over_plain": (learned["evals_per_s"] / best_plain["evals_per_s"]) if learned else None,
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
             "kernel": "k_walk<%d,%s,false,%s,false>" % (head_v["group_size"] // 64,
                                                        "true" if head_v["usequad"] else "false",
@@ -553,12 +591,12 @@ def main():
                                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a_mlp / BF16_PEAK_TFLOPS,
                                "traffic": None, "flop_convention": "18,304 flop per MLP evaluation"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, x, idx, dacc, best_plain["tol"], args.eps, best_plain["usequad"],
-                                           args.cpu_budget)
         nt = host_threads()
+        cb = cpu_baseline(m, x, idx, dacc, args.eps, args.target, args.cpu_budget,
+                          nthreads_list=(1, nt) if nt > 1 else (1,))
+        out["cpu_baseline"] = cb[1]
         if nt > 1:
-            out["cpu_baseline_allcores"] = cpu_baseline(m, x, idx, dacc, best_plain["tol"], args.eps,
-                                                        best_plain["usequad"], args.cpu_budget / 2, nthreads=nt)
+            out["cpu_baseline_allcores"] = cb[nt]
     eng.close()
     print(json.dumps(out))
 

@@ -156,7 +156,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
         if (q.mode == BH_MODE_STRICT) {
             bh::walk_strict(ctx, q);
         } else if (q.learned) {
-            if (!ctx->mlp.ready && q.learned != 2)
+            if (!ctx->mlp.ready && q.learned == 1)
                 throw ApiFail(BH_ERR_STATE, "learned correction requested but no model is loaded");
             bh::walk_fast(ctx, q, 1, 1);
             bh::mlp_correct(ctx, q);

@@ -750,6 +750,140 @@ __global__ __launch_bounds__(64) void k_exact_corr(const int *__restrict__ lrec,
     }
 }
 
+// Residual field of cell c near point t, FP64: the residual (a_exact -
+// a_multipole) at t and its Jacobian J[i][k] = d res_i / d t_k (row major, 9).
+// The exact part is differentiated analytically (m (3 d d^T / r^5 - I / r^3)
+// per body, d = body - t, softened r); the multipole part by central
+// differences of the multipole formula with step 1e-4 r (FP64: the
+// truncation error is ~1e-8 relative).  Valid in lane 0.
+__device__ void residual_field(int c, double tx, double ty, double tz, const int2 *__restrict__ fnb,
+                               const double *__restrict__ mo, const double4 *__restrict__ bd, double e2, int usequad,
+                               double *res, double *J)
+{
+    const int l = lane_id();
+    const int2 fn = fnb[c];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    double j[6] = {0, 0, 0, 0, 0, 0};   // xx xy xz yy yz zz
+    for (int b = l; b < fn.y; b += 64) {
+        const double4 q = bd[fn.x + b];
+        const double dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        const double ri = 1.0 / sqrt(r2);
+        const double w = q.w * ri * ri * ri;
+        const double w5 = 3.0 * w * ri * ri;
+        ex += dx * w;
+        ey += dy * w;
+        ez += dz * w;
+        j[0] += w5 * dx * dx - w;
+        j[1] += w5 * dx * dy;
+        j[2] += w5 * dx * dz;
+        j[3] += w5 * dy * dy - w;
+        j[4] += w5 * dy * dz;
+        j[5] += w5 * dz * dz - w;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        ex += __shfl_xor(ex, d, 64);
+        ey += __shfl_xor(ey, d, 64);
+        ez += __shfl_xor(ez, d, 64);
+#pragma unroll
+        for (int k = 0; k < 6; k++) j[k] += __shfl_xor(j[k], d, 64);
+    }
+    const double *m = mo + 10 * (int64_t)c;
+    auto multipole = [&](double px, double py, double pz, double *a) {
+        const double dx = m[1] - px, dy = m[2] - py, dz = m[3] - pz;
+        const double r2 = dx * dx + dy * dy + dz * dz + e2;
+        const double ri = 1.0 / sqrt(r2), ri2 = ri * ri;
+        const double mr3 = m[0] * ri * ri2;
+        a[0] = dx * mr3;
+        a[1] = dy * mr3;
+        a[2] = dz * mr3;
+        if (usequad) {
+            const double *Q = m + 4;
+            const double qx = Q[0] * dx + Q[1] * dy + Q[2] * dz;
+            const double qy = Q[1] * dx + Q[3] * dy + Q[4] * dz;
+            const double qz = Q[2] * dx + Q[4] * dy + Q[5] * dz;
+            const double r5 = ri2 * ri2 * ri;
+            const double phq = -0.5 * r5 * (dx * qx + dy * qy + dz * qz);
+            const double cf = 5.0 * phq * ri2;
+            a[0] -= dx * cf + qx * r5;
+            a[1] -= dy * cf + qy * r5;
+            a[2] -= dz * cf + qz * r5;
+        }
+    };
+    double a0[3];
+    multipole(tx, ty, tz, a0);
+    res[0] = ex - a0[0];
+    res[1] = ey - a0[1];
+    res[2] = ez - a0[2];
+    const double r = sqrt((m[1] - tx) * (m[1] - tx) + (m[2] - ty) * (m[2] - ty) + (m[3] - tz) * (m[3] - tz) + e2);
+    const double h = 1e-4 * r;
+    const double je[9] = {j[0], j[1], j[2], j[1], j[3], j[4], j[2], j[4], j[5]};
+    for (int k = 0; k < 3; k++) {
+        double ap[3], am[3];
+        multipole(tx + (k == 0 ? h : 0.0), ty + (k == 1 ? h : 0.0), tz + (k == 2 ? h : 0.0), ap);
+        multipole(tx - (k == 0 ? h : 0.0), ty - (k == 1 ? h : 0.0), tz - (k == 2 ? h : 0.0), am);
+        for (int i = 0; i < 3; i++) J[3 * i + k] = je[3 * i + k] - (ap[i] - am[i]) / (2.0 * h);
+    }
+}
+
+// Diagnostic: the first-order correction a perfect model of the residual
+// field would give: per group the sums of the exact residuals and of their
+// Jacobians at the group centre over the recorded cells (12 floats: R, J row
+// major), and the centre.
+__global__ __launch_bounds__(64) void k_exact_corr1(const int *__restrict__ lrec, int lcap,
+                                                    const int2 *__restrict__ fnb, const double *__restrict__ mo,
+                                                    const double4 *__restrict__ bd, double eps, int usequad,
+                                                    float4 *corr)
+{
+    const int *reg = lrec + (int64_t)blockIdx.x * lcap;
+    const int used = reg[0];
+    for (int p = 1; p < used;) {
+        const int g = reg[p], n = reg[p + 1];
+        const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
+        double s[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < n; k++) {
+            double r[3], J[9];
+            residual_field(reg[p + 5 + k], cx, cy, cz, fnb, mo, bd, eps * eps, usequad, r, J);
+            for (int t = 0; t < 3; t++) s[t] += r[t];
+            for (int t = 0; t < 9; t++) s[3 + t] += J[t];
+        }
+        if (lane_id() == 0) {
+            float4 *o = corr + 4 * (int64_t)g;
+            o[0] = make_float4((float)s[0], (float)s[1], (float)s[2], cx);
+            o[1] = make_float4((float)s[3], (float)s[4], (float)s[5], cy);
+            o[2] = make_float4((float)s[6], (float)s[7], (float)s[8], cz);
+            o[3] = make_float4((float)s[9], (float)s[10], (float)s[11], 0.f);
+        }
+        p += 5 + n;
+    }
+}
+
+// Adds R + J (x - centre) per member (first-order correction).
+__global__ void k_apply_corr1(const int *__restrict__ gfirst, const int *__restrict__ gcount,
+                              const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
+                              const float4 *__restrict__ thi, const float4 *__restrict__ tlo,
+                              const float4 *__restrict__ corr, const double4 *__restrict__ sacc, double *phi,
+                              double *acc)
+{
+    const int ng = (int)stat[S_NGROUPS];
+    for (int g = blockIdx.x; g < ng; g += gridDim.x) {
+        const float4 c0 = corr[4 * (int64_t)g], c1 = corr[4 * (int64_t)g + 1], c2 = corr[4 * (int64_t)g + 2],
+                     c3 = corr[4 * (int64_t)g + 3];
+        const float J[9] = {c1.x, c1.y, c1.z, c2.x, c2.y, c2.z, c3.x, c3.y, c3.z};
+        const int f = gfirst[g], n = gcount[g];
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const float4 h = thi[f + k], lo = tlo[f + k];
+            const float dx = (h.x - c0.w) + lo.x, dy = (h.y - c1.w) + lo.y, dz = (h.z - c2.w) + lo.z;
+            const int64_t o = tperm[f + k];
+            const double4 r = sacc[f + k];
+            acc[3 * o] = r.x + (double)(c0.x + J[0] * dx + J[1] * dy + J[2] * dz);
+            acc[3 * o + 1] = r.y + (double)(c0.y + J[3] * dx + J[4] * dy + J[5] * dz);
+            acc[3 * o + 2] = r.z + (double)(c0.z + J[6] * dx + J[7] * dy + J[8] * dz);
+            phi[o] = r.w;
+        }
+    }
+}
+
 // Training samples: for every recorded (group centre, cell), the features and
 // the exact residual (FP64 softened direct sum over the cell's bodies minus
 // the FP64 multipole force of the walk's formula) in units of M / r^2.
@@ -867,6 +1001,7 @@ void mlp_set(bh_ctx *ctx, const float *host_params, const float *norm)
 void mlp_correct(bh_ctx *ctx, const bh_params &p)
 {
     const bool exact = p.learned == 2;   // diagnostic: exact residual at the group centre
+    const bool exact1 = p.learned == 3;  // diagnostic: exact residual and Jacobian, first order per member
     Mlp &M = ctx->mlp;
     const Tree &T = ctx->tree;
     if (T.nb == 0) return;   // the walk wrote zero forces directly
@@ -874,7 +1009,15 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
     const int64_t ngcap = ctx->ntest;
     unsigned long long *stat = ptr<unsigned long long>(ctx->counters);
     const int gridg = (int)std::min<int64_t>(ngcap, (int64_t)ctx->ncu * 32);
-    if (exact) {
+    if (exact1) {
+        float4 *corr = ensure<float4>(M.corr, 4 * ngcap);
+        k_zero_f4<<<(int)((4 * ngcap + 255) / 256), 256, 0, st>>>(corr, (int)(4 * ngcap));
+        k_exact_corr1<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.fnb, T.mo, T.bd, p.eps,
+                                                  p.usequad, corr);
+        k_apply_corr1<<<gridg, 64, 0, st>>>(ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), stat, T.tperm, T.thi, T.tlo,
+                                            corr, ptr<double4>(ctx->sacc), ptr<double>(ctx->phi),
+                                            ptr<double>(ctx->acc));
+    } else if (exact) {
         float4 *corr = ensure<float4>(M.corr, ngcap);
         k_zero_f4<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(corr, (int)ngcap);
         k_exact_corr<<<ctx->lnwave, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, T.fnb, T.mo, T.bd, p.eps,

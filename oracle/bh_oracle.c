@@ -755,6 +755,37 @@ int orc_force_sample_mt(int n, const double *mass, const double *pos, int ns, co
     return 0;
 }
 
+/* Walk the bodies idx[0..ns) of the mass set over a built tree (each skips
+ * itself) on nthreads host threads; seconds of the walk in *t_walk.  Lets a
+ * caller try several tol on one tree. */
+int orc_tree_walk_idx(const orc_tree *t, const double *pos, int ns, const int64_t *idx, double tol, double eps,
+                      int nthreads, double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, double *t_walk)
+{
+    enum { MAXT = 256 };
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > MAXT) nthreads = MAXT;
+    orc_slice sl[MAXT];
+    pthread_t th[MAXT];
+    int started[MAXT] = {0};
+    double t1 = now_s();
+    for (int j = 0; j < nthreads; j++) {
+        sl[j] = (orc_slice){t, pos, idx, (int)((int64_t)ns * j / nthreads), (int)((int64_t)ns * (j + 1) / nthreads),
+                            tol, eps, phi, acc, 0, 0};
+        if (j > 0) started[j] = pthread_create(&th[j], NULL, orc_walk_slice, &sl[j]) == 0;
+    }
+    for (int j = 0; j < nthreads; j++)
+        if (!started[j]) orc_walk_slice(&sl[j]);
+    for (int j = 1; j < nthreads; j++)
+        if (started[j]) pthread_join(th[j], NULL);
+    *t_walk = now_s() - t1;
+    *n2btot = *nbctot = 0;
+    for (int j = 0; j < nthreads; j++) {
+        *n2btot += sl[j].n2b;
+        *nbctot += sl[j].nbc;
+    }
+    return 0;
+}
+
 int orc_force_sample(int n, const double *mass, const double *pos, int ns, const int64_t *idx,
                      double tol, double eps, int usequad, double *rmin, double *rsize, double fcells,
                      double *phi, double *acc, int64_t *n2btot, int64_t *nbctot, double *t_tree, double *t_walk)

@@ -28,7 +28,9 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # NEMO_ORACLE_LIB: another build of the same sources (the sanitizer
+        # build of oracle/san.mk)
+        path = os.environ.get("NEMO_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
@@ -59,6 +61,9 @@ def lib():
                                             C.c_int]
         L.orc_tree_build_capped.restype = C.c_int
         L.orc_tree_nbucket.argtypes = [C.c_void_p]
+        L.orc_tree_walk_idx.argtypes = [C.c_void_p, _d, C.c_int, _i64, C.c_double, C.c_double, C.c_int, _d, _d,
+                                        _i64, _i64, _d]
+        L.orc_tree_walk_idx.restype = C.c_int
         L.orc_tree_free.argtypes = [C.c_void_p]
         L.orc_tree_ncell.argtypes = [C.c_void_p]
         L.orc_tree_dump.argtypes = [C.c_void_p, _d, _d, _d, _i32, _i32]
@@ -135,6 +140,48 @@ def force_sample(mass, pos, idx, tol, eps, usequad, rsize=4.0, fcells=2.0, nthre
     if err:
         raise RuntimeError("oracle tree error %d" % err)
     return phi, acc, int(a[0]), int(b[0]), float(tt[0]), float(tw[0])
+
+
+class Tree:
+    """One reference tree (hackforce's box: rmin = -rsize/2) kept for several
+    walks of a subset of its bodies, e.g. at different tol."""
+
+    def __init__(self, mass, pos, usequad, rsize=4.0, fcells=2.0):
+        self.mass = np.ascontiguousarray(mass, np.float64)
+        self.pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        n = len(self.mass)
+        self.rmin = np.full(3, -rsize / 2.0)
+        rs = np.array([rsize])
+        self._h = C.c_void_p()
+        import time
+        t0 = time.perf_counter()
+        err = lib().orc_tree_build(C.byref(self._h), n, _p(self.mass), _p(self.pos), _p(self.rmin), _p(rs),
+                                   int(fcells * n), int(bool(usequad)))
+        self.t_build = time.perf_counter() - t0
+        self.rsize = float(rs[0])
+        if err:
+            self.close()
+            raise RuntimeError("oracle tree error %d" % err)
+
+    def walk(self, idx, tol, eps, nthreads=1):
+        """Field at the bodies idx (self skipped): (phi, acc, n2b, nbc, t_walk_s)."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        phi = np.zeros(len(idx))
+        acc = np.zeros((len(idx), 3))
+        a = np.zeros(1, np.int64)
+        b = np.zeros(1, np.int64)
+        tw = np.zeros(1)
+        lib().orc_tree_walk_idx(self._h, _p(self.pos), len(idx), _p(idx, _i64), float(tol), float(eps),
+                                int(nthreads), _p(phi), _p(acc), _p(a, _i64), _p(b, _i64), _p(tw))
+        return phi, acc, int(a[0]), int(b[0]), float(tw[0])
+
+    def close(self):
+        if self._h:
+            lib().orc_tree_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 def direct(mass, pos, eps, idx=None):

@@ -15,7 +15,9 @@ import pytest
 from nemo_amd import snapshot
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "nemo_amd", "bin")
+# NEMO_BIN: another build of the same programs (the sanitizer build of
+# nemo_amd/san.mk)
+BIN = os.environ.get("NEMO_BIN") or os.path.join(ROOT, "nemo_amd", "bin")
 
 
 def run(prog, *args, cwd=None):
@@ -436,8 +438,10 @@ def test_csf_convert_and_coerced_reads(golden, tmp_path):
     # typed reads through the C codec
     exe = str(tmp_path / "codec_check")
     cli = os.path.join(ROOT, "nemo_amd", "cli")
-    subprocess.run(["gcc", "-O1", "-std=gnu11", "-I", cli, "-o", exe, os.path.join(ROOT, "tests", "helpers", "codec_check.c"),
-                    os.path.join(cli, "filestruct.c")], check=True)
+    san = os.environ.get("NEMO_SAN_CFLAGS", "").split()
+    subprocess.run(["gcc", "-O1", "-std=gnu11"] + san + ["-I", cli, "-o", exe,
+                    os.path.join(ROOT, "tests", "helpers", "codec_check.c"), os.path.join(cli, "filestruct.c")],
+                   check=True)
     out = subprocess.run([exe, f], capture_output=True, text=True, check=True).stdout.splitlines()
     res = {ln.split()[0]: ln for ln in out}
     assert res["rigid"].startswith("rigid -1") and "types f, d don't match" in res["rigid"]

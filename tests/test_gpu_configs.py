@@ -3,7 +3,8 @@
 C1: Plummer N = 10,000 in FP64 (tol 0.7, eps 0.05, quadrupoles), one force
 evaluation and 64 leapfrog steps at dt = 1/32: the strict walk reproduces the
 oracle's hackcode1 trajectory bit for bit and the energy drift is small.
-C2: Plummer N = 2^20, eps 0.01, tol in {0.5, 0.7, 1.0}, FP32 group walk:
+C2: Plummer N = 2^20, eps 0.01, tol in {0.5, 0.7, 1.0}, FP32 group walk,
+monopole (hackforce) and quadrupole (hackforce_qp, Makefile:89-91) cells:
 median and 99th-percentile |da|/|a| against the FP64 direct sum on a seeded
 8192-body subset."""
 import numpy as np
@@ -58,3 +59,23 @@ def test_c2_plummer1m_error_bands(eng, c2, tol, med_band):
     e = np.linalg.norm(acc[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
     assert np.median(e) < med_band and np.percentile(e, 99) < 1e-2 * (tol / 0.5) ** 2
     assert st.n2btot > 0 and st.nbctot > 0
+
+
+@pytest.mark.parametrize("tol, med_band", [(0.5, 5e-4), (0.7, 1.5e-3), (1.0, 5e-3)])
+def test_c2_plummer1m_quadrupole_error_bands(eng, c2, tol, med_band):
+    """hackforce_qp at the same sizes: quadrupole cells lower the median and
+    the p99 error of the monopole walk at every tol (measured at tol 1.0:
+    median 4.6e-3 -> 2.8e-3, p99 1.7e-2 -> 1.2e-2)."""
+    m, x, idx, dacc = c2
+    eng.upload(m, x)
+    errs = {}
+    for quad in (0, 1):
+        st = eng.eval(E.default_params(tol=tol, eps=0.01, usequad=quad))
+        _, acc = eng.download()
+        errs[quad] = np.linalg.norm(acc[idx] - dacc, axis=1) / np.linalg.norm(dacc, axis=1)
+        assert st.n2btot > 0 and st.nbctot > 0
+    e0, e1 = errs[0], errs[1]
+    print("tol %.1f median mono %.3e quad %.3e; p99 %.3e %.3e" % (
+        tol, np.median(e0), np.median(e1), np.percentile(e0, 99), np.percentile(e1, 99)), flush=True)
+    assert np.median(e1) < med_band and np.percentile(e1, 99) < 1e-2 * (tol / 0.5) ** 2
+    assert np.median(e1) < 0.85 * np.median(e0) and np.percentile(e1, 99) < np.percentile(e0, 99)

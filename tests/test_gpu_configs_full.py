@@ -1,0 +1,163 @@
+"""Configurations 3, 4 and 5 at their stated sizes.
+
+C3: exponential disk N = 2^21 in a Hernquist halo N = 2^22 (eps 0.02, tol 0.7,
+quadrupoles): first-evaluation |da|/|a| against the FP64 direct sum on a
+seeded 8192-body subset, then 32 hipGraph-captured leapfrog steps at
+dt = 1/64 (hackcode1's stepsystem, code.c:159-191) with bounded energy drift.
+C4: 512 Plummer systems, N log-uniform in [256, 65536], eps_i = 1/(4 N_i^1/3),
+tol 0.7, quadrupoles, group and leaf size from the autotuner: per-system
+|da|/|a| against each system's own FP64 direct sum, then 32 captured steps.
+C5: the learned correction at Plummer N = 2^22: a model trained in the test
+on the engine's own residual samples, the tol of the learned and the plain
+walk each bisected to median |da|/|a| = 1e-3 on a seeded 8192-body subset:
+the learned walk reaches it with p99 <= 1e-2 at a larger opening angle."""
+import numpy as np
+import pytest
+
+from nemo_amd import engine as E
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def test_c3_disk_halo_first_eval_and_32_graph_steps():
+    e = E.Engine(0)
+    try:
+        m, x, v = E.mkdiskhalo(1 << 21, 1 << 22, seed=123)
+        n = len(m)
+        print("C3 bodies", n, flush=True)
+        e.upload(m, x)
+        idx = np.sort(np.random.default_rng(123).permutation(n)[:8192])
+        _, dacc = e.direct(0.02, idx)
+        p = E.default_params(tol=0.7, eps=0.02, usequad=1, group_size=192)
+        st = e.eval(p)
+        _, acc = e.download()
+        err = _rel(acc[idx], dacc)
+        disk = idx < (1 << 21)
+        print("C3 first eval median %.3e p99 %.3e (disk %.3e, halo %.3e), %d interactions" % (
+            np.median(err), np.percentile(err, 99), np.median(err[disk]), np.median(err[~disk]),
+            st.n2btot + st.nbctot), flush=True)
+        assert np.median(err) < 2e-3 and np.percentile(err, 99) < 1e-2
+        assert np.median(err[disk]) < 3e-3 and np.median(err[~disk]) < 3e-3
+        lf = E.Leapfrog(e, m, x, v, 1.0 / 64.0, tol=0.7, eps=0.02, usequad=1, group_size=192)
+        d, n2b, nbc = lf.run(32, graph=True)
+        etot = d[:, 0]
+        drift = np.abs(etot - etot[0]).max() / abs(etot[0])
+        print("C3 32 steps: E0 %.9f E32 %.9f max drift %.3e" % (etot[0], etot[-1], drift), flush=True)
+        assert np.isfinite(d).all() and (n2b > 0).all() and (nbc > 0).all()
+        assert drift < 1e-4
+        # virial ratio of the generated equilibrium stays put
+        q = 2 * d[:, 1] / np.abs(d[:, 2])
+        assert np.abs(q - q[0]).max() < 0.01 and 0.8 < q[0] < 1.1
+    finally:
+        e.close()
+
+
+def _ensemble(nsys, nmin, nmax, seed):
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nsys))).astype(int)
+    ms, xs, vs = [], [], []
+    for s, k in enumerate(ns):
+        m, x, v = E.mkplummer(int(k), seed=100000 + 1000 * seed + s)
+        ms.append(m)
+        xs.append(x)
+        vs.append(v)
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    eps = 1.0 / (4.0 * ns ** (1.0 / 3.0))
+    return off, np.concatenate(ms), np.concatenate(xs), np.concatenate(vs), eps, np.full(nsys, 0.7)
+
+
+def test_c4_ensemble_512_systems_autotuned_eval_and_32_steps():
+    e = E.Engine(0)
+    try:
+        off, m, x, v, eps, tol = _ensemble(512, 256, 65536, seed=123)
+        ns = np.diff(off)
+        print("C4 systems", len(ns), "bodies", off[-1], "N range", ns.min(), ns.max(), flush=True)
+        assert ns.min() >= 256 and ns.max() <= 65536 and off[-1] > 4_000_000
+        e.batch_upload(off, m, x, eps, tol)
+        best, ms = e.autotune(E.default_params(usequad=1), reps=2)
+        print("C4 autotuner: group %d nleaf %d" % (best.group_size, best.nleaf), flush=True)
+        assert best.group_size in (64, 128, 192, 256) and len(ms) == 20 and np.isfinite(ms).all()
+        st = e.eval(E.default_params(usequad=1))          # the tuned defaults
+        phi, acc = e.download()
+        assert np.isfinite(phi).all() and np.isfinite(acc).all()
+        rng = np.random.default_rng(5)
+        meds, all_err = [], []
+        for s in range(len(ns)):
+            a, b = off[s], off[s + 1]
+            k = rng.choice(b - a, size=min(24, b - a), replace=False)
+            _, da = oracle.direct(m[a:b], x[a:b], eps[s], k)
+            err = _rel(acc[a + k], da)
+            meds.append(np.median(err))
+            all_err.append(err)
+        meds = np.array(meds)
+        all_err = np.concatenate(all_err)
+        print("C4 per-system median |da|/|a|: median %.3e max %.3e; all p99 %.3e; %d interactions" % (
+            np.median(meds), meds.max(), np.percentile(all_err, 99), st.n2btot + st.nbctot), flush=True)
+        assert meds.max() < 2e-3 and np.median(meds) < 1e-3 and np.percentile(all_err, 99) < 1e-2
+        lf = E.Leapfrog(e, m, x, v, 1.0 / 32.0, batch=(off, eps, tol), usequad=1)
+        d, n2b, nbc = lf.run(32, graph=True)
+        assert d.shape == (32, len(ns), 36) and np.isfinite(d).all()
+        etot = d[:, :, 0]
+        drift = np.abs(etot[-1] - etot[0]) / np.abs(etot[0])
+        print("C4 32 steps: relative energy drift per system median %.3e max %.3e" % (
+            np.median(drift), drift.max()), flush=True)
+        assert np.median(drift) < 1e-3 and drift.max() < 3e-2
+    finally:
+        e.close()
+
+
+def _calibrate(e, base, idx, dacc, target=1e-3, lo=0.3, hi=1.5, iters=8):
+    best = None
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        e.eval(E.default_params(**dict(base, tol=mid)))
+        _, acc = e.download()
+        err = _rel(acc[idx], dacc)
+        med = float(np.median(err))
+        if med <= target:
+            best, lo = (mid, med, float(np.percentile(err, 99))), mid
+        else:
+            hi = mid
+    return best
+
+
+def test_c5_learned_correction_plummer_4m():
+    e = E.Engine(0)
+    try:
+        n, eps = 1 << 22, 0.01
+        e.mlp_init(5)
+        e.mlp_clear_samples()
+        for s in (41, 43):
+            m, x, _ = E.mkplummer(n, seed=s, with_vel=False)
+            e.upload(m, x)
+            for tol in (0.8, 1.0, 1.2):
+                e.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=0), stride=2, rlo=0.45, rhi=1.3)
+        nsamp = e.mlp_nsamples()
+        loss = e.mlp_train(epochs=6, batch=8192, lr=1e-3)
+        e.mlp_clear_samples()
+        print("C5 training: %d samples, loss %.3f -> %.3f" % (nsamp, loss[0], loss[-1]), flush=True)
+        assert nsamp > 5_000_000 and loss[-1] < 0.5 * loss[0]
+        m, x, _ = E.mkplummer(n, seed=123, with_vel=False)
+        e.upload(m, x)
+        idx = np.sort(np.random.default_rng(123).permutation(n)[:8192])
+        _, dacc = e.direct(eps, idx)
+        plain = _calibrate(e, dict(eps=eps, usequad=0, group_size=192, nleaf=1), idx, dacc)
+        learned = _calibrate(e, dict(eps=eps, usequad=0, group_size=256, nleaf=1, learned=1), idx, dacc)
+        print("C5 plain tol %.4f median %.3e p99 %.3e; learned tol %.4f median %.3e p99 %.3e" % (
+            plain + learned), flush=True)
+        assert plain is not None and learned is not None
+        assert learned[1] <= 1e-3 and learned[2] <= 1e-2
+        assert learned[0] > plain[0] + 0.1
+        st = e.eval(E.default_params(tol=learned[0], eps=eps, usequad=0, group_size=256, nleaf=1, learned=1))
+        assert st.n_mlp > 1_000_000
+        # at the learned tol the plain walk misses the target
+        e.eval(E.default_params(tol=learned[0], eps=eps, usequad=0, group_size=256, nleaf=1))
+        _, a0 = e.download()
+        assert np.median(_rel(a0[idx], dacc)) > 1.2e-3
+    finally:
+        e.close()

@@ -45,26 +45,37 @@ def test_gradient_matches_oracle(eng):
         k += n
 
 
-def test_training_reduces_loss_and_error(eng):
+@pytest.fixture(scope="module")
+def trained():
+    """An engine of its own holding a model trained on samples from two
+    Plummer snapshots (every test that needs a trained model gets it here,
+    whichever tests run before it)."""
+    e = E.Engine(0)
+    e.mlp_init(seed=5)
+    e.mlp_clear_samples()
+    for seed in (31, 33):
+        m, x, _ = E.mkplummer(300000, seed=seed)
+        e.upload(m, x)
+        for tol in (0.8, 1.0, 1.2):
+            e.mlp_collect(E.default_params(tol=tol, eps=0.01, usequad=1), stride=1, rlo=0.25, rhi=1.3)
+    X, Y = e.mlp_samples()
+    loss = e.mlp_train(epochs=8, batch=8192, lr=1e-3)
+    e.mlp_clear_samples()
+    yield e, X, Y, loss
+    e.close()
+
+
+def test_training_reduces_loss_and_error(trained):
     """Samples from two Plummer snapshots, training on the device, and the
     trained model lowering the force error of the group walk at tol=1 on a
     third snapshot (the plain tree is the fallback outside the range)."""
-    eng.mlp_init(seed=5)
-    eng.mlp_clear_samples()
-    for seed in (31, 33):
-        m, x, _ = E.mkplummer(300000, seed=seed)
-        eng.upload(m, x)
-        for tol in (0.8, 1.0, 1.2):
-            eng.mlp_collect(E.default_params(tol=tol, eps=0.01, usequad=1), stride=1, rlo=0.25, rhi=1.3)
-    X, Y = eng.mlp_samples()
+    eng, X, Y, loss = trained
     assert len(X) > 1000000
     assert np.isfinite(X).all() and np.isfinite(Y).all()
     # the residual grows with size/distance (higher multipoles)
     sr = X[:, 3]
     mag = np.linalg.norm(Y, axis=1)
     assert np.median(mag[sr > 0.8]) > 2 * np.median(mag[sr < 0.45])
-    del X, Y
-    loss = eng.mlp_train(epochs=8, batch=8192, lr=1e-3)
     assert loss[-1] < 0.8 * loss[0], loss
     # evaluation on an independent snapshot
     m2, x2, _ = E.mkplummer(300000, seed=32)
@@ -92,10 +103,11 @@ def test_training_reduces_loss_and_error(eng):
     assert np.array_equal(a1, a2)
 
 
-def test_learned_correction_on_test_points(eng):
-    """The model trained by the previous test applied to a separate test
-    set: the groups are cut from the test points' own topology and the
-    correction still lowers the error against the direct sum."""
+def test_learned_correction_on_test_points(trained):
+    """The trained model applied to a separate test set: the groups are cut
+    from the test points' own topology and the correction still lowers the
+    error against the direct sum."""
+    eng = trained[0]
     m, x, _ = E.mkplummer(200000, seed=34)
     t = x + 1e-4          # as dense as the mass set, so the groups are alike
     eng.upload(m, x, tpos=t)
