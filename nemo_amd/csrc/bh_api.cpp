@@ -346,7 +346,7 @@ void bh_destroy(bh_ctx *ctx)
     bh::DevBuf *bufs[] = {&ctx->mass,   &ctx->pos,    &ctx->tpos,   &ctx->dbox,    &ctx->dboxi,  &ctx->keys0,
                           &ctx->keys1,  &ctx->vals0,  &ctx->vals1,  &ctx->hist,    &ctx->scan_tmp, &ctx->tkeys0,
                           &ctx->tkeys1, &ctx->tvals0, &ctx->tvals1, &ctx->nmassive, &ctx->bd,    &ctx->bpk,   &ctx->bhi,
-                          &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself,
+                          &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself, &ctx->tinv, &ctx->tmap,
                           &ctx->cpl,    &ctx->ccnt,   &ctx->cbase,  &ctx->cmin1,   &ctx->cmin2,  &ctx->cfirst,
                           &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->crc2,
                           &ctx->cskip,  &ctx->pre,    &ctx->cmo,

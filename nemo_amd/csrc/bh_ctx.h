@@ -49,7 +49,7 @@ struct bh_ctx {
     bh::DevBuf nmassive;        // int
     // sorted bodies of the tree and of the test set
     bh::DevBuf bd, bhi, blo, bpk;   // bpk: (x, y, z, m) records in input order
-    bh::DevBuf thi, tlo, td, tperm, tself;
+    bh::DevBuf thi, tlo, td, tperm, tself, tinv, tmap;
     // topology
     bh::DevBuf cpl, ccnt, cbase, cmin1, cmin2;
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cskip, pre, cowner, gowner;

@@ -54,7 +54,8 @@ struct Tree {
     const double4 *td = nullptr;
     const uint32_t *tperm = nullptr;   // sorted test index -> input index
     const int *tself = nullptr;        // selfmode 2: tree body of each test point or -1
-    int selfmode = 0;                  // 0 no self pairs, 1 test index == tree body, 2 tself[]
+    const int *tinv = nullptr;         // selfmode 2: test index of each tree body
+    int selfmode = 0;                  // 0 no self pairs, 1 test index == tree body, 2 tself[] / tinv[]
     const uint64_t *keys = nullptr;    // sorted Morton keys of the mass set
     int cap = 0;                       // cell capacity of the arrays
     const unsigned long long *ncellp = nullptr;   // device cell count

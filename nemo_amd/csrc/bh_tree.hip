@@ -605,10 +605,12 @@ __global__ void k_gather_tree(const uint32_t *__restrict__ perm, int nb, const d
     blo[s] = l;
 }
 
-// Test set in its sorted order.  selfidx: tree body of the same point when
-// the mass set is the test set (sorted index s < nb), else -1.
+// Test set in its sorted order.  tmap (input index -> tree body or -1, NULL
+// when no test point is a tree body) gives tself[s], the tree body of test
+// point s, and tinv, the test index of each tree body.
 __global__ void k_gather_test(const uint32_t *__restrict__ perm, int64_t nt, const double *__restrict__ pos,
-                              float4 *thi, float4 *tlo, double4 *td, int *tself, int nb, int self_test)
+                              float4 *thi, float4 *tlo, double4 *td, int *tself, const int *__restrict__ tmap,
+                              int *tinv)
 {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= nt) return;
@@ -623,7 +625,17 @@ __global__ void k_gather_test(const uint32_t *__restrict__ perm, int64_t nt, con
     thi[s] = h;
     tlo[s] = l;
     td[s] = make_double4(x, y, z, 0.0);
-    tself[s] = (self_test && s < nb) ? (int)s : -1;
+    const int b = tmap ? tmap[o] : -1;
+    tself[s] = b;
+    if (b >= 0) tinv[b] = (int)s;
+}
+
+// Input index -> tree body (sorted index among the massive bodies) or -1 for
+// a massless body (which sorts after every massive one).
+__global__ void k_tree_index(const uint32_t *__restrict__ vs, int64_t n, int nb, int *tmap)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s < n) tmap[vs[s]] = s < nb ? (int)s : -1;
 }
 
 // ---- topology ----------------------------------------------------------------
@@ -1573,6 +1585,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.td = td;
     T.tperm = tperm;
     T.tself = tself;
+    T.tinv = nullptr;
     T.selfmode = ctx->self_test ? 2 : 0;
     if (alias) {
         // the test set is the tree's body set in the same order
@@ -1582,13 +1595,16 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         T.tperm = vs;
         T.tself = nullptr;
         T.selfmode = 1;
-    } else if (ctx->self_test) {
-        BH_HIP(hipMemcpyAsync(tperm, vs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-        k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_pos, thi, tlo, td, tself, nb, 1);
     } else {
+        // the test set sorted by its own keys in the tree's box.  When it is
+        // the mass set with massless bodies among it, those bodies sort with
+        // their neighbours here (not after every massive body, as in the
+        // tree's order), so each group stays compact in space; a test point
+        // finds its tree body (self pair) through the inverse of the tree's
+        // sort, and each tree body its test index through tinv.
         uint64_t *tk0 = ensure<uint64_t>(ctx->tkeys0, nt), *tk1 = ensure<uint64_t>(ctx->tkeys1, nt);
         uint32_t *tv0 = ensure<uint32_t>(ctx->tvals0, nt), *tv1 = ensure<uint32_t>(ctx->tvals1, nt);
-        double *d_tpos = ptr<double>(ctx->tpos);
+        const double *d_tpos = ctx->self_test ? d_pos : ptr<double>(ctx->tpos);
         k_keys<<<(int)std::min<int64_t>(nblocks(nt), (int64_t)ctx->ncu * 16), kTpb, 0, st>>>(d_tpos, nullptr, nt, d_box,
                                                                                             tk0, tv0, nullptr);
         uint64_t *tks;
@@ -1596,7 +1612,14 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         radix_sort_pairs(ctx, tk0, tk1, tv0, tv1, nt, 0, 64, &tks, &tvs);
         tks_for_groups = tks;
         BH_HIP(hipMemcpyAsync(tperm, tvs, nt * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-        k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_tpos, thi, tlo, td, tself, nb, 0);
+        int *tmap = nullptr, *tinv = nullptr;
+        if (ctx->self_test) {
+            tmap = ensure<int>(ctx->tmap, nt);
+            tinv = ensure<int>(ctx->tinv, std::max(nb, 1));
+            k_tree_index<<<nblocks(nt), kTpb, 0, st>>>(vs, nt, nb, tmap);
+        }
+        k_gather_test<<<nblocks(nt), kTpb, 0, st>>>(tperm, nt, d_tpos, thi, tlo, td, tself, tmap, tinv);
+        T.tinv = tinv;
     }
 
     // -- topology over the cell capacity

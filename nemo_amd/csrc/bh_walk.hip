@@ -214,7 +214,7 @@ __device__ __forceinline__ void p2c_slots(int nq, Lane<NPL> &L, const float4 *sr
 
 struct WalkArgs {
     const float4 *thi, *tlo;
-    const int *tself;
+    const int *tself, *tinv;
     int selfmode;
     const uint32_t *tperm;
     int64_t ntest;
@@ -439,10 +439,11 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     if (isBody) {
                         const float4 bh = a.bhi[node], bl = a.blo[node];
                         P[np + mbcnt(m)] = make_float4((bh.x - cx) + bl.x, (bh.y - cy) + bl.y, (bh.z - cz) + bl.z, bh.w);
-                        const int64_t rel = (int64_t)node - gbase;
-                        if (rel >= 0 && rel < gcount &&
-                            (a.selfmode == 1 || (a.selfmode == 2 && a.tself[gbase + rel] == node)))
-                            sSelf[w][rel] = 1;
+                        // the group member that is this body, if any
+                        const int64_t rel = a.selfmode == 1   ? (int64_t)node - gbase
+                                            : a.selfmode == 2 ? (int64_t)a.tinv[node] - gbase
+                                                              : -1;
+                        if (rel >= 0 && rel < gcount) sSelf[w][rel] = 1;
                     }
                     np += __popcll(m);
                 }
@@ -565,7 +566,10 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                 // the self pair added m rsq(eps^2) unless eps = 0 masked it
                 // (a batch with an unsoftened system runs the masking variant
                 // for every system)
-                if (eps2 > 0.f) ph -= (double)(a.bhi[gbase + k].w * rsq(eps2));
+                if (eps2 > 0.f) {
+                    const int64_t tb = a.selfmode == 1 ? gbase + k : a.tself[gbase + k];
+                    ph -= (double)(a.bhi[tb].w * rsq(eps2));
+                }
             }
             const double ax = (double)L.axh[q] + (double)L.axl[q];
             const double ay = (double)L.ayh[q] + (double)L.ayl[q];
@@ -628,6 +632,7 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     a.thi = T.thi;
     a.tlo = T.tlo;
     a.tself = T.tself;
+    a.tinv = T.tinv;
     a.selfmode = T.selfmode;
     a.tperm = T.tperm;
     a.ntest = nt;

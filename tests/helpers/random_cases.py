@@ -44,7 +44,10 @@ def case(seed):
 def _bucket_case(seed):
     rng = np.random.default_rng(5000 + seed)
     n = int(rng.choice([300, 1500, 4097]))
-    k = max(1, n // 40)
+    # clusters of >= 300 bodies: a group of test points then never spans two
+    # clusters (its FP32 coordinates relative to the group centre resolve
+    # separations d inside a group of extent R only to ~2^-24 R / d)
+    k = max(1, n // 300)
     c = rng.uniform(-1.2, 1.2, (k, 3))
     # cluster scales around the 21-level cell side (4 / 2^21 = 1.9e-6)
     x = c[rng.integers(0, k, n)] + rng.normal(scale=10.0 ** rng.uniform(-6.7, -5.7), size=(n, 3))
