@@ -121,6 +121,16 @@ bh_status bh_eval(bh_ctx *ctx, const bh_params *p, bh_stats *st);
 bh_status bh_download(bh_ctx *ctx, double *phi, double *acc);
 /* Evaluate FP64 direct sum on the device for a subset of the test set. */
 bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, double *phi, double *acc);
+/* bh_direct without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code that must not call
+ * hipMalloc, e.g. around HIP graph capture or in a serving loop).  The
+ * per-call staging of bh_direct (target indices and FP64 results) lives in a
+ * fixed array of the library, one per device, held until the process ends;
+ * at most BH_DIRECT_NOALLOC_MAX targets (BH_ERR_ARG beyond).  Results equal
+ * bh_direct's bit for bit.  Every call on one device shares that array:
+ * calls that overlap on one device (from several threads or contexts) race. */
+#define BH_DIRECT_NOALLOC_MAX 32768
+bh_status bh_direct_noalloc(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, double *phi, double *acc);
 /* Synchronise and return the device time of the last bh_eval kernels. */
 bh_status bh_sync(bh_ctx *ctx);
 

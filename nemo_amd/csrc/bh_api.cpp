@@ -6,9 +6,49 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 namespace bh {
 void debug_keys(bh_ctx *ctx, uint64_t *d_out);
 }
+
+// Staging of bh_direct_noalloc: target indices (int64) then FP64 results
+// (phi, then acc[3]) for at most BH_DIRECT_NOALLOC_MAX targets, 40 bytes each
+// (1.25 MiB), one copy per device for the life of the process.
+static __device__ __align__(256) unsigned char scratch[(size_t)BH_DIRECT_NOALLOC_MAX * 40];
+
+namespace {
+
+// Device address of each device's copy of `scratch`, looked up once per
+// device (with that device current) by bh_create.  The host-side name of the
+// array is not a device address and is never passed to the device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+hipError_t scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    g_scratch[device] = p;
+    return hipSuccess;
+}
+
+void *scratch_of(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
+
+}  // namespace
 
 namespace {
 
@@ -328,6 +368,9 @@ bh_status bh_create(bh_ctx **out, int device)
         return BH_ERR_DEVICE;
     }
     for (auto &e : ctx->fork_ev) (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    // the device's copy of the bh_direct_noalloc staging array (failure is
+    // reported by that entry point; the others do not need it)
+    (void)scratch_setup(device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->ncu = prop.multiProcessorCount;
@@ -435,6 +478,30 @@ bh_status bh_direct(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, d
         TmpDev t_idx(nidx * sizeof(int64_t)), t_out(4 * nidx * sizeof(double));
         int64_t *d_idx = t_idx.as<int64_t>();
         double *d_out = t_out.as<double>();
+        BH_HIP(hipMemcpyAsync(d_idx, idx, nidx * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+        bh::direct_sum(ctx, eps, nidx, d_idx, d_out, d_out + nidx);
+        BH_HIP(hipMemcpyAsync(phi, d_out, nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(acc, d_out + nidx, 3 * nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+bh_status bh_direct_noalloc(bh_ctx *ctx, double eps, int64_t nidx, const int64_t *idx, double *phi, double *acc)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_direct_noalloc before bh_upload");
+        if (nidx < 1 || !idx || !phi || !acc) throw ApiFail(BH_ERR_ARG, "bad direct-sum arguments");
+        if (nidx > BH_DIRECT_NOALLOC_MAX)
+            throw ApiFail(BH_ERR_ARG, "bh_direct_noalloc takes at most " + std::to_string(BH_DIRECT_NOALLOC_MAX) +
+                                          " targets (got " + std::to_string(nidx) + "); use bh_direct");
+        if (!(eps >= 0.0)) throw ApiFail(BH_ERR_ARG, "eps must be >= 0");
+        for (int64_t i = 0; i < nidx; i++)
+            if (idx[i] < 0 || idx[i] >= ctx->nmass) throw ApiFail(BH_ERR_ARG, "direct-sum index out of range");
+        unsigned char *base = static_cast<unsigned char *>(scratch_of(ctx->device));
+        if (!base) throw ApiFail(BH_ERR_DEVICE, "no device address for the staging array of this device");
+        // [idx: nidx int64][phi: nidx][acc: 3 nidx], all written before they are read
+        int64_t *d_idx = reinterpret_cast<int64_t *>(base);
+        double *d_out = reinterpret_cast<double *>(base + (size_t)nidx * sizeof(int64_t));
         BH_HIP(hipMemcpyAsync(d_idx, idx, nidx * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
         bh::direct_sum(ctx, eps, nidx, d_idx, d_out, d_out + nidx);
         BH_HIP(hipMemcpyAsync(phi, d_out, nidx * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -19,6 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libnemo_bh.so")
 
 MODE_FAST, MODE_STRICT, MODE_DIRECT = 0, 1, 2
+DIRECT_NOALLOC_MAX = 32768     # BH_DIRECT_NOALLOC_MAX of include/bh.h
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_DEVICE", 3: "ERR_NOMEM", 4: "ERR_NONFINITE", 5: "ERR_CAPACITY",
           6: "ERR_STATE", 7: "ERR_INJECTED"}
@@ -69,6 +70,7 @@ SIGNATURES = {
     "bh_eval": (C.c_int, [_vp, C.POINTER(Params), C.POINTER(Stats)]),
     "bh_download": (C.c_int, [_vp, _d, _d]),
     "bh_direct": (C.c_int, [_vp, C.c_double, C.c_int64, _i64, _d, _d]),
+    "bh_direct_noalloc": (C.c_int, [_vp, C.c_double, C.c_int64, _i64, _d, _d]),
     "bh_sync": (C.c_int, [_vp]),
     "bh_debug_keys": (C.c_int, [_vp, _u64]),
     "bh_debug_cells": (C.c_int, [_vp, C.c_int64, _i32, _i32, _d, _d, _d, _i64]),
@@ -215,6 +217,15 @@ class Engine:
         phi = np.zeros(len(idx))
         acc = np.zeros((len(idx), 3))
         self._check(lib().bh_direct(self._h, eps, len(idx), _p(idx, _i64), _p(phi), _p(acc)))
+        return phi, acc
+
+    def direct_noalloc(self, eps, idx):
+        """bh_direct with its staging in the library's fixed device array
+        (no device allocation; at most DIRECT_NOALLOC_MAX targets)."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        phi = np.zeros(len(idx))
+        acc = np.zeros((len(idx), 3))
+        self._check(lib().bh_direct_noalloc(self._h, eps, len(idx), _p(idx, _i64), _p(phi), _p(acc)))
         return phi, acc
 
     def force(self, mass, pos, params: Params, tpos=None):
