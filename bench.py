@@ -13,8 +13,9 @@ Plain tree: monopole and quadrupole cells are both calibrated to the target
 median error and the faster is the plain figure.  Learned: for walks with
 quadrupole and with monopole cells, an MLP is trained on the device from
 residual samples of three other Plummer snapshots (seeds 41, 43 and 47, tol
-0.8/1.0/1.2, cells with size/r in [0.45, 1.3]; the residual is the exact
-cell force minus that walk's multipole force), then that learned walk is
+0.9/1.1/1.3, cells with size/r in [0.45, 1.5]; the targets are the exact
+cell force minus that walk's multipole force at the group centre and its
+Jacobian, applied to each member to first order), then that learned walk is
 calibrated to the same target on the seed-123 snapshot; the faster of the
 two is the learned figure (both are in `config.learned_variants`).
 `value` is the evals/s of the faster of the two (`config.value_is` names it;
@@ -25,7 +26,7 @@ Prints one JSON line.  `roofline` is for the dominant kernel, the walk: 20
 flop per interaction (P2P and P2C, counted by the kernel) over the walk
 kernel's own event-timed duration, against MI355X's FP32 vector peak (the
 walk issues no MFMA: its bound is labelled "valu", the vector ALU).
-`roofline_mlp` is the MLP inference kernel: 18,304 flop per evaluation
+`roofline_mlp` is the MLP inference kernel: 19,072 flop per evaluation
 against the dense BF16 MFMA peak.  `cpu_baseline` times the CPU restatement
 of hackforce (FP64, single thread) on a bounded sample of the same snapshot
 at the oracle's own tol for the same median error (bisected on the error
@@ -49,7 +50,7 @@ METRIC = "particle-force evals/s, N=4M, median |da|/|a|=1e-3: learned vs plain t
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector packed / dense MFMA figure)
 BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 FLOP_PER_INTERACTION = 20
-FLOP_PER_MLP = 2 * (12 * 64 + 2 * 64 * 64 + 64 * 3)   # 18,304
+FLOP_PER_MLP = 2 * (12 * 64 + 2 * 64 * 64 + 64 * 9)   # 19,072: residual and its Jacobian
 
 
 def rel_err(a, ref):
@@ -168,7 +169,7 @@ def cpu_baseline(m, x, idx_err, dacc, eps, target, budget_s, nthreads_list=(1,))
     return out
 
 
-def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.8, 1.0, 1.2), stride=2, epochs=10, rlo=0.45, rhi=1.3,
+def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.9, 1.1, 1.3), stride=2, epochs=10, rlo=0.45, rhi=1.5,
                   usequad=1):
     """Residual samples of other snapshots, then training on the device.
     The correction (and the samples) cover accepted cells with size/r in
@@ -419,7 +420,7 @@ def main():
     ap.add_argument("--learned-quad", type=int, default=-1,
                     help="learned walk cells: 1 quadrupole, 0 monopole, -1 both (each with its own model; the faster "
                          "is the learned figure)")
-    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -490,7 +491,7 @@ def main():
         tune = None
         if base.get("group_size", 0) == 0:
             # group and leaf size from the autotuner, timed at a nominal tol
-            nominal = fixed_tol or (0.93 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
+            nominal = fixed_tol or (1.1 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
             t0 = time.perf_counter()
             best, ms = eng.autotune(E.default_params(**dict(base, tol=nominal)), reps=2)
             # near-ties (within 2 %) go to the larger group: its conservative
@@ -501,7 +502,7 @@ def main():
             tune = {"nominal_tol": nominal, "candidates": [list(c) for c in E.AUTOTUNE_CANDIDATES],
                     "ms": [round(float(t), 3) for t in ms], "seconds": time.perf_counter() - t0}
         if fixed_tol is None:
-            tol, med, p99 = calibrate_tol(eng, E, base, idx, dacc, args.target)
+            tol, med, p99 = calibrate_tol(eng, E, base, idx, dacc, args.target, hi=1.8 if base.get("learned") else 1.4)
         else:
             tol = fixed_tol
             med, p99 = measure_err(eng, E, E.default_params(**dict(base, tol=tol)), idx, dacc)
@@ -589,7 +590,7 @@ def main():
         a_mlp = FLOP_PER_MLP * learned["n_mlp"] / (learned["t_mlp_kernel_ms"] * 1e-3) / 1e12
         out["roofline_mlp"] = {"bound": "mfma", "kernel": "k_learned_reg", "achieved": a_mlp,
                                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": a_mlp / BF16_PEAK_TFLOPS,
-                               "traffic": None, "flop_convention": "18,304 flop per MLP evaluation"}
+                               "traffic": None, "flop_convention": "19,072 flop per MLP evaluation (12-64-64-64-9)"}
     if not args.no_cpu:
         nt = host_threads()
         cb = cpu_baseline(m, x, idx, dacc, args.eps, args.target, args.cpu_budget,

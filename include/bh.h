@@ -204,24 +204,40 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
 #define BH_AUTOTUNE_NCAND 20
 bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms);
 
-/* Learned far-field correction (no reference counterpart).  A 12-64-64-64-3
+/* Learned far-field correction (no reference counterpart).  A 12-64-64-64-9
  * SiLU MLP on BF16 MFMA predicts, for a cell accepted by the group walk whose
  * size/distance lies in the trained range, the residual between the exact
- * force of the cell's bodies and its multipole force at the group centre;
- * the sum over a group's cells is added to every member (params.learned).
- * params: kMlpParams = 9347 floats (W1[64][12] b1 W2[64][64] b2 W3[64][64] b3
- * W4[3][64] b4); norm[4] = output scale, size/distance range lo, hi, 0. */
+ * force of the cell's bodies and its multipole force at the group centre,
+ * and that residual's Jacobian there; the group's sums R and J give each
+ * member at offset dx from the centre the correction R + J dx
+ * (params.learned).  params: kMlpParams = 9737 floats (W1[64][12] b1
+ * W2[64][64] b2 W3[64][64] b3 W4[9][64] b4; outputs: residual x y z, then
+ * the symmetric Jacobian xx xy xz yy yz zz); norm[4] = residual scale,
+ * size/distance range lo, hi, Jacobian scale. */
 bh_status bh_mlp_init(bh_ctx *ctx, uint64_t seed);
 bh_status bh_mlp_set(bh_ctx *ctx, const float *params, const float *norm);
 bh_status bh_mlp_get(bh_ctx *ctx, float *params, float *norm);
-/* Batch inference / mean-squared-error gradient on host buffers (tests). */
+/* Batch inference / mean-squared-error gradient on host buffers (tests);
+ * x: [n][12], y: [n][9]. */
 bh_status bh_mlp_forward(bh_ctx *ctx, int64_t n, const float *x, float *y);
 bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, float *grad, float *loss);
 /* Training samples from the uploaded snapshot: a group walk that records, for
  * every stride-th group, its accepted cells with size/distance in [rlo, rhi];
- * each gives (features, exact FP64 residual at the group centre). */
+ * each gives (features[12], targets[9]: the exact FP64 residual at the group
+ * centre in units of M/r^2 and its symmetric Jacobian in units of M/r^3). */
 bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo, double rhi, int64_t *nadded);
 bh_status bh_mlp_samples(bh_ctx *ctx, int64_t cap, float *x, float *y, int64_t *n);
+/* Tests: the cell (DFS pre-order index, as bh_debug_cells) and the group
+ * centre ([3]) of each sample added by the last bh_mlp_collect. */
+bh_status bh_debug_samples(bh_ctx *ctx, int64_t cap, int *cell, float *centre, int64_t *n);
+/* Tests: the entries of the last learned evaluation (cell, group, predicted
+ * field res[12] = residual xyz, Jacobian xx xy xz yy yz zz, 0 0 0, in force
+ * units) and each group's centre ([4], w unused). */
+bh_status bh_debug_learned(bh_ctx *ctx, int64_t cap, int *cell, int *group, float *res, int64_t *nent, int64_t gcap,
+                           float *centre, int64_t *ngroups);
+/* Tests: each cell's first body (index in the tree's sorted order, see
+ * bh_debug_perm) and body count. */
+bh_status bh_debug_cell_ranges(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ncell);
 bh_status bh_mlp_clear_samples(bh_ctx *ctx);
 /* Adam on the collected samples (mean squared error of the scaled residual). */
 bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss);

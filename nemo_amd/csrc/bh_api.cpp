@@ -3,6 +3,7 @@
 // point exits the process; failures come back as bh_status + bh_last_error.
 #include "bh_ctx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -261,6 +262,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     }
     ctx->last = s;
     ctx->last_mode = q.mode;
+    ctx->mlp.nentries = (q.mode == BH_MODE_FAST && q.learned == 1) ? std::min<int64_t>(s.n_mlp, ctx->mlp.ecap) : -1;
     if (st) *st = s;
 }
 
@@ -404,7 +406,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->mlp.adam_m, &ctx->mlp.adam_v, &ctx->mlp.grad, &ctx->mlp.loss, &ctx->mlp.sx,
                           &ctx->mlp.sy, &ctx->mlp.snew_x, &ctx->mlp.snew_y, &ctx->mlp.soff, &ctx->mlp.corr,
                           &ctx->mlp.ecell, &ctx->mlp.egrp, &ctx->mlp.gstart, &ctx->mlp.gcnt, &ctx->mlp.gcentre,
-                          &ctx->mlp.res};
+                          &ctx->mlp.res, &ctx->mlp.smeta};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : ctx->ev)
@@ -887,11 +889,11 @@ bh_status bh_mlp_forward(bh_ctx *ctx, int64_t n, const float *x, float *y)
     return guarded(ctx, [&] {
         if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
         if (n < 1 || !x || !y) throw ApiFail(BH_ERR_ARG, "bad arguments");
-        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * 3 * sizeof(float));
+        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * bh::kMlpOut * sizeof(float));
         float *dx = tx.as<float>(), *dy = ty.as<float>();
         BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
         bh::mlp_forward(ctx, dx, n, dy);
-        BH_HIP(hipMemcpyAsync(y, dy, n * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        BH_HIP(hipMemcpyAsync(y, dy, n * bh::kMlpOut * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         BH_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
@@ -901,10 +903,11 @@ bh_status bh_mlp_grad(bh_ctx *ctx, int64_t n, const float *x, const float *y, fl
     return guarded(ctx, [&] {
         if (!ctx->mlp.ready) throw ApiFail(BH_ERR_STATE, "no model");
         if (n < 1 || !x || !y || !grad) throw ApiFail(BH_ERR_ARG, "bad arguments");
-        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * 3 * sizeof(float)), tg((bh::kMlpParams + 1) * sizeof(float));
+        TmpDev tx(n * bh::kMlpIn * sizeof(float)), ty(n * bh::kMlpOut * sizeof(float)),
+            tg((bh::kMlpParams + 1) * sizeof(float));
         float *dx = tx.as<float>(), *dy = ty.as<float>(), *dg = tg.as<float>();
         BH_HIP(hipMemcpy(dx, x, n * bh::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
-        BH_HIP(hipMemcpy(dy, y, n * 3 * sizeof(float), hipMemcpyHostToDevice));
+        BH_HIP(hipMemcpy(dy, y, n * bh::kMlpOut * sizeof(float), hipMemcpyHostToDevice));
         bh::mlp_grad(ctx, dx, dy, n, dg, dg + bh::kMlpParams);
         BH_HIP(hipMemcpyAsync(grad, dg, bh::kMlpParams * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
         float l = 0;
@@ -920,6 +923,7 @@ bh_status bh_mlp_collect(bh_ctx *ctx, const bh_params *p, int stride, double rlo
         check_params(p);
         if (!ctx->uploaded) throw ApiFail(BH_ERR_STATE, "bh_mlp_collect before bh_upload");
         if (!(rlo >= 0.0 && rhi > rlo)) throw ApiFail(BH_ERR_ARG, "bad size/distance range");
+        ctx->mlp.nentries = -1;   // the tree (and its cell numbering) is rebuilt
         if (p->criterion != BH_CRIT_HACK)
             throw ApiFail(BH_ERR_ARG, "training samples are collected with the hackcode1 criterion");
         ctx->mlp.norm[1] = (float)rlo;
@@ -962,7 +966,61 @@ bh_status bh_mlp_samples(bh_ctx *ctx, int64_t cap, float *x, float *y, int64_t *
         if (n) *n = ns;
         const int64_t k = std::min(cap, ns);
         if (k > 0 && x) BH_HIP(hipMemcpy(x, ctx->mlp.sx.p, k * bh::kMlpIn * sizeof(float), hipMemcpyDeviceToHost));
-        if (k > 0 && y) BH_HIP(hipMemcpy(y, ctx->mlp.sy.p, k * 3 * sizeof(float), hipMemcpyDeviceToHost));
+        if (k > 0 && y) BH_HIP(hipMemcpy(y, ctx->mlp.sy.p, k * bh::kMlpOut * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_debug_samples(bh_ctx *ctx, int64_t cap, int *cell, float *centre, int64_t *n)
+{
+    return guarded(ctx, [&] {
+        const int64_t ns = ctx->mlp.nlast;
+        if (n) *n = ns;
+        const int64_t k = std::min(cap, ns);
+        if (k <= 0) return;
+        std::vector<float> m(4 * (size_t)k);
+        BH_HIP(hipMemcpy(m.data(), ctx->mlp.smeta.p, m.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < k; i++) {
+            if (cell) memcpy(cell + i, &m[4 * i + 3], sizeof(int));
+            if (centre)
+                for (int a = 0; a < 3; a++) centre[3 * i + a] = m[4 * i + a];
+        }
+    });
+}
+
+bh_status bh_debug_learned(bh_ctx *ctx, int64_t cap, int *cell, int *group, float *res, int64_t *nent, int64_t gcap,
+                           float *centre, int64_t *ngroups)
+{
+    return guarded(ctx, [&] {
+        bh::Mlp &M = ctx->mlp;
+        if (M.nentries < 0) throw ApiFail(BH_ERR_STATE, "the last evaluation was not a learned one");
+        const int64_t ne = M.nentries;
+        if (nent) *nent = ne;
+        if (ngroups) *ngroups = ctx->ngroups;
+        const int64_t k = std::min(cap, ne);
+        if (k > 0) {
+            if (cell) BH_HIP(hipMemcpy(cell, M.ecell.p, k * sizeof(int), hipMemcpyDeviceToHost));
+            if (group) BH_HIP(hipMemcpy(group, M.egrp.p, k * sizeof(int), hipMemcpyDeviceToHost));
+            if (res) BH_HIP(hipMemcpy(res, M.res.p, 12 * k * sizeof(float), hipMemcpyDeviceToHost));
+        }
+        const int64_t kg = std::min<int64_t>(gcap, ctx->ngroups);
+        if (kg > 0 && centre) BH_HIP(hipMemcpy(centre, M.gcentre.p, 4 * kg * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+bh_status bh_debug_cell_ranges(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ncell)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->box_valid) throw ApiFail(BH_ERR_STATE, "no tree built");
+        const int64_t nc = std::min<int64_t>(ctx->tree.ncell, ctx->tree.cap);
+        if (ncell) *ncell = nc;
+        const int64_t k = std::min(cap, nc);
+        if (k <= 0) return;
+        std::vector<int2> fn(k);
+        BH_HIP(hipMemcpy(fn.data(), ctx->tree.fnb, k * sizeof(int2), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < k; i++) {
+            if (first) first[i] = fn[i].x;
+            if (count) count[i] = fn[i].y;
+        }
     });
 }
 
@@ -980,17 +1038,35 @@ bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t se
         if (n < 64) throw ApiFail(BH_ERR_STATE, "too few training samples");
         if (epochs < 1 || batch < 64 || !(lr > 0.f)) throw ApiFail(BH_ERR_ARG, "bad training arguments");
         hipStream_t st = ctx->stream;
-        // output scale: rms of the targets on a sample of the set
+        // output scales of the residual and Jacobian blocks: the rms of the
+        // targets below the block's 99th percentile of |y| on a sample of the
+        // set (a few cells reach bodies close to the group centre, where the
+        // field is far from linear; those targets are also clipped when the
+        // batches are gathered, so they cannot dominate the loss)
         if (M.adam_step == 0) {
             const int64_t k = std::min<int64_t>(n, 1 << 20);
-            std::vector<float> h(3 * k);
-            BH_HIP(hipMemcpy(h.data(), M.sy.p, 3 * k * sizeof(float), hipMemcpyDeviceToHost));
-            double s2 = 0.0;
-            for (float v : h) s2 += (double)v * v;
-            M.norm[0] = (float)std::max(1e-12, std::sqrt(s2 / (3.0 * k)));
+            std::vector<float> h((size_t)bh::kMlpOut * k);
+            BH_HIP(hipMemcpy(h.data(), M.sy.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int blk = 0; blk < 2; blk++) {
+                std::vector<float> a;
+                a.reserve(h.size());
+                for (size_t i = 0; i < h.size(); i++)
+                    if (((int)(i % bh::kMlpOut) >= bh::kMlpOutR) == (blk == 1)) a.push_back(std::fabs(h[i]));
+                const size_t q = (size_t)(0.99 * (double)(a.size() - 1));
+                std::nth_element(a.begin(), a.begin() + q, a.end());
+                const float lim = a[q];
+                double s2 = 0.0;
+                size_t cnt = 0;
+                for (float v : a)
+                    if (v <= lim) {
+                        s2 += (double)v * v;
+                        cnt++;
+                    }
+                M.norm[blk == 0 ? 0 : 3] = (float)std::max(1e-12, std::sqrt(s2 / (double)std::max<size_t>(cnt, 1)));
+            }
         }
         float *bx = bh::ensure<float>(M.snew_x, (size_t)batch * bh::kMlpIn);
-        float *by = bh::ensure<float>(M.snew_y, (size_t)batch * 3);
+        float *by = bh::ensure<float>(M.snew_y, (size_t)batch * bh::kMlpOut);
         float *grad = bh::ensure<float>(M.grad, bh::kMlpParams);
         const int64_t nb = n / batch;
         float *lossb = bh::ensure<float>(M.loss, std::max<int64_t>(nb, 1));

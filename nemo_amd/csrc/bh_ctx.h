@@ -12,9 +12,12 @@ namespace bh {
 struct Mlp {
     DevBuf params, wb, bias, gpart, lpart, adam_m, adam_v, grad, loss;
     DevBuf sx, sy, snew_x, snew_y, soff, corr, ecell, egrp, gstart, gcnt, gcentre, res;
+    DevBuf smeta;               // per sample of the last collection: cell, group centre (tests)
     int64_t nsamples = 0;
+    int64_t nlast = 0;          // samples added by the last collection
+    int64_t nentries = -1;      // entries of the last evaluation if it was a learned one, else -1
     int64_t ecap = 0;           // capacity of the flattened correction entries
-    float norm[kMlpNorm] = {1.f, 0.25f, 1.6f, 0.f};
+    float norm[kMlpNorm] = {1.f, 0.25f, 1.6f, 1.f};
     bool ready = false;
     int adam_step = 0;
 };

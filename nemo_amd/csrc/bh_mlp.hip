@@ -1,7 +1,10 @@
-// Learned far-field correction: a 12 -> 64 -> 64 -> 64 -> 3 SiLU MLP on BF16
+// Learned far-field correction: a 12 -> 64 -> 64 -> 64 -> 9 SiLU MLP on BF16
 // MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation) that predicts the
 // residual between the exact force of a cell's bodies and the cell's
-// multipole (monopole + quadrupole, softened as in grav.c:42-73) force.
+// multipole (monopole + quadrupole, softened as in grav.c:42-73) force at a
+// point, and the residual's Jacobian there (symmetric: xx xy xz yy yz zz), so
+// that the correction of a group member at offset dx from the group centre is
+// R + J dx.
 //
 // One wavefront handles 64 samples (rows).  A layer is a 64 x out x in GEMM
 // cut into 16x16 output tiles, each the sum of in/32 MFMAs.  Operand layout
@@ -10,11 +13,12 @@
 //
 // Features of a (cell, point) pair, all O(1): d/r (3, d = com - point),
 // size/r (1), (com - cell centre)/size (3), Q/(M size^2) components xx, yy,
-// xy, xz, yz (5).  Target: (a_exact - a_multipole) * r^2 / M / yscale.
+// xy, xz, yz (5).  Targets: R = (a_exact - a_multipole) r^2 / M / yscale_R
+// and J = d R / d point r^3 / M / yscale_J.
 //
 // Parameters: FP32 master copy (W1[64][12] b1[64] W2[64][64] b2[64]
-// W3[64][64] b3[64] W4[3][64] b4[3]) and BF16 copies padded for the MFMA
-// operands (W1[64][32], W2[64][64], W3[64][64], W4[16][64]).
+// W3[64][64] b3[64] W4[9][64] b4[9]) and BF16 copies padded for the MFMA
+// operands (W1[64][32], W2[64][64], W3[64][64], W4[32][64]).
 #include "bh_ctx.h"
 #include "bh_mlp.h"
 
@@ -109,7 +113,7 @@ __device__ void layer_hidden(const bf16 *in, const bf16 *W, const float *b, bf16
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Output layer: y[64][3] (rows in lanes as D fragments) = h W4^T + b4.
+// Output layer: y[64][kMlpOut] (rows in lanes as D fragments) = h W4^T + b4.
 // Returns for lane l the 4 rows 4*(l/16)+e of tile ti in column l%16.
 __device__ void layer_out(const bf16 *h, const bf16 *W4, const float *b4, float y[4][4])
 {
@@ -120,7 +124,7 @@ __device__ void layer_out(const bf16 *h, const bf16 *W4, const float *b4, float 
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 32) acc = mfma(load_rows(h, 64, ti * 16, kk), load_rows(W4, 64, 0, kk), acc);
         const int col = l & 15;
-        const float bias = col < 3 ? b4[col] : 0.f;
+        const float bias = col < kMlpOut ? b4[col] : 0.f;
 #pragma unroll
         for (int e = 0; e < 4; e++) y[ti][e] = acc[e] + bias;
     }
@@ -185,7 +189,7 @@ __device__ __forceinline__ void load_regw(RegW &R, const bf16 *Wb)
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int e = 0; e < 8; e++) R.w4[s][e] = Wb[kW4b + r * 64 + kperm(s, q, e)];   // rows >= 3 are zero
+        for (int e = 0; e < 8; e++) R.w4[s][e] = Wb[kW4b + r * 64 + kperm(s, q, e)];   // rows >= kMlpOut are zero
 }
 
 // SiLU with the hardware reciprocal (one rounding step more than a division,
@@ -210,7 +214,8 @@ __device__ __forceinline__ f32x4 bias4(const float *sb, int off)
 }
 
 // Forward of 16 samples (lane l: sample l%16) from the layer-1 B operand xb
-// (inputs 8(l/16)..+7 of the sample); lanes l < 16 return outputs 0..2.
+// (inputs 8(l/16)..+7 of the sample); lane l returns outputs 4(l/16)..+3 of
+// its sample (outputs >= kMlpOut are zero).
 __device__ __forceinline__ f32x4 mlp_fwd_reg(const RegW &R, const float *sb, bf16x8 xb)
 {
     const int q = lane_id() >> 4;
@@ -225,8 +230,7 @@ __device__ __forceinline__ f32x4 mlp_fwd_reg(const RegW &R, const float *sb, bf1
 #pragma unroll
     for (int t = 0; t < 4; t++) h[t] = mfma(R.w3[t][1], hb[1], mfma(R.w3[t][0], hb[0], bias4(sb, 128 + 16 * t + 4 * q)));
     act_pack(h, hb);
-    f32x4 y = q == 0 ? bias4(sb, 192) : f32x4{0.f, 0.f, 0.f, 0.f};
-    return mfma(R.w4[1], hb[1], mfma(R.w4[0], hb[0], y));
+    return mfma(R.w4[1], hb[1], mfma(R.w4[0], hb[0], bias4(sb, 192 + 4 * q)));
 }
 
 // Layer-1 B operand of sub-tile sub from 64 staged rows of 16 bf16.
@@ -277,12 +281,12 @@ __global__ __launch_bounds__(64) void k_mlp_forward(const float *__restrict__ X,
 #pragma unroll
         for (int sub = 0; sub < 4; sub++) {
             const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
-            const int64_t row = r0 + 16 * sub + l;
-            if (l < 16 && row < n) {
-                Y[row * 3] = y[0];
-                Y[row * 3 + 1] = y[1];
-                Y[row * 3 + 2] = y[2];
-            }
+            const int64_t row = r0 + 16 * sub + (l & 15);
+            const int o = 4 * (l >> 4);
+            if (row < n)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (o + e < kMlpOut) Y[row * kMlpOut + o + e] = y[e];
         }
         wave_sync_lds();
     }
@@ -394,7 +398,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ X, co
     float y[4][4];
     mlp_forward_wave(a, Wb, bias, y);
     float *g = gpart + (int64_t)blockIdx.x * kMlpParams;
-    // dY = 2 (y - t) / batch on real rows/columns; kept as bf16 dZ4 [64][64] (cols 0..2)
+    // dY = 2 (y - t) / batch on real rows/columns; kept as bf16 dZ4 [64][64] (cols < kMlpOut)
     const int col = l & 15;
     float lsum = 0.f, b4s = 0.f;
     for (int i = l; i < 64 * 64; i += 64) sd[i] = (bf16)0.f;
@@ -404,8 +408,8 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ X, co
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int row = ti * 16 + 4 * (l >> 4) + e;
-            if (col < 3 && row < nrow) {
-                const float d = y[ti][e] - Yt[(r0 + row) * 3 + col];
+            if (col < kMlpOut && row < nrow) {
+                const float d = y[ti][e] - Yt[(r0 + row) * kMlpOut + col];
                 lsum += d * d;
                 const bf16 gd = (bf16)(2.f * d * inv_batch);
                 sd[row * 64 + col] = gd;
@@ -418,9 +422,9 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float *__restrict__ X, co
     if (l == 0) lpart[blockIdx.x] = lsum * inv_batch;
     b4s += __shfl_xor(b4s, 16, 64);
     b4s += __shfl_xor(b4s, 32, 64);
-    if (l < 3) g[kOffB4 + l] = b4s;
-    // layer 4 (W4 rows 3..31 are zero, dZ4 columns 3..63 are zero)
-    grad_w<1, 4>(sd, 64, sh[2], 64, g + kOffW4, 64, 3);
+    if (l < kMlpOut) g[kOffB4 + l] = b4s;
+    // layer 4 (W4 rows >= kMlpOut are zero, and so are dZ4's columns)
+    grad_w<1, 4>(sd, 64, sh[2], 64, g + kOffW4, 64, kMlpOut);
     float dH[4][4][4];
     grad_h<32>(sd, 64, Wb + kW4b, 64, dH);
     // layer 3
@@ -471,16 +475,19 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
 // ---- features and the learned correction of the walk ---------------------------
 
 // Features of cell c seen from point p (FP32, relative coordinates: rel = com -
-// p).  Returns scale = M / r^2 (the output unit) or 0 if r == 0.
+// p).  Returns scale = M / r^2 (the unit of the residual; the Jacobian's is
+// scale / r) or 0 if r == 0; *rinv = 1 / r.
 __device__ __forceinline__ float cell_features(float rx, float ry, float rz, float4 A, float4 B, float4 F0, float4 F1,
-                                               float *f)
+                                               float *f, float *rinv)
 {
     const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
     if (!(r2 > 0.f)) {
         for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+        *rinv = 0.f;
         return 0.f;
     }
     const float inv = rsqrtf(r2);
+    *rinv = inv;
     f[0] = rx * inv;
     f[1] = ry * inv;
     f[2] = rz * inv;
@@ -542,15 +549,16 @@ __global__ void k_learned_count(const uint32_t *total, int64_t ecap, unsigned lo
     if ((int64_t)*total > ecap) stat[S_FLAGS] |= (unsigned long long)F_ECAP_OVF;
 }
 
-// Predicted residual acceleration of every flattened entry: each wave keeps
-// the weights in registers and strides over 64-entry tiles (the entry count
-// is read on the device); features of a tile go through LDS once.
+// Predicted residual field of every flattened entry: res[3e..3e+2] = (R.xyz,
+// J.xx), (J.xy, J.xz, J.yy, J.yz), (J.zz, 0, 0, 0) in force units.  Each wave
+// keeps the weights in registers and strides over 64-entry tiles (the entry
+// count is read on the device); features of a tile go through LDS once.
 __global__ __launch_bounds__(64) void k_learned_reg(const int *__restrict__ ecell, const int *__restrict__ egrp,
                                                     const uint32_t *__restrict__ total, int64_t ecap,
                                                     const float4 *__restrict__ gcentre, const float4 *__restrict__ cA,
                                                     const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
                                                     const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
-                                                    const float *__restrict__ bias, float yscale,
+                                                    const float *__restrict__ bias, float yscale, float jscale,
                                                     float4 *__restrict__ res)
 {
     __shared__ __attribute__((aligned(16))) float sb[208];
@@ -563,36 +571,43 @@ __global__ __launch_bounds__(64) void k_learned_reg(const int *__restrict__ ecel
     for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < ne; t0 += (int64_t)gridDim.x * 64) {
         const int64_t e = t0 + l;
         float f[kMlpIn];
-        float scale = 0.f;
+        float scale = 0.f, rinv = 0.f;
         if (e < ne) {
             const int c = ecell[e];
             const float4 ctr = gcentre[egrp[e]];
             const float4 A = cA[c], B = cB[c];
             scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c], cF1[c],
-                                  f);
+                                  f, &rinv);
         } else {
 #pragma unroll
             for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
         }
         stage_row(sx, f);
         wave_sync_lds();
+        const int q = l >> 4;
 #pragma unroll
         for (int sub = 0; sub < 4; sub++) {
+            // lane l: outputs 4q..4q+3 of entry 16 sub + l % 16
             const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
             const float sc = __shfl(scale, 16 * sub + (l & 15), 64);
-            const int64_t er = t0 + 16 * sub + l;
-            if (l < 16 && er < ne) {
-                const float w = yscale * sc;
-                res[er] = make_float4(y[0] * w, y[1] * w, y[2] * w, 0.f);
+            const float ri = __shfl(rinv, 16 * sub + (l & 15), 64);
+            const int64_t er = t0 + 16 * sub + (l & 15);
+            if (q < 3 && er < ne) {
+                const float wr = yscale * sc, wj = jscale * sc * ri;
+                float4 o;
+                if (q == 0) o = make_float4(y[0] * wr, y[1] * wr, y[2] * wr, y[3] * wj);
+                else if (q == 1) o = make_float4(y[0] * wj, y[1] * wj, y[2] * wj, y[3] * wj);
+                else o = make_float4(y[0] * wj, 0.f, 0.f, 0.f);
+                res[3 * er + q] = o;
             }
         }
         wave_sync_lds();
     }
 }
 
-// Adds the correction c to the accelerations of group g's members.
-// The recording walk left each member's (a, phi) in sorted order in sacc;
-// the member's final values (plus the group's correction c) go to the
+// Adds the correction c (uniform per group) to the accelerations of group
+// g's members.  The recording walk left each member's (a, phi) in sorted
+// order in sacc; the member's final values (plus the correction) go to the
 // input-order arrays in one write.
 __device__ __forceinline__ void apply_group(int g, float4 c, const int *__restrict__ gfirst,
                                             const int *__restrict__ gcount, const uint32_t *__restrict__ tperm,
@@ -609,30 +624,60 @@ __device__ __forceinline__ void apply_group(int g, float4 c, const int *__restri
     }
 }
 
-// Per group (one wave, striding over the groups): sum of its entries'
-// residuals in a fixed order, added to the members' accelerations.
+// First-order correction of group g's members: R + J (x - centre), with the
+// member's position x from its FP32 hi/lo parts and J symmetric
+// (j = xx xy xz yy yz zz).
+__device__ __forceinline__ void apply_group1(int g, const float *R, const float *j, float4 ctr,
+                                             const int *__restrict__ gfirst, const int *__restrict__ gcount,
+                                             const uint32_t *__restrict__ tperm, const float4 *__restrict__ thi,
+                                             const float4 *__restrict__ tlo, const double4 *__restrict__ sacc,
+                                             double *phi, double *acc)
+{
+    const int f = gfirst[g], n = gcount[g];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const float4 h = thi[f + k], lo = tlo[f + k];
+        const float dx = (h.x - ctr.x) + lo.x, dy = (h.y - ctr.y) + lo.y, dz = (h.z - ctr.z) + lo.z;
+        const int64_t o = tperm[f + k];
+        const double4 r = sacc[f + k];
+        acc[3 * o] = r.x + (double)fmaf(j[0], dx, fmaf(j[1], dy, fmaf(j[2], dz, R[0])));
+        acc[3 * o + 1] = r.y + (double)fmaf(j[1], dx, fmaf(j[3], dy, fmaf(j[4], dz, R[1])));
+        acc[3 * o + 2] = r.z + (double)fmaf(j[2], dx, fmaf(j[4], dy, fmaf(j[5], dz, R[2])));
+        phi[o] = r.w;
+    }
+}
+
+// Per group (one wave, striding over the groups): sums of its entries'
+// residuals and Jacobians in a fixed order, then each member's first-order
+// correction added to its acceleration.
 __global__ __launch_bounds__(64) void k_group_apply(const int *__restrict__ gstart, const int *__restrict__ gcnt,
                                                     const unsigned long long *__restrict__ stat,
-                                                    const float4 *__restrict__ res, const int *__restrict__ gfirst,
-                                                    const int *__restrict__ gcount,
+                                                    const float4 *__restrict__ res, const float4 *__restrict__ gcentre,
+                                                    const int *__restrict__ gfirst, const int *__restrict__ gcount,
                                                     const uint32_t *__restrict__ tperm,
+                                                    const float4 *__restrict__ thi, const float4 *__restrict__ tlo,
                                                     const double4 *__restrict__ sacc, double *phi, double *acc)
 {
     const int ng = (int)stat[S_NGROUPS];
     const int l = lane_id();
     for (int g = blockIdx.x; g < ng; g += gridDim.x) {
         const int s = gstart[g], n = gcnt[g];
-        float ax = 0.f, ay = 0.f, az = 0.f;
+        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int k = l; k < n; k += 64) {
-            const float4 r = res[s + k];
-            ax += r.x;
-            ay += r.y;
-            az += r.z;
+            const float4 a = res[3 * (int64_t)(s + k)], b = res[3 * (int64_t)(s + k) + 1],
+                         c = res[3 * (int64_t)(s + k) + 2];
+            v[0] += a.x;
+            v[1] += a.y;
+            v[2] += a.z;
+            v[3] += a.w;
+            v[4] += b.x;
+            v[5] += b.y;
+            v[6] += b.z;
+            v[7] += b.w;
+            v[8] += c.x;
         }
-        ax = wave_sum(ax);
-        ay = wave_sum(ay);
-        az = wave_sum(az);
-        apply_group(g, make_float4(ax, ay, az, 0.f), gfirst, gcount, tperm, sacc, phi, acc);
+#pragma unroll
+        for (int t = 0; t < 9; t++) v[t] = wave_sum(v[t]);
+        apply_group1(g, v, v + 3, gcentre[g], gfirst, gcount, tperm, thi, tlo, sacc, phi, acc);
     }
 }
 
@@ -886,7 +931,8 @@ __global__ void k_apply_corr1(const int *__restrict__ gfirst, const int *__restr
 
 // Training samples: for every recorded (group centre, cell), the features and
 // the exact residual (FP64 softened direct sum over the cell's bodies minus
-// the FP64 multipole force of the walk's formula) in units of M / r^2.
+// the FP64 multipole force of the walk's formula) in units of M / r^2, and
+// its Jacobian in units of M / r^3.
 // Group g's samples go to goff[g] onwards, so the set is in group order
 // whichever wave walked which group (training is then reproducible).
 __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ goff,
@@ -895,7 +941,8 @@ __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, in
                                                 const float4 *__restrict__ cF0, const float4 *__restrict__ cF1,
                                                 const int2 *__restrict__ fnb, const double *__restrict__ mo,
                                                 const double4 *__restrict__ bd, double eps, int usequad,
-                                                float *__restrict__ X, float *__restrict__ Y)
+                                                float *__restrict__ X, float *__restrict__ Y,
+                                                float4 *__restrict__ meta)
 {
     const int l = lane_id();
     const int *reg = lrec + (int64_t)blockIdx.x * lcap;
@@ -911,18 +958,29 @@ __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, in
         const float cx = __int_as_float(reg[p + 2]), cy = __int_as_float(reg[p + 3]), cz = __int_as_float(reg[p + 4]);
         for (int k = 0; k < n; k++, j++) {
             const int c = reg[p + 5 + k];
-            double r[3];
-            exact_residual(c, cx, cy, cz, fnb, mo, bd, e2, usequad, r);
+            double r[3], J[9];
+            residual_field(c, cx, cy, cz, fnb, mo, bd, e2, usequad, r, J);
             if (l == 0) {
-                float f[kMlpIn];
+                float f[kMlpIn], rinv;
                 const float4 A = cA[c], B = cB[c];
-                const float scale =
-                    cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c], cF1[c], f);
+                const float scale = cell_features((A.x - cx) + B.x, (A.y - cy) + B.y, (A.z - cz) + B.z, A, B, cF0[c],
+                                                  cF1[c], f, &rinv);
                 for (int t = 0; t < kMlpIn; t++) X[j * kMlpIn + t] = f[t];
+                meta[j] = make_float4(cx, cy, cz, __int_as_float(c));
                 const double inv = scale > 0.f ? 1.0 / (double)scale : 0.0;
-                Y[j * 3] = (float)(r[0] * inv);
-                Y[j * 3 + 1] = (float)(r[1] * inv);
-                Y[j * 3 + 2] = (float)(r[2] * inv);
+                const double invj = scale > 0.f ? inv / (double)rinv : 0.0;
+                float *y = Y + j * kMlpOut;
+                y[0] = (float)(r[0] * inv);
+                y[1] = (float)(r[1] * inv);
+                y[2] = (float)(r[2] * inv);
+                // the symmetric part of the Jacobian (the exact one is symmetric;
+                // the average removes the difference-quotient noise)
+                y[3] = (float)(J[0] * invj);
+                y[4] = (float)(0.5 * (J[1] + J[3]) * invj);
+                y[5] = (float)(0.5 * (J[2] + J[6]) * invj);
+                y[6] = (float)(J[4] * invj);
+                y[7] = (float)(0.5 * (J[5] + J[7]) * invj);
+                y[8] = (float)(J[8] * invj);
             }
         }
         p += 5 + n;
@@ -932,15 +990,18 @@ __global__ __launch_bounds__(64) void k_samples(const int *__restrict__ lrec, in
 }  // namespace
 
 // Training batch b of an epoch: rows (a*i + c) mod n (a coprime with n), targets
-// divided by yscale.
+// divided by their block's scale and clipped to +-kClip.
+constexpr float kClip = 10.f;
 __global__ void k_gather_batch(const float *__restrict__ X, const float *__restrict__ Y, int64_t n, uint64_t a,
-                               uint64_t c, int64_t first, int64_t cnt, float inv_yscale, float *bx, float *by)
+                               uint64_t c, int64_t first, int64_t cnt, float inv_yscale, float inv_jscale, float *bx,
+                               float *by)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= cnt) return;
     const uint64_t r = (uint64_t)((a * (uint64_t)(first + i) + c) % (uint64_t)n);
     for (int k = 0; k < kMlpIn; k++) bx[i * kMlpIn + k] = X[r * kMlpIn + k];
-    for (int k = 0; k < 3; k++) by[i * 3 + k] = Y[r * 3 + k] * inv_yscale;
+    for (int k = 0; k < kMlpOut; k++)
+        by[i * kMlpOut + k] = fminf(fmaxf(Y[r * kMlpOut + k] * (k < kMlpOutR ? inv_yscale : inv_jscale), -kClip), kClip);
 }
 
 // Converts the FP32 master parameters to the padded BF16 MFMA layout and the
@@ -959,7 +1020,7 @@ __global__ void k_mlp_pack(const float *__restrict__ P, bf16 *__restrict__ Wb, f
             v = P[kOffW3 + (t - kW3b)];
         } else {
             int o = (t - kW4b) / 64, i = (t - kW4b) % 64;
-            v = o < 3 ? P[kOffW4 + o * 64 + i] : 0.f;
+            v = o < kMlpOut ? P[kOffW4 + o * 64 + i] : 0.f;
         }
         Wb[t] = (bf16)v;
     }
@@ -968,7 +1029,7 @@ __global__ void k_mlp_pack(const float *__restrict__ P, bf16 *__restrict__ Wb, f
         bias[64 + t] = P[kOffB2 + t];
         bias[128 + t] = P[kOffB3 + t];
     }
-    if (t < 16) bias[192 + t] = t < 3 ? P[kOffB4 + t] : 0.f;
+    if (t < 16) bias[192 + t] = t < kMlpOut ? P[kOffB4 + t] : 0.f;
 }
 
 // ---- host side ---------------------------------------------------------------
@@ -1036,7 +1097,7 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         int *ecell = ensure<int>(M.ecell, ecap), *egrp = ensure<int>(M.egrp, ecap);
         int *gstart = ensure<int>(M.gstart, ngcap), *gcnt = ensure<int>(M.gcnt, ngcap);
         float4 *gctr = ensure<float4>(M.gcentre, ngcap);
-        float4 *res = ensure<float4>(M.res, ecap);
+        float4 *res = ensure<float4>(M.res, 3 * ecap);
         k_zero_i32n<<<(int)((ngcap + 255) / 256), 256, 0, st>>>(gcnt, (int)ngcap);
         k_flatten<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, off, ecap, ecell, egrp, gstart, gcnt, gctr);
         // the number of entries is only known on the device: launch over the
@@ -1046,11 +1107,11 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[8], st));
         k_learned_reg<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 8), 64, 0, st>>>(
             ecell, egrp, off + nreg, ecap, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias),
-            M.norm[0], res);
+            M.norm[0], M.norm[3], res);
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
-        k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount),
-                                            T.tperm, ptr<double4>(ctx->sacc), ptr<double>(ctx->phi),
-                                            ptr<double>(ctx->acc));
+        k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, gctr, ptr<int>(ctx->gfirst),
+                                            ptr<int>(ctx->gcount), T.tperm, T.thi, T.tlo, ptr<double4>(ctx->sacc),
+                                            ptr<double>(ctx->phi), ptr<double>(ctx->acc));
     }
     BH_HIP(hipGetLastError());
 }
@@ -1074,20 +1135,23 @@ int64_t mlp_collect(bh_ctx *ctx, const bh_params &p)
     BH_HIP(hipMemcpyAsync(&total, goff + ng, sizeof total, hipMemcpyDeviceToHost, st));
     BH_HIP(hipStreamSynchronize(st));
     if (total == 0) return 0;
-    float *nx = ensure<float>(M.snew_x, (size_t)total * kMlpIn), *ny = ensure<float>(M.snew_y, (size_t)total * 3);
+    float *nx = ensure<float>(M.snew_x, (size_t)total * kMlpIn), *ny = ensure<float>(M.snew_y, (size_t)total * kMlpOut);
+    float4 *meta = ensure<float4>(M.smeta, total);
     k_samples<<<nreg, 64, 0, st>>>(ptr<int>(ctx->lrec), (int)ctx->lcap, goff, ng, T.cA, T.cB, T.cF0, T.cF1, T.fnb, T.mo,
-                                   T.bd, p.eps, p.usequad, nx, ny);
+                                   T.bd, p.eps, p.usequad, nx, ny, meta);
+    M.nlast = total;
     BH_HIP(hipGetLastError());
     // append to the training set
     const int64_t old = M.nsamples, nn = old + total;
     DevBuf gx, gy;
-    float *X = ensure<float>(gx, (size_t)nn * kMlpIn), *Y = ensure<float>(gy, (size_t)nn * 3);
+    float *X = ensure<float>(gx, (size_t)nn * kMlpIn), *Y = ensure<float>(gy, (size_t)nn * kMlpOut);
     if (old) {
         BH_HIP(hipMemcpyAsync(X, M.sx.p, old * kMlpIn * sizeof(float), hipMemcpyDeviceToDevice, st));
-        BH_HIP(hipMemcpyAsync(Y, M.sy.p, old * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+        BH_HIP(hipMemcpyAsync(Y, M.sy.p, old * kMlpOut * sizeof(float), hipMemcpyDeviceToDevice, st));
     }
     BH_HIP(hipMemcpyAsync(X + old * kMlpIn, nx, (size_t)total * kMlpIn * sizeof(float), hipMemcpyDeviceToDevice, st));
-    BH_HIP(hipMemcpyAsync(Y + old * 3, ny, (size_t)total * 3 * sizeof(float), hipMemcpyDeviceToDevice, st));
+    BH_HIP(hipMemcpyAsync(Y + old * kMlpOut, ny, (size_t)total * kMlpOut * sizeof(float), hipMemcpyDeviceToDevice,
+                          st));
     BH_HIP(hipStreamSynchronize(st));
     if (M.sx.p) BH_HIP(hipFree(M.sx.p));
     if (M.sy.p) BH_HIP(hipFree(M.sy.p));
@@ -1124,7 +1188,8 @@ void mlp_gather(bh_ctx *ctx, uint64_t a, uint64_t c, int64_t first, int64_t cnt,
 {
     Mlp &M = ctx->mlp;
     k_gather_batch<<<(int)((cnt + 255) / 256), 256, 0, ctx->stream>>>(ptr<float>(M.sx), ptr<float>(M.sy), M.nsamples, a, c,
-                                                                    first, cnt, 1.f / M.norm[0], bx, by);
+                                                                    first, cnt, 1.f / M.norm[0], 1.f / M.norm[3], bx,
+                                                                    by);
     BH_HIP(hipGetLastError());
 }
 

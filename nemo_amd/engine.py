@@ -55,8 +55,9 @@ _u64 = C.POINTER(C.c_uint64)
 _i32 = C.POINTER(C.c_int)
 _vp = C.c_void_p
 _f = C.POINTER(C.c_float)
-MLP_PARAMS = 9347
+MLP_PARAMS = 9737
 MLP_IN = 12
+MLP_OUT = 9      # residual (3) and its symmetric Jacobian (xx xy xz yy yz zz)
 
 # every symbol include/bh.h declares, with its ctypes signature
 SIGNATURES = {
@@ -71,6 +72,9 @@ SIGNATURES = {
     "bh_download": (C.c_int, [_vp, _d, _d]),
     "bh_direct": (C.c_int, [_vp, C.c_double, C.c_int64, _i64, _d, _d]),
     "bh_direct_noalloc": (C.c_int, [_vp, C.c_double, C.c_int64, _i64, _d, _d]),
+    "bh_debug_samples": (C.c_int, [_vp, C.c_int64, _i32, _f, _i64]),
+    "bh_debug_learned": (C.c_int, [_vp, C.c_int64, _i32, _i32, _f, _i64, C.c_int64, _f, _i64]),
+    "bh_debug_cell_ranges": (C.c_int, [_vp, C.c_int64, _i32, _i32, _i64]),
     "bh_sync": (C.c_int, [_vp]),
     "bh_debug_keys": (C.c_int, [_vp, _u64]),
     "bh_debug_cells": (C.c_int, [_vp, C.c_int64, _i32, _i32, _d, _d, _d, _i64]),
@@ -314,7 +318,7 @@ class Engine:
 
     def mlp_forward(self, x):
         x = np.ascontiguousarray(x, np.float32)
-        y = np.zeros((len(x), 3), np.float32)
+        y = np.zeros((len(x), MLP_OUT), np.float32)
         self._check(lib().bh_mlp_forward(self._h, len(x), _p(x, _f), _p(y, _f)))
         return y
 
@@ -336,9 +340,43 @@ class Engine:
         self._check(lib().bh_mlp_samples(self._h, 0, None, None, _p(n, _i64)))
         k = int(n[0])
         x = np.zeros((k, MLP_IN), np.float32)
-        y = np.zeros((k, 3), np.float32)
+        y = np.zeros((k, MLP_OUT), np.float32)
         self._check(lib().bh_mlp_samples(self._h, k, _p(x, _f), _p(y, _f), _p(n, _i64)))
         return x, y
+
+    def debug_samples(self):
+        """Cell id and group centre of each sample of the last collection."""
+        n = np.zeros(1, np.int64)
+        self._check(lib().bh_debug_samples(self._h, 0, None, None, _p(n, _i64)))
+        k = int(n[0])
+        cell = np.zeros(k, np.int32)
+        ctr = np.zeros((k, 3), np.float32)
+        self._check(lib().bh_debug_samples(self._h, k, _p(cell, _i32), _p(ctr, _f), _p(n, _i64)))
+        return cell, ctr
+
+    def debug_learned(self):
+        """Entries of the last learned evaluation: (cell, group, res[n][12],
+        centre[ngroups][4])."""
+        n = np.zeros(1, np.int64)
+        ng = np.zeros(1, np.int64)
+        self._check(lib().bh_debug_learned(self._h, 0, None, None, None, _p(n, _i64), 0, None, _p(ng, _i64)))
+        k, g = int(n[0]), int(ng[0])
+        cell = np.zeros(k, np.int32)
+        grp = np.zeros(k, np.int32)
+        res = np.zeros((k, 12), np.float32)
+        ctr = np.zeros((g, 4), np.float32)
+        self._check(lib().bh_debug_learned(self._h, k, _p(cell, _i32), _p(grp, _i32), _p(res, _f), _p(n, _i64), g,
+                                           _p(ctr, _f), _p(ng, _i64)))
+        return cell, grp, res, ctr
+
+    def debug_cell_ranges(self):
+        n = np.zeros(1, np.int64)
+        self._check(lib().bh_debug_cell_ranges(self._h, 0, None, None, _p(n, _i64)))
+        k = int(n[0])
+        first = np.zeros(k, np.int32)
+        count = np.zeros(k, np.int32)
+        self._check(lib().bh_debug_cell_ranges(self._h, k, _p(first, _i32), _p(count, _i32), _p(n, _i64)))
+        return first, count
 
     def mlp_nsamples(self):
         n = np.zeros(1, np.int64)

@@ -2,17 +2,17 @@
 
 TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's checks).
 
-Restates the device network 12 -> 64 -> 64 -> 64 -> 3 with SiLU: weights and
+Restates the device network 12 -> 64 -> 64 -> 64 -> 9 with SiLU: weights and
 activations rounded to bfloat16 where the device rounds them (inputs,
 weights, hidden activations, pre-activations kept for the backward pass,
 back-propagated dZ), products accumulated in float64 here (the device uses
 FP32 accumulation; the difference is far below the bf16 rounding).
 Parameter layout: W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
-W4[3][64] b4[3].
+W4[9][64] b4[9] (outputs: the residual, then its symmetric Jacobian).
 """
 import numpy as np
 
-IN, HID, OUT = 12, 64, 3
+IN, HID, OUT = 12, 64, 9
 SIZES = [(HID, IN), (HID,), (HID, HID), (HID,), (HID, HID), (HID,), (OUT, HID), (OUT,)]
 NPARAMS = sum(int(np.prod(s)) for s in SIZES)
 

@@ -71,6 +71,12 @@ def lib():
         L.tc_force.argtypes = [C.c_int, _d, _d, C.c_int, _d, _d, C.c_double, C.c_double, C.c_double, C.c_int,
                                C.c_int, _d, _d, _i64, _i64, _i64, C.c_int, _i32, _d, _d, _d]
         L.tc_force.restype = C.c_int
+        L.lc_residual_field.argtypes = [C.c_int, _d, _d, C.c_double, _d, _d, C.c_double, _d, _d, _d]
+        L.lc_residual_field.restype = None
+        L.lc_features.argtypes = [_d, C.c_double, _d, _d, C.c_double, _d, _d, _d]
+        L.lc_features.restype = C.c_double
+        L.lc_apply.argtypes = [C.c_int, _d, _d, _d, _d, _d]
+        L.lc_apply.restype = None
         _LIB = L
     return _LIB
 
@@ -320,4 +326,45 @@ def tc_force(mass, pos, rmin, rsize, theta=1.0, eps=0.025, criterion="rcrit", us
     if cells:
         k = int(nc[0])
         out.update(level=lev[:k], mass=cm_[:k], com=ccm[:k], rcrit2=rc[:k])
+    return out
+
+
+# ---- learned correction data path (lc_oracle.c) -------------------------------
+
+def residual_field(m, x, M, com, Q, eps, t):
+    """Exact residual (softened pair sum over the cell's bodies m, x minus
+    gravsub's multipole force of mass M at com, quadrupole Q[3][3] or None)
+    at point t, and its Jacobian J[i][k] = d res_i / d t_k."""
+    m = np.ascontiguousarray(m, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    com = np.ascontiguousarray(com, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    q = None if Q is None else np.ascontiguousarray(Q, np.float64).reshape(9)
+    res = np.zeros(3)
+    J = np.zeros(9)
+    lib().lc_residual_field(len(m), _p(m), _p(x), float(M), _p(com), None if q is None else _p(q), float(eps), _p(t),
+                            _p(res), _p(J))
+    return res, J.reshape(3, 3)
+
+
+def features(com, M, Q, centre, size, t):
+    """MLP input features of a cell seen from t: (f[12], M/r^2, 1/r)."""
+    com = np.ascontiguousarray(com, np.float64)
+    q = np.ascontiguousarray(Q, np.float64).reshape(9)
+    cc = np.ascontiguousarray(centre, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    f = np.zeros(12)
+    ri = np.zeros(1)
+    scale = lib().lc_features(_p(com), float(M), _p(q), _p(cc), float(size), _p(t), _p(f), _p(ri))
+    return f, float(scale), float(ri[0])
+
+
+def apply_first_order(x, centre, R, j6):
+    """R + J (x - centre) for each point (J symmetric from xx xy xz yy yz zz)."""
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    c = np.ascontiguousarray(centre, np.float64)
+    R = np.ascontiguousarray(R, np.float64)
+    j6 = np.ascontiguousarray(j6, np.float64)
+    out = np.zeros_like(x)
+    lib().lc_apply(len(x), _p(x), _p(c), _p(R), _p(j6), _p(out))
     return out

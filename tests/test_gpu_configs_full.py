@@ -111,7 +111,7 @@ def test_c4_ensemble_512_systems_autotuned_eval_and_32_steps():
         e.close()
 
 
-def _calibrate(e, base, idx, dacc, target=1e-3, lo=0.3, hi=1.5, iters=8):
+def _calibrate(e, base, idx, dacc, target=1e-3, lo=0.3, hi=1.8, iters=8):
     best = None
     for _ in range(iters):
         mid = 0.5 * (lo + hi)
@@ -135,8 +135,8 @@ def test_c5_learned_correction_plummer_4m():
         for s in (41, 43):
             m, x, _ = E.mkplummer(n, seed=s, with_vel=False)
             e.upload(m, x)
-            for tol in (0.8, 1.0, 1.2):
-                e.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=0), stride=2, rlo=0.45, rhi=1.3)
+            for tol in (0.9, 1.1, 1.3):
+                e.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=0), stride=2, rlo=0.45, rhi=1.5)
         nsamp = e.mlp_nsamples()
         loss = e.mlp_train(epochs=6, batch=8192, lr=1e-3)
         e.mlp_clear_samples()
