@@ -80,6 +80,9 @@ typedef struct {
     int nleaf;           /* fast mode: a cell holding <= nleaf bodies interacts body by body (exact);
                             0 keeps the reference's cell interactions; -1 = tuned default */
     int criterion;       /* bh_criterion (default BH_CRIT_HACK); not with learned or batches */
+    int supergroup;      /* fast mode: K consecutive groups share one traversal (1 = off, 2..4;
+                            0 = tuned default); the interactions are those of K = 1.  Used by the
+                            plain single-system walk with the hackcode1 criterion */
 } bh_params;
 
 typedef struct {

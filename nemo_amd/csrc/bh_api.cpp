@@ -100,6 +100,7 @@ void check_params(const bh_params *p)
         for (int k = 0; k < 3; k++)
             if (!std::isfinite(p->rmin[k])) throw ApiFail(BH_ERR_ARG, "rmin must be finite");
     if (p->criterion < BH_CRIT_HACK || p->criterion > BH_CRIT_SW94) throw ApiFail(BH_ERR_ARG, "unknown criterion");
+    if (p->supergroup < 0 || p->supergroup > 4) throw ApiFail(BH_ERR_ARG, "supergroup must be 0 (tuned), 1 (off) or 2..4");
     if (p->criterion != BH_CRIT_HACK && p->learned)
         throw ApiFail(BH_ERR_ARG, "the learned correction is trained for the hackcode1 criterion only");
 }
@@ -179,6 +180,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     bh_params q = *p;
     if (q.group_size == 0) q.group_size = ctx->tuned_group;
     if (q.nleaf < 0) q.nleaf = ctx->tuned_nleaf;
+    if (q.supergroup == 0) q.supergroup = ctx->tuned_sg;
     if (q.mode == BH_MODE_DIRECT) throw ApiFail(BH_ERR_ARG, "BH_MODE_DIRECT is evaluated with bh_direct");
     if (ctx->nsys && (q.mode != BH_MODE_FAST || q.learned))
         throw ApiFail(BH_ERR_ARG, "batches are evaluated by the fast walk without the learned correction");
@@ -321,6 +323,7 @@ bh_params lf_params(const bh_ctx *ctx)
     bh_params q = ctx->lfp;
     if (q.group_size == 0) q.group_size = ctx->tuned_group;
     if (q.nleaf < 0) q.nleaf = ctx->tuned_nleaf;
+    if (q.supergroup == 0) q.supergroup = ctx->tuned_sg;
     q.keep_box = ctx->box_valid ? 1 : 0;
     return q;
 }
@@ -344,6 +347,7 @@ void bh_default_params(bh_params *p)
     p->keep_box = 0;
     p->nleaf = -1;
     p->criterion = BH_CRIT_HACK;
+    p->supergroup = 0;
 }
 
 const char *bh_version(void) { return "nemo_amd-bh 0.1 (gfx950)"; }
@@ -396,7 +400,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->crc2,
                           &ctx->cskip,  &ctx->pre,    &ctx->cmo,
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
-                          &ctx->gctr,   &ctx->werr,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
+                          &ctx->gctr,   &ctx->werr, &ctx->shlist, &ctx->shid,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,
                           &ctx->gm2,    &ctx->gcf,    &ctx->gcl,    &ctx->gclv,   &ctx->lhist,  &ctx->llist,  &ctx->bcand,  &ctx->bcount, &ctx->lkeys0, &ctx->lkeys1,
                           &ctx->lvals0, &ctx->lvals1, &ctx->cfnb, &ctx->cowner, &ctx->gowner,

@@ -65,6 +65,7 @@ struct bh_ctx {
     int ngroups = 0;          // host copy after an evaluation
     // walk scratch
     bh::DevBuf frontier, gctr, werr;
+    bh::DevBuf shlist, shid;    // super-group walk: shared lists per wave
     int64_t fcap = 0;
     // outputs (input order of the test set)
     bh::DevBuf phi, acc;
@@ -99,8 +100,9 @@ struct bh_ctx {
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays
     bool mlp_sampling = false;  // a sample-collection walk is running (the tree writes learned features)
-    int tuned_group = 128;      // defaults for group_size = 0 / nleaf = -1 (set by bh_autotune)
+    int tuned_group = 128;      // defaults for group_size = 0 / nleaf = -1 / supergroup = 0 (set by bh_autotune)
     int tuned_nleaf = 2;
+    int tuned_sg = 1;
     int tcell_cap = 0;          // cell capacity of a separate test-set topology
     int box_rounds = 6;         // expansion rounds from a fresh box
     int box_rounds_keep = 2;    // expansion rounds from the previous box

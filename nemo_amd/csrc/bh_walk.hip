@@ -250,6 +250,12 @@ struct WalkArgs {
     const int *sysroot;
     // treecode1 criteria: root box; cB.w holds each cell's Rcrit2
     const double4 *box;
+    // super-groups: K consecutive groups share one traversal; per-wave
+    // scratch for the shared lists (shcap entries each)
+    int K;
+    float4 *shlist;       // [wave][shcap P entries][shcap * S C entries]
+    int *shid;            // [wave][shcap body ids of the P entries][shcap deferred nodes]
+    int shcap;
 };
 
 // CRIT: treecode1's accept (treegrav.c:137-156) for the whole group.  With
@@ -596,6 +602,442 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
     }
 }
 
+
+// ---- super-groups: one traversal shared by K consecutive groups --------------
+//
+// A wave takes K consecutive groups (Morton-adjacent) and first walks the
+// tree for all of them at once, testing every node against each group's
+// bounding box (hackforce's rule, grav.c:122-131, applied per group as in
+// k_walk): a cell that every group would open is opened, a cell that every
+// group would accept goes to a shared P2C list, a body to a shared P2P list,
+// and a cell on which the groups disagree to a deferred list.  Then each
+// group in turn consumes the shared lists and walks on from the deferred
+// nodes alone.  Every group ends with exactly the interactions k_walk gives
+// it (the counters are equal); only the order of the sums differs.  The
+// shared lists live in a per-wave global scratch region, coordinates
+// relative to the super-group's centre.
+constexpr int kMaxK = 4;
+
+template <int NPL, bool QUAD, bool SOFT0>
+__global__ __launch_bounds__(kTpb) void k_walk_sg(WalkArgs a)
+{
+    constexpr int G = 64 * NPL;
+    constexpr int S = QUAD ? 3 : 1;
+    __shared__ float4 sP[kWaves][kListCap];
+    __shared__ float4 sC[kWaves][kListCap * S];
+    __shared__ unsigned char sSelf[kWaves][G];
+    __shared__ float4 sBox[kWaves][kMaxK][2];   // each group's bounding box (lo, hi), relative to the centre
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int64_t wid = (int64_t)blockIdx.x * kWaves + w;
+    float4 *P = sP[w];
+    float4 *Cl = sC[w];
+    int *frA = a.frontier + wid * 2 * a.fcap;
+    int *frB = frA + a.fcap;
+    float4 *shP = a.shlist + wid * (int64_t)a.shcap * (1 + S);
+    float4 *shC = shP + a.shcap;
+    int *shId = a.shid + wid * 2 * (int64_t)a.shcap;
+    int *shDef = shId + a.shcap;
+    const int K = a.K;
+    const float tolsq = a.tolsq, eps2 = a.eps2;
+    for (;;) {
+        int sg = 0;
+        if (lane == 0) sg = atomicAdd(a.gctr, 1);
+        sg = __shfl(sg, 0, 64);
+        const int ng = (int)a.stat[S_NGROUPS];
+        if ((int64_t)sg * K >= ng) break;
+        const int g0 = sg * K, gn = min(K, ng - g0);
+
+        // centre of the super-group (midpoint of the hi parts' extent)
+        float mnx = 3.0e38f, mny = 3.0e38f, mnz = 3.0e38f, mxx = -3.0e38f, mxy = -3.0e38f, mxz = -3.0e38f;
+        for (int i = 0; i < gn; i++) {
+            const int64_t gb = a.gfirst[g0 + i];
+            const int gc = a.gcount[g0 + i];
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                const int k = q * 64 + lane;
+                const float4 h = a.thi[gb + (k < gc ? k : 0)];
+                mnx = fminf(mnx, h.x); mny = fminf(mny, h.y); mnz = fminf(mnz, h.z);
+                mxx = fmaxf(mxx, h.x); mxy = fmaxf(mxy, h.y); mxz = fmaxf(mxz, h.z);
+            }
+        }
+        const float cx = 0.5f * (wave_min(mnx) + wave_max(mxx));
+        const float cy = 0.5f * (wave_min(mny) + wave_max(mxy));
+        const float cz = 0.5f * (wave_min(mnz) + wave_max(mxz));
+        // each group's bounding box relative to that centre (in LDS: the
+        // values are wave-uniform and would cost 6 VGPRs per group)
+        wave_lds_sync();
+        for (int i = 0; i < gn; i++) {
+            float lo0 = 3.0e38f, lo1 = 3.0e38f, lo2 = 3.0e38f, hi0 = -3.0e38f, hi1 = -3.0e38f, hi2 = -3.0e38f;
+            const int64_t gb = a.gfirst[g0 + i];
+            const int gc = a.gcount[g0 + i];
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                const int k = q * 64 + lane;
+                const int64_t t = gb + (k < gc ? k : 0);
+                const float4 h = a.thi[t], l = a.tlo[t];
+                const float px = (h.x - cx) + l.x, py = (h.y - cy) + l.y, pz = (h.z - cz) + l.z;
+                lo0 = fminf(lo0, px); lo1 = fminf(lo1, py); lo2 = fminf(lo2, pz);
+                hi0 = fmaxf(hi0, px); hi1 = fmaxf(hi1, py); hi2 = fmaxf(hi2, pz);
+            }
+            lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+            hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+            if (lane == 0) {
+                sBox[w][i][0] = make_float4(lo0, lo1, lo2, 0.f);
+                sBox[w][i][1] = make_float4(hi0, hi1, hi2, 0.f);
+            }
+        }
+        wave_lds_sync();
+
+        // ---- phase A: the shared traversal
+        int np = 0, nc = 0, nd = 0, nf = 0;
+        bool overflow = false;
+        if (*a.ncellp == 0) {
+            if (a.nb == 1 && lane == 0) frA[0] = 0;
+            nf = a.nb == 1 ? 1 : 0;
+        } else {
+            if (lane == 0) frA[0] = ~0;
+            nf = 1;
+        }
+        int *fr0 = frA, *fr1 = frB;
+        while (nf > 0 && !overflow) {
+            int nn = 0;
+            for (int base = 0; base < nf; base += 64) {
+                const int k = base + lane;
+                const bool v = k < nf;
+                const int node = v ? fr0[k] : 0;
+                const bool isBody = v && node >= 0;
+                const bool isCell = v && node < 0;
+                bool open = false, accept = false, defer = false;
+                int c = 0, nk = 0, bfirst = -1;
+                float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+                float rx = 0.f, ry = 0.f, rz = 0.f;
+                if (isCell) {
+                    c = ~node;
+                    A = a.cA[c];
+                    const float4 B = a.cB[c];
+                    rx = (A.x - cx) + B.x;
+                    ry = (A.y - cy) + B.y;
+                    rz = (A.z - cz) + B.z;
+                    const int2 fn = a.fnb[c];
+                    const int lv = a.level[c];
+                    const bool small = fn.y <= a.nleaf || lv == kMaxLevel;
+                    bool any_open = fn.y <= a.nleaf, all_open = true;
+                    for (int i = 0; i < gn; i++) {
+                        const float4 lo = sBox[w][i][0], hi = sBox[w][i][1];
+                        const float dx = fmaxf(fmaxf(lo.x - rx, rx - hi.x), 0.f);
+                        const float dy = fmaxf(fmaxf(lo.y - ry, ry - hi.y), 0.f);
+                        const float dz = fmaxf(fmaxf(lo.z - rz, rz - hi.z), 0.f);
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const bool o = tolsq * d2 < B.w || fn.y <= a.nleaf;
+                        any_open = any_open || o;
+                        all_open = all_open && o;
+                    }
+                    open = all_open;
+                    accept = !any_open;
+                    defer = any_open && !all_open;
+                    if (open) nk = small ? fn.y : a.nkids[c];
+                    bfirst = small ? fn.x : -1;
+                }
+                // bodies -> shared P2P list (with their ids for the self pairs)
+                {
+                    const unsigned long long m = __ballot(isBody);
+                    const int cnt = __popcll(m);
+                    if (np + cnt > a.shcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (isBody) {
+                        const float4 bh = a.bhi[node], bl = a.blo[node];
+                        const int at = np + mbcnt(m);
+                        shP[at] = make_float4((bh.x - cx) + bl.x, (bh.y - cy) + bl.y, (bh.z - cz) + bl.z, bh.w);
+                        shId[at] = node;
+                    }
+                    np += cnt;
+                }
+                // cells every group accepts -> shared P2C list
+                {
+                    const unsigned long long m = __ballot(accept);
+                    const int cnt = __popcll(m);
+                    if (nc + cnt > a.shcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (accept) {
+                        const int at = nc + mbcnt(m);
+                        shC[S * at] = make_float4(rx, ry, rz, A.w);
+                        if (QUAD) {
+                            shC[S * at + 1] = a.cQ0[c];
+                            shC[S * at + 2] = a.cQ1[c];
+                        }
+                    }
+                    nc += cnt;
+                }
+                // cells the groups disagree on -> deferred
+                {
+                    const unsigned long long m = __ballot(defer);
+                    const int cnt = __popcll(m);
+                    if (nd + cnt > a.shcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (defer) shDef[nd + mbcnt(m)] = node;
+                    nd += cnt;
+                }
+                // cells every group opens -> next frontier
+                {
+                    int tot;
+                    const int off = wave_excl_scan(nk, &tot);
+                    if (nn + tot > a.fcap) {
+                        overflow = true;
+                        break;
+                    }
+                    if (open) {
+                        int *dst = fr1 + nn + off;
+                        if (bfirst >= 0) {
+                            for (int j = 0; j < nk; j++) dst[j] = bfirst + j;
+                        } else {
+                            const int *sub = a.sub + 8 * (int64_t)c;
+                            int j = 0;
+#pragma unroll
+                            for (int o = 0; o < 8; o++) {
+                                const int r = sub[o];
+                                if (r != kNoChild) dst[j++] = r;
+                            }
+                        }
+                    }
+                    nn += tot;
+                }
+            }
+            int *tmp = fr0;
+            fr0 = fr1;
+            fr1 = tmp;
+            nf = nn;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+        if (overflow) {
+            if (lane == 0) atomicOr(&a.counters[S_FLAGS], (unsigned long long)F_FRONTIER);
+            continue;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+
+        // ---- phase B: each group consumes the shared lists, then walks on alone
+        for (int i = 0; i < gn; i++) {
+            const int g = g0 + i;
+            const int64_t gbase = a.gfirst[g];
+            const int gcount = a.gcount[g];
+            const int nq = (gcount + 63) >> 6;
+            bool act[NPL];
+            Lane<NPL> L;
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                const int k = q * 64 + lane;
+                act[q] = k < gcount;
+                const int64_t t = gbase + (act[q] ? k : 0);
+                const float4 h = a.thi[t], l = a.tlo[t];
+                L.px[q] = (h.x - cx) + l.x;
+                L.py[q] = (h.y - cy) + l.y;
+                L.pz[q] = (h.z - cz) + l.z;
+                L.axh[q] = L.axl[q] = L.ayh[q] = L.ayl[q] = L.azh[q] = L.azl[q] = L.ph[q] = L.pl[q] = 0.f;
+                sSelf[w][k] = 0;
+            }
+            const float4 blo_i = sBox[w][i][0], bhi_i = sBox[w][i][1];
+            const float bminx = blo_i.x, bminy = blo_i.y, bminz = blo_i.z;
+            const float bmaxx = bhi_i.x, bmaxy = bhi_i.y, bmaxz = bhi_i.z;
+            wave_lds_sync();
+            // shared bodies, 64 at a time through LDS; members of this group
+            // among them are its self pairs
+            for (int t0 = 0; t0 < np; t0 += 64) {
+                const int cnt = min(64, np - t0);
+                if (lane < cnt) {
+                    P[lane] = shP[t0 + lane];
+                    const int node = shId[t0 + lane];
+                    const int64_t rel = a.selfmode == 1   ? (int64_t)node - gbase
+                                        : a.selfmode == 2 ? (int64_t)a.tinv[node] - gbase
+                                                          : -1;
+                    if (rel >= 0 && rel < gcount) sSelf[w][rel] = 1;
+                }
+                wave_lds_sync();
+                p2p_slots<NPL, SOFT0>(nq, L, P, cnt, eps2);
+                wave_lds_sync();
+            }
+            for (int t0 = 0; t0 < nc; t0 += 64) {
+                const int cnt = min(64, nc - t0);
+                if (lane < cnt) {
+#pragma unroll
+                    for (int s = 0; s < S; s++) Cl[S * lane + s] = shC[S * (t0 + lane) + s];
+                }
+                wave_lds_sync();
+                p2c_slots<NPL, QUAD, SOFT0>(nq, L, Cl, cnt, eps2);
+                wave_lds_sync();
+            }
+            long long tot_pp = np, tot_pc = nc;
+            // own walk from the deferred nodes (read from the shared region in
+            // the first step, then between the two frontier buffers)
+            int mp = 0, mc = 0;
+            int nf2 = nd;
+            const int *src = shDef;
+            int *dstb = frA, *other = frB;
+            bool ovf = false;
+            while (nf2 > 0 && !ovf) {
+                int nn = 0;
+                for (int base = 0; base < nf2; base += 64) {
+                    wave_lds_sync();
+                    const int k = base + lane;
+                    const bool v = k < nf2;
+                    const int node = v ? src[k] : 0;
+                    const bool isBody = v && node >= 0;
+                    const bool isCell = v && node < 0;
+                    bool open = false, accept = false;
+                    int c = 0, nk = 0, bfirst = -1;
+                    float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float rx = 0.f, ry = 0.f, rz = 0.f;
+                    if (isCell) {
+                        c = ~node;
+                        A = a.cA[c];
+                        const float4 B = a.cB[c];
+                        rx = (A.x - cx) + B.x;
+                        ry = (A.y - cy) + B.y;
+                        rz = (A.z - cz) + B.z;
+                        const float dx = fmaxf(fmaxf(bminx - rx, rx - bmaxx), 0.f);
+                        const float dy = fmaxf(fmaxf(bminy - ry, ry - bmaxy), 0.f);
+                        const float dz = fmaxf(fmaxf(bminz - rz, rz - bmaxz), 0.f);
+                        const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const int2 fn = a.fnb[c];
+                        const int lv = a.level[c];
+                        const bool small = fn.y <= a.nleaf || lv == kMaxLevel;
+                        open = tolsq * d2 < B.w || fn.y <= a.nleaf;
+                        accept = !open;
+                        if (open) nk = small ? fn.y : a.nkids[c];
+                        bfirst = small ? fn.x : -1;
+                    }
+                    {
+                        const unsigned long long m = __ballot(isBody);
+                        if (isBody) {
+                            const float4 bh = a.bhi[node], bl = a.blo[node];
+                            P[mp + mbcnt(m)] =
+                                make_float4((bh.x - cx) + bl.x, (bh.y - cy) + bl.y, (bh.z - cz) + bl.z, bh.w);
+                            const int64_t rel = a.selfmode == 1   ? (int64_t)node - gbase
+                                                : a.selfmode == 2 ? (int64_t)a.tinv[node] - gbase
+                                                                  : -1;
+                            if (rel >= 0 && rel < gcount) sSelf[w][rel] = 1;
+                        }
+                        mp += __popcll(m);
+                    }
+                    {
+                        const unsigned long long m = __ballot(accept);
+                        if (accept) {
+                            const int slot = mc + mbcnt(m);
+                            Cl[S * slot] = make_float4(rx, ry, rz, A.w);
+                            if (QUAD) {
+                                Cl[S * slot + 1] = a.cQ0[c];
+                                Cl[S * slot + 2] = a.cQ1[c];
+                            }
+                        }
+                        mc += __popcll(m);
+                    }
+                    {
+                        int tot;
+                        const int off = wave_excl_scan(nk, &tot);
+                        if (nn + tot > a.fcap) {
+                            ovf = true;
+                            break;
+                        }
+                        if (open) {
+                            int *dst = dstb + nn + off;
+                            if (bfirst >= 0) {
+                                for (int j = 0; j < nk; j++) dst[j] = bfirst + j;
+                            } else {
+                                const int *sub = a.sub + 8 * (int64_t)c;
+                                int j = 0;
+#pragma unroll
+                                for (int o = 0; o < 8; o++) {
+                                    const int r = sub[o];
+                                    if (r != kNoChild) dst[j++] = r;
+                                }
+                            }
+                        }
+                        nn += tot;
+                    }
+                    wave_lds_sync();
+                    while (mp >= 64) {
+                        p2p_slots<NPL, SOFT0>(nq, L, P, 64, eps2);
+                        tot_pp += 64;
+                        mp -= 64;
+                        wave_lds_sync();
+                        float4 t = P[64 + (lane < mp ? lane : 0)];
+                        wave_lds_sync();
+                        if (lane < mp) P[lane] = t;
+                        wave_lds_sync();
+                    }
+                    while (mc >= 64) {
+                        p2c_slots<NPL, QUAD, SOFT0>(nq, L, Cl, 64, eps2);
+                        tot_pc += 64;
+                        mc -= 64;
+                        wave_lds_sync();
+                        float4 t0 = Cl[S * (64 + (lane < mc ? lane : 0))];
+                        float4 t1, t2;
+                        if (QUAD) {
+                            t1 = Cl[S * (64 + (lane < mc ? lane : 0)) + 1];
+                            t2 = Cl[S * (64 + (lane < mc ? lane : 0)) + 2];
+                        }
+                        wave_lds_sync();
+                        if (lane < mc) {
+                            Cl[S * lane] = t0;
+                            if (QUAD) {
+                                Cl[S * lane + 1] = t1;
+                                Cl[S * lane + 2] = t2;
+                            }
+                        }
+                        wave_lds_sync();
+                    }
+                }
+                src = dstb;
+                int *tmp = dstb;
+                dstb = other;
+                other = tmp;
+                nf2 = nn;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            }
+            if (ovf) {
+                if (lane == 0) atomicOr(&a.counters[S_FLAGS], (unsigned long long)F_FRONTIER);
+                continue;
+            }
+            wave_lds_sync();
+            if (mp > 0) p2p_slots<NPL, SOFT0>(nq, L, P, mp, eps2);
+            if (mc > 0) p2c_slots<NPL, QUAD, SOFT0>(nq, L, Cl, mc, eps2);
+            tot_pp += mp;
+            tot_pc += mc;
+            wave_lds_sync();
+            int nself = 0;
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                if (!act[q]) continue;
+                const int k = q * 64 + lane;
+                double ph = (double)L.ph[q] + (double)L.pl[q];
+                if (sSelf[w][k]) {
+                    nself++;
+                    if (eps2 > 0.f) {
+                        const int64_t tb = a.selfmode == 1 ? gbase + k : a.tself[gbase + k];
+                        ph -= (double)(a.bhi[tb].w * rsq(eps2));
+                    }
+                }
+                const int64_t o = a.tperm[gbase + k];
+                a.phi[o] = -ph;
+                a.acc[3 * o] = (double)L.axh[q] + (double)L.axl[q];
+                a.acc[3 * o + 1] = (double)L.ayh[q] + (double)L.ayl[q];
+                a.acc[3 * o + 2] = (double)L.azh[q] + (double)L.azl[q];
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) nself += __shfl_xor(nself, d, 64);
+            if (lane == 0) {
+                atomicAdd(&a.counters[0], (unsigned long long)(tot_pp * gcount - nself));
+                atomicAdd(&a.counters[1], (unsigned long long)(tot_pc * gcount));
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
 __global__ void k_fill_zero(double *phi, int64_t n, double *acc)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -613,6 +1055,7 @@ void launch(bh_ctx *ctx, WalkArgs &a, int grid, bool crit)
 {
     if (crit) k_walk<NPL, QUAD, SOFT0, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
     else if (a.lmode) k_walk<NPL, QUAD, SOFT0, true, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else if (a.K > 1) k_walk_sg<NPL, QUAD, SOFT0><<<grid, kTpb, 0, ctx->stream>>>(a);
     else k_walk<NPL, QUAD, SOFT0, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
 }
 
@@ -679,6 +1122,16 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     // build (every tree build is followed by exactly one fast walk)
     a.gctr = reinterpret_cast<int *>(ptr<unsigned long long>(ctx->counters) + S_GCTR);
     const int nwave = grid * kWaves;
+    // super-groups: the plain walk of a single system with the hackcode1
+    // criterion (learned recordings, treecode1 criteria and batches walk
+    // group by group)
+    a.K = (lmode == 0 && p.criterion == BH_CRIT_HACK && ctx->nsys == 0) ? std::max(1, std::min(p.supergroup, kMaxK)) : 1;
+    if (a.K > 1) {
+        const int S = p.usequad ? 3 : 1;
+        a.shcap = (int)std::max<int64_t>(ctx->fcap / 4, 1024);
+        a.shlist = ensure<float4>(ctx->shlist, (size_t)nwave * a.shcap * (1 + S));
+        a.shid = ensure<int>(ctx->shid, (size_t)nwave * 2 * a.shcap);
+    }
     if (lmode) {
         if (ctx->lcap == 0) ctx->lcap = 1 << 16;
         a.lcap = (int)ctx->lcap;

@@ -35,7 +35,7 @@ class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("eps", C.c_double), ("usequad", C.c_int), ("mode", C.c_int),
                 ("rsize", C.c_double), ("rmin", C.c_double * 3), ("rmin_given", C.c_int),
                 ("group_size", C.c_int), ("learned", C.c_int), ("keep_box", C.c_int), ("nleaf", C.c_int),
-                ("criterion", C.c_int)]
+                ("criterion", C.c_int), ("supergroup", C.c_int)]
 
 
 class Stats(C.Structure):
