@@ -498,7 +498,9 @@ def main():
             # opening is more accurate at the same tol
             near = [k for k, t in enumerate(ms) if t <= 1.02 * min(ms)]
             g, nl = max((E.AUTOTUNE_CANDIDATES[k] for k in near), key=lambda c: (c[0], -ms[E.AUTOTUNE_CANDIDATES.index(c)]))
-            base = dict(base, group_size=g, nleaf=nl)
+            # super-groups as the autotuner's second stage chose them for its best (group, leaf)
+            sg = best.supergroup if (g, nl) == (best.group_size, best.nleaf) else 1
+            base = dict(base, group_size=g, nleaf=nl, supergroup=sg)
             tune = {"nominal_tol": nominal, "candidates": [list(c) for c in E.AUTOTUNE_CANDIDATES],
                     "ms": [round(float(t), 3) for t in ms], "seconds": time.perf_counter() - t0}
         if fixed_tol is None:
@@ -510,7 +512,8 @@ def main():
         sec, stats = time_steps(eng, p, args.steps, args.warmup)
         last = stats[-1]
         return {"usequad": base["usequad"], "learned": base.get("learned", 0), "tol": tol,
-                "group_size": base["group_size"], "nleaf": base.get("nleaf", -1), "autotune": tune,
+                "group_size": base["group_size"], "nleaf": base.get("nleaf", -1),
+                "supergroup": base.get("supergroup", 1), "autotune": tune,
                 "median_rel_err": med, "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
                 "interactions_per_step": int(last.n2btot + last.nbctot), "n2b": int(last.n2btot),
                 "nbc": int(last.nbctot), "n_mlp": int(last.n_mlp), "ngroups": int(last.ngroups),

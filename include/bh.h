@@ -201,9 +201,11 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
 /* Autotuner (no reference counterpart): times the fast walk on the uploaded
  * problem (single or batch) for group_size in {64, 128, 192, 256} x nleaf in
  * {0, 1, 2, 4, 8}, best of reps evaluations each (device time of tree +
- * walk), and makes the fastest the default used when group_size = 0 and
- * nleaf = -1.  ms (may be NULL) receives BH_AUTOTUNE_NCAND times in that
- * order; best (may be NULL) the chosen parameters. */
+ * walk), then (single systems, hackcode1 criterion) supergroup 2, 3, 4 for
+ * the fastest of those, and makes the fastest the default used when
+ * group_size = 0, nleaf = -1 and supergroup = 0.  ms (may be NULL) receives
+ * the BH_AUTOTUNE_NCAND times of the first stage in that order; best (may be
+ * NULL) the chosen parameters. */
 #define BH_AUTOTUNE_NCAND 20
 bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms);
 

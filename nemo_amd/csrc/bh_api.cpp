@@ -800,8 +800,29 @@ bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best
                     qbest = q;
                 }
             }
+        // then super-groups for the chosen group and leaf size (single
+        // systems, hackcode1 criterion: the only walks that use them)
+        qbest.supergroup = 1;
+        if (ctx->nsys == 0 && p->criterion == BH_CRIT_HACK) {
+            for (int K : {2, 3, 4}) {
+                bh_params q = qbest;
+                q.supergroup = K;
+                bh_stats st{};
+                eval(ctx, &q, &st);
+                double t = 1e300;
+                for (int r = 0; r < std::max(1, reps); r++) {
+                    eval(ctx, &q, &st);
+                    t = std::min(t, st.t_tree_ms + st.t_walk_ms);
+                }
+                if (t < tbest) {
+                    tbest = t;
+                    qbest = q;
+                }
+            }
+        }
         ctx->tuned_group = qbest.group_size;
         ctx->tuned_nleaf = qbest.nleaf;
+        ctx->tuned_sg = qbest.supergroup;
         if (best) *best = qbest;
     });
 }
