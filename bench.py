@@ -413,6 +413,7 @@ def main():
     ap.add_argument("--nsub", type=int, default=8192)
     ap.add_argument("--tol", type=float, default=None, help="skip the plain calibration and use this tol")
     ap.add_argument("--group", type=int, default=0, help="group size (64, 128, 192, 256); 0: autotuned per variant")
+    ap.add_argument("--nleaf", type=int, default=-1, help="with --group: leaf size (-1: the engine's default)")
     ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
@@ -525,7 +526,7 @@ def main():
 
     plain = []
     for usequad in ((0, 1) if args.quad < 0 else (args.quad,)):
-        v = variant(dict(eps=args.eps, usequad=usequad, group_size=args.group), args.tol)
+        v = variant(dict(eps=args.eps, usequad=usequad, group_size=args.group, nleaf=args.nleaf), args.tol)
         log("plain tree usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
             usequad, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
         plain.append(v)
@@ -535,7 +536,8 @@ def main():
         for lq in ((1, 0) if args.learned_quad < 0 else (args.learned_quad,)):
             tr = learned_model(lq)
             eng.upload(m, x)
-            v = variant(dict(eps=args.eps, usequad=lq, group_size=args.group, learned=1), args.learned_tol)
+            v = variant(dict(eps=args.eps, usequad=lq, group_size=args.group, nleaf=args.nleaf, learned=1),
+                        args.learned_tol)
             v["training"] = tr
             log("learned walk usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step (MLP kernel "
                 "%.3f ms, %d evaluations)" % (lq, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"],
