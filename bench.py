@@ -576,9 +576,11 @@ def main():
         "roofline": {
             "bound": "valu",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
-            "kernel": "k_walk<%d,%s,false,%s,false>" % (head_v["group_size"] // 64,
-                                                       "true" if head_v["usequad"] else "false",
-                                                       "true" if head_v["learned"] else "false"),
+            "kernel": (("k_walk_sg<%d,%s,false>" % (head_v["group_size"] // 64, "true" if head_v["usequad"] else "false"))
+                       if head_v.get("supergroup", 1) > 1 and not head_v["learned"] else
+                       "k_walk<%d,%s,false,%s,false>" % (head_v["group_size"] // 64,
+                                                         "true" if head_v["usequad"] else "false",
+                                                         "true" if head_v["learned"] else "false")),
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
