@@ -421,7 +421,7 @@ def main():
     ap.add_argument("--learned-quad", type=int, default=-1,
                     help="learned walk cells: 1 quadrupole, 0 monopole, -1 both (each with its own model; the faster "
                          "is the learned figure)")
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -518,6 +518,7 @@ def main():
                 "median_rel_err": med, "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
                 "interactions_per_step": int(last.n2btot + last.nbctot), "n2b": int(last.n2btot),
                 "nbc": int(last.nbctot), "n_mlp": int(last.n_mlp), "ngroups": int(last.ngroups),
+                "deep_buckets": int(last.deep_buckets),
                 "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])),
                 "t_sort_ms": float(np.mean([s.t_sort_ms for s in stats])),
                 "t_walk_kernel_ms": float(np.mean([s.t_walk_kernel_ms for s in stats])),

@@ -244,7 +244,8 @@ bh_status bh_debug_learned(bh_ctx *ctx, int64_t cap, int *cell, int *group, floa
  * bh_debug_perm) and body count. */
 bh_status bh_debug_cell_ranges(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ncell);
 bh_status bh_mlp_clear_samples(bh_ctx *ctx);
-/* Adam on the collected samples (mean squared error of the scaled residual). */
+/* Adam on the collected samples (mean squared error of the scaled targets),
+ * the learning rate decaying from lr to lr/10 (cosine) over the call's steps. */
 bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *epoch_loss);
 
 /* Synthetic inputs: equal-mass Plummer sphere in virial units (G=M=1,

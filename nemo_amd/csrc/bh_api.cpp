@@ -1110,6 +1110,9 @@ bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t se
             }
             return a;
         };
+        // learning rate: cosine decay from lr to lr/10 over the steps of this call
+        const int64_t total_steps = (int64_t)epochs * nb;
+        int64_t step = 0;
         for (int e = 0; e < epochs; e++) {
             uint64_t a = (rnd() % (uint64_t)n) | 1;
             while (gcd(a, (uint64_t)n) != 1) a = (a + 2) % (uint64_t)n;
@@ -1117,7 +1120,10 @@ bh_status bh_mlp_train(bh_ctx *ctx, int epochs, int batch, float lr, uint64_t se
             for (int64_t b = 0; b < nb; b++) {
                 bh::mlp_gather(ctx, a, c, b * batch, batch, bx, by);
                 bh::mlp_grad(ctx, bx, by, batch, grad, lossb + b);
-                bh::mlp_adam(ctx, grad, lr, ++M.adam_step);
+                const double f = total_steps > 1 ? (double)step / (double)(total_steps - 1) : 0.0;
+                const float lr_t = (float)(lr * (0.1 + 0.45 * (1.0 + std::cos(3.141592653589793 * f))));
+                step++;
+                bh::mlp_adam(ctx, grad, lr_t, ++M.adam_step);
             }
             if (epoch_loss) {
                 std::vector<float> lh(std::max<int64_t>(nb, 1));
