@@ -9,8 +9,9 @@ Morton keys, radix sort, topology, FP64 moments, group walk with the
 interactions and, for the learned variant, the MLP correction.  Inputs are
 resident in HBM before timing starts; results stay on the device.
 
-Plain tree: monopole and quadrupole cells are both calibrated to the target
-median error and the faster is the plain figure.  Learned: for walks with
+Plain tree: monopole and quadrupole cells, and monopole cells with the
+group local expansion of far cells (local_alpha in --local), are each
+calibrated to the target median error and the fastest is the plain figure.  Learned: for walks with
 quadrupole and with monopole cells, an MLP is trained on the device from
 residual samples of three other Plummer snapshots (seeds 41, 43 and 47, tol
 0.9/1.1/1.3, cells with size/r in [0.45, 1.5]; the targets are the exact
@@ -23,8 +24,9 @@ both variants and their ratio `learned_over_plain` are in `config`); with
 --no-learned it is the plain figure.
 
 Prints one JSON line.  `roofline` is for the dominant kernel, the walk: 20
-flop per interaction (P2P and P2C, counted by the kernel) over the walk
-kernel's own event-timed duration, against MI355X's FP32 vector peak (the
+flop per interaction (P2P and P2C, counted by the kernel; with local
+expansions only the interactions evaluated one by one, nbctot - n_local of
+the cell interactions) over the walk kernel's own event-timed duration, against MI355X's FP32 vector peak (the
 walk issues no MFMA: its bound is labelled "valu", the vector ALU).
 `roofline_mlp` is the MLP inference kernel: 19,072 flop per evaluation
 against the dense BF16 MFMA peak.  `cpu_baseline` times the CPU restatement
@@ -304,7 +306,7 @@ def run_ensemble(args, E, log):
                                 "ms_per_step": t_lf / args.lf_steps * 1e3,
                                 "interactions_per_s": float((n2b + nbc).sum() / t_lf),
                                 "energy_drift": {"median": float(np.median(drift)), "max": float(np.max(drift))}}},
-        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false,false>" % (best.group_size // 64)),
+        "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,true,false,false,false,false>" % (best.group_size // 64)),
     }
 
 
@@ -333,7 +335,7 @@ def run_plummer1m(args, E, log):
                      "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
                      "interactions_per_step": n_int, "interactions_per_s": n_int / sec,
                      "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])), "t_walk_kernel_ms": t_walk * 1e3,
-                     "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,false,false,false,false>" %
+                     "roofline": walk_roofline(n_int, t_walk, "k_walk<%d,false,false,false,false,false>" %
                                                (best.group_size // 64))})
         log("plummer 2^20 tol %.1f: group %d nleaf %d median %.3g p99 %.3g  %.3f ms/step  %.3g interactions/s" % (
             tol, best.group_size, best.nleaf, med, p99, sec * 1e3, n_int / sec))
@@ -389,7 +391,7 @@ def run_diskhalo(args, E, log):
                    "first_eval": {"median_rel_err": float(np.median(e)), "p99_rel_err": float(np.percentile(e, 99)),
                                   "t_tree_ms": st.t_tree_ms, "t_walk_kernel_ms": st.t_walk_kernel_ms}},
         "roofline": walk_roofline(int(st.n2btot + st.nbctot), st.t_walk_kernel_ms * 1e-3,
-                                  "k_walk<%d,true,false,false,false>" % (group // 64)),
+                                  "k_walk<%d,true,false,false,false,false>" % (group // 64)),
     }
 
 
@@ -415,6 +417,9 @@ def main():
     ap.add_argument("--group", type=int, default=0, help="group size (64, 128, 192, 256); 0: autotuned per variant")
     ap.add_argument("--nleaf", type=int, default=-1, help="with --group: leaf size (-1: the engine's default)")
     ap.add_argument("--quad", type=int, default=-1, help="plain tree: -1 calibrates both and keeps the faster")
+    ap.add_argument("--local", default="0.12,0.2",
+                    help="plain monopole tree with group local expansions at these local_alpha values "
+                         "(comma-separated; 'none' skips)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
                     help="skip the learned calibration and use this tol (profiling runs)")
@@ -514,10 +519,12 @@ def main():
         last = stats[-1]
         return {"usequad": base["usequad"], "learned": base.get("learned", 0), "tol": tol,
                 "group_size": base["group_size"], "nleaf": base.get("nleaf", -1),
-                "supergroup": base.get("supergroup", 1), "autotune": tune,
+                "supergroup": base.get("supergroup", 1), "local_alpha": base.get("local_alpha", 0.0),
+                "autotune": tune,
                 "median_rel_err": med, "p99_rel_err": p99, "ms_per_step": sec * 1e3, "evals_per_s": n / sec,
                 "interactions_per_step": int(last.n2btot + last.nbctot), "n2b": int(last.n2btot),
-                "nbc": int(last.nbctot), "n_mlp": int(last.n_mlp), "ngroups": int(last.ngroups),
+                "nbc": int(last.nbctot), "n_local": int(last.n_local), "n_mlp": int(last.n_mlp),
+                "ngroups": int(last.ngroups),
                 "deep_buckets": int(last.deep_buckets),
                 "t_tree_ms": float(np.mean([s.t_tree_ms for s in stats])),
                 "t_sort_ms": float(np.mean([s.t_sort_ms for s in stats])),
@@ -531,6 +538,16 @@ def main():
         log("plain tree usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
             usequad, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))
         plain.append(v)
+    alphas = [] if args.local.strip().lower() in ("", "none") else [float(a) for a in args.local.split(",")]
+    if args.quad != 1:
+        for alpha in alphas:
+            v = variant(dict(eps=args.eps, usequad=0, group_size=args.group, nleaf=args.nleaf, local_alpha=alpha),
+                        args.tol)
+            log("plain tree + local expansions (alpha %.3f): group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  "
+                "%.3f ms/step (%.1f %% of the cell interactions by expansion)" % (
+                    alpha, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"],
+                    v["ms_per_step"], 100.0 * v["n_local"] / max(v["nbc"], 1)))
+            plain.append(v)
     best_plain = min(plain, key=lambda v: v["ms_per_step"])
     learned_variants = []
     if not args.no_learned:
@@ -550,7 +567,9 @@ def main():
     # config, with their ratio)
     head_v = learned if learned is not None and learned["evals_per_s"] > best_plain["evals_per_s"] else best_plain
     t_walk = head_v["t_walk_kernel_ms"] * 1e-3
-    achieved = FLOP_PER_INTERACTION * head_v["interactions_per_step"] / t_walk / 1e12
+    tile_int = head_v["interactions_per_step"] - head_v["n_local"]     # evaluated one by one
+    achieved = FLOP_PER_INTERACTION * tile_int / t_walk / 1e12
+    head_local = head_v["local_alpha"] > 0
     out = {
         "metric": METRIC,
         "value": head_v["evals_per_s"],
@@ -578,19 +597,24 @@ def main():
             "bound": "valu",
             "compute_unit": "FP32 vector ALU (the walk issues no MFMA)",
             "kernel": (("k_walk_sg<%d,%s,false>" % (head_v["group_size"] // 64, "true" if head_v["usequad"] else "false"))
-                       if head_v.get("supergroup", 1) > 1 and not head_v["learned"] else
-                       "k_walk<%d,%s,false,%s,false>" % (head_v["group_size"] // 64,
-                                                         "true" if head_v["usequad"] else "false",
-                                                         "true" if head_v["learned"] else "false")),
+                       if head_v.get("supergroup", 1) > 1 and not head_v["learned"] and not head_local else
+                       "k_walk<%d,%s,false,%s,false,%s>" % (head_v["group_size"] // 64,
+                                                            "true" if head_v["usequad"] else "false",
+                                                            "true" if head_v["learned"] else "false",
+                                                            "true" if head_local else "false")),
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "flop_convention": "20 flop per interaction (P2P and P2C), interactions counted by the kernel",
+            "flop_convention": "20 flop per interaction evaluated one by one (P2P and P2C, counted by the "
+                               "kernel; cell interactions taken by group local expansions are not counted)",
+            "interactions_evaluated": int(tile_int),
+            "interactions_by_expansion": int(head_v["n_local"]),
         },
     }
-    traffic, src = profiled_traffic(out["roofline"]["kernel"])
+    traffic, src = profiled_traffic(out["roofline"]["kernel"],
+                                    "walk_plummer4M_local_mono.md" if head_local else "walk_plummer4M_plain_mono.md")
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_source"] = src

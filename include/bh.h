@@ -83,6 +83,13 @@ typedef struct {
     int supergroup;      /* fast mode: K consecutive groups share one traversal (1 = off, 2..4;
                             0 = tuned default); the interactions are those of K = 1.  Used by the
                             plain single-system walk with the hackcode1 criterion */
+    double local_alpha;  /* fast mode, monopole cells, no learned correction: an accepted cell whose
+                            c.o.m. lies farther than R/local_alpha from the group centre (R: the
+                            group's radius about that centre) enters the group's second-order local
+                            expansion (the cell's softened monopole potential expanded to third
+                            order, its field to second, about the centre) instead of one
+                            interaction per member.  0 = off (default), at most 0.5; runs with
+                            supergroup 1 */
 } bh_params;
 
 typedef struct {
@@ -99,6 +106,8 @@ typedef struct {
     double t_sort_ms;    /* device time of the mass-set key sort */
     int64_t ngroups;     /* fast mode: groups of test points walked */
     double t_mlp_ms;     /* learned mode: device time of the MLP inference kernel alone */
+    int64_t n_local;     /* fast mode: the body-cell interactions of nbctot evaluated through
+                            group local expansions (local_alpha > 0) */
 } bh_stats;
 
 typedef struct bh_ctx bh_ctx;
@@ -201,7 +210,7 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
 /* Autotuner (no reference counterpart): times the fast walk on the uploaded
  * problem (single or batch) for group_size in {64, 128, 192, 256} x nleaf in
  * {0, 1, 2, 4, 8}, best of reps evaluations each (device time of tree +
- * walk), then (single systems, hackcode1 criterion) supergroup 2, 3, 4 for
+ * walk), then (single systems, hackcode1 criterion, local_alpha 0) supergroup 2, 3, 4 for
  * the fastest of those, and makes the fastest the default used when
  * group_size = 0, nleaf = -1 and supergroup = 0.  ms (may be NULL) receives
  * the BH_AUTOTUNE_NCAND times of the first stage in that order; best (may be

@@ -103,6 +103,10 @@ void check_params(const bh_params *p)
     if (p->supergroup < 0 || p->supergroup > 4) throw ApiFail(BH_ERR_ARG, "supergroup must be 0 (tuned), 1 (off) or 2..4");
     if (p->criterion != BH_CRIT_HACK && p->learned)
         throw ApiFail(BH_ERR_ARG, "the learned correction is trained for the hackcode1 criterion only");
+    if (!(p->local_alpha >= 0.0 && p->local_alpha <= 0.5))
+        throw ApiFail(BH_ERR_ARG, "local_alpha must lie in [0, 0.5]");
+    if (p->local_alpha > 0.0 && p->mode == BH_MODE_FAST && (p->usequad || p->learned))
+        throw ApiFail(BH_ERR_ARG, "local expansions take monopole cells without the learned correction");
 }
 
 void lf_drop_graph(bh_ctx *ctx)
@@ -242,6 +246,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     s.nbctot = (int64_t)stat[bh::S_NBC];
     s.deep_buckets = (int64_t)stat[bh::S_DEEP];
     s.n_mlp = (int64_t)stat[bh::S_NMLP];
+    s.n_local = (int64_t)stat[bh::S_NLOC];
     s.ncell = ctx->tree.ncell;
     for (int k = 0; k < 3; k++) s.rmin[k] = box[k];
     s.rsize = box[3];
@@ -348,6 +353,7 @@ void bh_default_params(bh_params *p)
     p->nleaf = -1;
     p->criterion = BH_CRIT_HACK;
     p->supergroup = 0;
+    p->local_alpha = 0.0;
 }
 
 const char *bh_version(void) { return "nemo_amd-bh 0.1 (gfx950)"; }
@@ -801,9 +807,10 @@ bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best
                 }
             }
         // then super-groups for the chosen group and leaf size (single
-        // systems, hackcode1 criterion: the only walks that use them)
+        // systems, hackcode1 criterion, no local expansions: the only walks
+        // that use them)
         qbest.supergroup = 1;
-        if (ctx->nsys == 0 && p->criterion == BH_CRIT_HACK) {
+        if (ctx->nsys == 0 && p->criterion == BH_CRIT_HACK && p->local_alpha == 0.0) {
             for (int K : {2, 3, 4}) {
                 bh_params q = qbest;
                 q.supergroup = K;

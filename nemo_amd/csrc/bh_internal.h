@@ -76,8 +76,9 @@ enum StatSlot {
     S_NGROUPS = 6,
     S_FLAGS = 7,
     S_TNCELL = 8, S_TNCELL_RAW = 9,   // topology of a separate test set (grouping only)
-    S_ZERO = 10,        // slots below are cleared at the start of every evaluation
-    S_STICKY = 10,      // flags or-ed over the steps of a leapfrog batch
+    S_NLOC = 10,        // body-cell interactions evaluated through group local expansions
+    S_ZERO = 11,        // slots below are cleared at the start of every evaluation
+    S_STICKY = 11,      // flags or-ed over the steps of a leapfrog batch
     // the fast walk's group counter (low 32 bits; cleared at the start of
     // every tree build, one walk per build), on a cache line of its own: its
     // atomics would otherwise serialise the walk's reads of S_NGROUPS

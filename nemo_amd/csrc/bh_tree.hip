@@ -1041,8 +1041,8 @@ __device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8
     cnkids[c] = nk;
 }
 
-// Bottom-up moments.  Threads start at cells without child cells; the last
-// child to finish carries on with the parent.  Accumulation follows
+// Moments of cell c from its children's (computed before it: the levels
+// are processed bottom-up).  Accumulation follows
 // hackcofm's order: children in octant order, FP64, no FMA (this file is
 // compiled with -ffp-contract=off).
 template <bool QUAD>
@@ -1683,13 +1683,15 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     // monopole cell), so a learned or sampling build computes it whatever
     // the walk's cells are
     for (int L = kMaxLevel; L >= 0; L--) {
+        // level L holds at most 8^L cells: the top levels need a block or two
+        const int lgrid = L <= 3 ? nblocks((int64_t)1 << (3 * L)) : mgrid * 2;
         if (p.usequad || feat)
-            k_moments_level<true><<<mgrid * 2, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
+            k_moments_level<true><<<lgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
                                                               cfirst, clast, clevel, bd, mo, cA, cB,
                                                               p.usequad ? cQ0 : nullptr, cQ1, d_box, lsys,
                                                               ctx->sysbits, sysbox);
         else
-            k_moments_level<false><<<mgrid * 2, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
+            k_moments_level<false><<<lgrid, kTpb, 0, st>>>(llist, loffd, L, ks, feat ? cF0 : nullptr, cF1, csub,
                                                                cfirst, clast, clevel, bd, mo, cA, cB, cQ0, cQ1, d_box,
                                                                lsys, ctx->sysbits, sysbox);
     }

@@ -21,6 +21,23 @@
 // inner loops subtract small numbers.  Each lane sums a tile in plain FP32
 // and folds the tile sum into a (hi, lo) pair with TwoSum.
 //
+// Local expansions (local_alpha > 0, monopole cells): an accepted cell whose
+// c.o.m. lies farther than R/alpha from the group centre (R: the group's
+// radius about the centre) is not put on the P2C list.  Its softened
+// monopole potential, expanded about the centre to third order (field to
+// second order), is summed into 20 coefficients per group: in units of the
+// group radius l, with n = d/rho, F = m/rho^2 and s = l/rho (< alpha),
+//   G0  += m/rho                       (potential)
+//   G1  += F n_i                       (field at the centre)
+//   G2  += F s (3 n_i n_j - delta_ij)  (its gradient times l)
+//   G3  += F s^2 (15 n_i n_j n_k - 3 (delta_ij n_k + delta_ik n_j + delta_jk n_i))
+// and each member at u = (x - centre)/l receives
+//   a   = G1 + G2 u + 1/2 G3[u, u]
+//   phi = -(G0 + l u.(G1 + 1/2 G2 u + 1/6 G3[u, u])).
+// Far cells wait in an LDS list; every 64 of them the wave computes one
+// cell's terms per lane and sums them across the wave into the group's row
+// of 20 coefficients (LDS).
+//
 // A body's own contribution is removed exactly: with eps > 0 the self pair
 // adds m*rsq(eps^2) to the potential sum and nothing to the acceleration, so
 // that term is subtracted at the end; with eps == 0 the kernel masks r = 0.
@@ -212,6 +229,93 @@ __device__ __forceinline__ void p2c_slots(int nq, Lane<NPL> &L, const float4 *sr
     else eval_p2c<NPL, NPL, QUAD, SOFT0>(L, src, cnt, eps2);
 }
 
+// Local expansion coefficients, in the order G0, G1 (x y z), G2 (xx xy xz yy
+// yz zz), G3 (xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz).
+constexpr int kLoc = 20;
+
+// v + (v of another lane chosen by the DPP control CTRL); rows outside RM add 0.
+template <int CTRL, int RM = 0xF>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xF, true));
+}
+
+// Sum over the wave, valid in lane 63: quad swaps (xor 1, xor 2), half-row
+// and row mirrors (sums of 8 and 16 lanes), then row 15 -> next row and
+// lane 31 -> rows 2-3 broadcasts.  All VALU, no LDS round trips.
+__device__ __forceinline__ float wave_sum63(float v)
+{
+    v = dpp_add<0xB1>(v);          // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E>(v);          // quad_perm [2,3,0,1]
+    v = dpp_add<0x141>(v);         // row_half_mirror
+    v = dpp_add<0x140>(v);         // row_mirror
+    v = dpp_add<0x142, 0xA>(v);    // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xC>(v);    // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// Adds the first cnt far cells of F (c.o.m. relative to the centre, mass) to
+// the group's coefficients row[kLoc]: lane k takes cell k, then the wave sums.
+__device__ __forceinline__ void local_flush(const float4 *F, int cnt, float *row, float eps2, float ell)
+{
+    const int lane = lane_id();
+    const float4 e = lane < cnt ? F[lane] : make_float4(1.f, 0.f, 0.f, 0.f);   // massless: no terms
+    const float rho2 = fmaf(e.x, e.x, fmaf(e.y, e.y, fmaf(e.z, e.z, eps2)));
+    const float ri = rsq(rho2);
+    const float nx = e.x * ri, ny = e.y * ri, nz = e.z * ri;
+    const float F0 = e.w * ri * ri;
+    const float sc = ell * ri;
+    const float Fs = F0 * sc, Fs2 = Fs * sc;
+    const float e3 = 3.f * Fs, t15 = 15.f * Fs2, f3 = 3.f * Fs2;
+    // one coefficient at a time: its wave sum, added by lane 63
+    auto put = [&](int k, float v) {
+        v = wave_sum63(v);
+        if (lane == 63) row[k] += v;
+    };
+    put(0, e.w * ri);
+    put(1, F0 * nx);
+    put(2, F0 * ny);
+    put(3, F0 * nz);
+    put(4, fmaf(e3 * nx, nx, -Fs));
+    put(5, e3 * nx * ny);
+    put(6, e3 * nx * nz);
+    put(7, fmaf(e3 * ny, ny, -Fs));
+    put(8, e3 * ny * nz);
+    put(9, fmaf(e3 * nz, nz, -Fs));
+    put(10, fmaf(t15 * nx, nx * nx, -3.f * f3 * nx));
+    put(11, fmaf(t15 * nx, nx * ny, -f3 * ny));
+    put(12, fmaf(t15 * nx, nx * nz, -f3 * nz));
+    put(13, fmaf(t15 * nx, ny * ny, -f3 * nx));
+    put(14, t15 * nx * (ny * nz));
+    put(15, fmaf(t15 * nx, nz * nz, -f3 * nx));
+    put(16, fmaf(t15 * ny, ny * ny, -3.f * f3 * ny));
+    put(17, fmaf(t15 * ny, ny * nz, -f3 * nz));
+    put(18, fmaf(t15 * ny, nz * nz, -f3 * ny));
+    put(19, fmaf(t15 * nz, nz * nz, -3.f * f3 * nz));
+}
+
+// Field and potential sum (+m/r convention of the tiles) of the summed
+// coefficients G at the member u = (x - centre)/l.
+__device__ __forceinline__ void local_eval(const float *G, float ux, float uy, float uz, float ell, float &ax,
+                                           float &ay, float &az, float &ph)
+{
+    const float gx = fmaf(G[4], ux, fmaf(G[5], uy, G[6] * uz));
+    const float gy = fmaf(G[5], ux, fmaf(G[7], uy, G[8] * uz));
+    const float gz = fmaf(G[6], ux, fmaf(G[8], uy, G[9] * uz));
+    const float xx = ux * ux, yy = uy * uy, zz = uz * uz;
+    const float xy2 = 2.f * ux * uy, xz2 = 2.f * ux * uz, yz2 = 2.f * uy * uz;
+    const float hx = fmaf(G[10], xx, fmaf(G[11], xy2, fmaf(G[12], xz2, fmaf(G[13], yy, fmaf(G[14], yz2, G[15] * zz)))));
+    const float hy = fmaf(G[11], xx, fmaf(G[13], xy2, fmaf(G[14], xz2, fmaf(G[16], yy, fmaf(G[17], yz2, G[18] * zz)))));
+    const float hz = fmaf(G[12], xx, fmaf(G[14], xy2, fmaf(G[15], xz2, fmaf(G[17], yy, fmaf(G[18], yz2, G[19] * zz)))));
+    ax = fmaf(0.5f, hx, G[1] + gx);
+    ay = fmaf(0.5f, hy, G[2] + gy);
+    az = fmaf(0.5f, hz, G[3] + gz);
+    const float px = fmaf(1.f / 6.f, hx, fmaf(0.5f, gx, G[1]));
+    const float py = fmaf(1.f / 6.f, hy, fmaf(0.5f, gy, G[2]));
+    const float pz = fmaf(1.f / 6.f, hz, fmaf(0.5f, gz, G[3]));
+    ph = fmaf(ell, fmaf(ux, px, fmaf(uy, py, uz * pz)), G[0]);
+}
+
 struct WalkArgs {
     const float4 *thi, *tlo;
     const int *tself, *tinv;
@@ -253,6 +357,7 @@ struct WalkArgs {
     // super-groups: K consecutive groups share one traversal; per-wave
     // scratch for the shared lists (shcap entries each)
     int K;
+    float lalpha2;        // local expansions: local_alpha^2 (0 = off)
     float4 *shlist;       // [wave][shcap P entries][shcap * S C entries]
     int *shid;            // [wave][shcap body ids of the P entries][shcap deferred nodes]
     int shcap;
@@ -265,14 +370,19 @@ struct WalkArgs {
 // c.o.m. lies more than s beyond U.  Every body's own sink cube lies in U, so
 // each body's treecode1 test passes whenever the group's does.  U is formed
 // in integer coordinates of the finest level (2^21 per side).
-template <int NPL, bool QUAD, bool SOFT0, bool LREC, bool CRIT>
-__global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
+// The local-expansion variants ask for 4 waves per SIMD (128 VGPRs): left
+// to itself the compiler gives them ~160 and 3 waves.
+template <int NPL, bool QUAD, bool SOFT0, bool LREC, bool CRIT, bool LOCAL>
+__global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && NPL == 3 ? 4 : 1))) void k_walk(WalkArgs a)
 {
+    static_assert(!(LOCAL && (QUAD || LREC)), "local expansions take monopole cells, no records");
     constexpr int G = 64 * NPL;
     constexpr int S = QUAD ? 3 : 1;
     __shared__ float4 sP[kWaves][kListCap];
     __shared__ float4 sC[kWaves][kListCap * S];
     __shared__ unsigned char sSelf[kWaves][G];
+    __shared__ float4 sF[LOCAL ? kWaves : 1][LOCAL ? kListCap : 1];    // far cells awaiting the expansion
+    __shared__ float sG[LOCAL ? kWaves : 1][kLoc];                     // the group's coefficients
     const int w = threadIdx.x >> 6, lane = lane_id();
     float4 *P = sP[w];
     float4 *Cl = sC[w];
@@ -337,6 +447,20 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         }
         bminx = wave_min(bminx); bminy = wave_min(bminy); bminz = wave_min(bminz);
         bmaxx = wave_max(bmaxx); bmaxy = wave_max(bmaxy); bmaxz = wave_max(bmaxz);
+        // local expansions: group radius^2 about the centre (inactive slots
+        // copy the first particle), its length unit, this lane's column
+        float lrg2 = 0.f, lell = 0.f;
+        float4 *Fl = sF[LOCAL ? w : 0];
+        float *lrow = sG[LOCAL ? w : 0];
+        int nfl = 0;
+        if (LOCAL) {
+#pragma unroll
+            for (int q = 0; q < NPL; q++)
+                lrg2 = fmaxf(lrg2, fmaf(L.px[q], L.px[q], fmaf(L.py[q], L.py[q], L.pz[q] * L.pz[q])));
+            lrg2 = wave_max(lrg2);
+            lell = sqrtf(lrg2);
+            if (lane < kLoc) lrow[lane] = 0.f;
+        }
         int ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};
         float uoff[3] = {0.f, 0.f, 0.f}, uunit = 0.f, urs = 0.f;
         if (CRIT) {
@@ -366,7 +490,7 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         for (int q = 0; q < NPL; q++) sSelf[w][q * 64 + lane] = 0;
 
         int np = 0, nc = 0, nf = 0;
-        long long tot_pp = 0, tot_pc = 0;
+        long long tot_pp = 0, tot_pc = 0, tot_lc = 0;
         const bool lrecord = LREC && (a.lmode == 1 || (a.lmode == 2 && g % a.lstride == 0));
         int *lreg = a.lrec + (int64_t)(blockIdx.x * kWaves + w) * a.lcap;
         int lhead = 0, lcnt = 0;
@@ -438,6 +562,17 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     accept = !open;
                     if (open) nk = small ? fn.y : a.nkids[c];
                     bfirst = small ? fn.x : -1;
+                }
+                // far accepted cells -> the local expansion's list
+                if (LOCAL) {
+                    const bool far = accept && lrg2 < a.lalpha2 * fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+                    const unsigned long long m = __ballot(far);
+                    if (far) {
+                        Fl[nfl + mbcnt(m)] = make_float4(rx, ry, rz, A.w);
+                        accept = false;
+                    }
+                    nfl += __popcll(m);
+                    tot_lc += __popcll(m);
                 }
                 // bodies -> P2P list
                 {
@@ -536,6 +671,15 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
                     }
                     wave_lds_sync();
                 }
+                if (LOCAL && nfl >= 64) {
+                    local_flush(Fl, 64, lrow, eps2, lell);
+                    nfl -= 64;
+                    wave_lds_sync();
+                    const float4 t = Fl[64 + (lane < nfl ? lane : 0)];
+                    wave_lds_sync();
+                    if (lane < nfl) Fl[lane] = t;
+                    wave_lds_sync();
+                }
             }
             int *tmp = fr0;
             fr0 = fr1;
@@ -553,6 +697,24 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         tot_pp += np;
         tot_pc += nc;
         wave_lds_sync();
+        if (LOCAL && tot_lc > 0) {
+            // the last far cells, then the expansion at every member
+            if (nfl > 0) local_flush(Fl, nfl, lrow, eps2, lell);
+            wave_lds_sync();
+            float Gs[kLoc];
+#pragma unroll
+            for (int k = 0; k < kLoc; k++) Gs[k] = lrow[k];
+            const float linv = lell > 0.f ? 1.f / lell : 0.f;
+#pragma unroll
+            for (int q = 0; q < NPL; q++) {
+                float ex, ey, ez, ep;
+                local_eval(Gs, L.px[q] * linv, L.py[q] * linv, L.pz[q] * linv, lell, ex, ey, ez, ep);
+                fold(L.axh[q], L.axl[q], ex);
+                fold(L.ayh[q], L.ayl[q], ey);
+                fold(L.azh[q], L.azl[q], ez);
+                fold(L.ph[q], L.pl[q], ep);
+            }
+        }
 
         if (lrecord && lhead + 5 + 64 <= a.lcap) {
             lused = lhead + 5 + lcnt;
@@ -597,7 +759,8 @@ __global__ __launch_bounds__(kTpb) void k_walk(WalkArgs a)
         for (int d = 32; d >= 1; d >>= 1) nself += __shfl_xor(nself, d, 64);
         if (lane == 0) {
             atomicAdd(&a.counters[0], (unsigned long long)(tot_pp * gcount - nself));
-            atomicAdd(&a.counters[1], (unsigned long long)(tot_pc * gcount));
+            atomicAdd(&a.counters[1], (unsigned long long)((tot_pc + tot_lc) * gcount));
+            if (LOCAL) atomicAdd(&a.counters[S_NLOC], (unsigned long long)(tot_lc * gcount));
         }
     }
 }
@@ -1053,10 +1216,13 @@ __global__ void k_clear_regions(int *lrec, int cap)
 template <int NPL, bool QUAD, bool SOFT0>
 void launch(bh_ctx *ctx, WalkArgs &a, int grid, bool crit)
 {
-    if (crit) k_walk<NPL, QUAD, SOFT0, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
-    else if (a.lmode) k_walk<NPL, QUAD, SOFT0, true, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    if (a.lalpha2 > 0.f && !QUAD) {
+        if (crit) k_walk<NPL, false, SOFT0, false, true, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+        else k_walk<NPL, false, SOFT0, false, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+    } else if (crit) k_walk<NPL, QUAD, SOFT0, false, true, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else if (a.lmode) k_walk<NPL, QUAD, SOFT0, true, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
     else if (a.K > 1) k_walk_sg<NPL, QUAD, SOFT0><<<grid, kTpb, 0, ctx->stream>>>(a);
-    else k_walk<NPL, QUAD, SOFT0, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
+    else k_walk<NPL, QUAD, SOFT0, false, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
 }
 
 }  // namespace
@@ -1123,9 +1289,12 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     a.gctr = reinterpret_cast<int *>(ptr<unsigned long long>(ctx->counters) + S_GCTR);
     const int nwave = grid * kWaves;
     // super-groups: the plain walk of a single system with the hackcode1
-    // criterion (learned recordings, treecode1 criteria and batches walk
-    // group by group)
-    a.K = (lmode == 0 && p.criterion == BH_CRIT_HACK && ctx->nsys == 0) ? std::max(1, std::min(p.supergroup, kMaxK)) : 1;
+    // criterion (learned recordings, treecode1 criteria, batches and local
+    // expansions walk group by group)
+    a.lalpha2 = (lmode == 0 && !p.usequad) ? (float)(p.local_alpha * p.local_alpha) : 0.f;
+    a.K = (lmode == 0 && p.criterion == BH_CRIT_HACK && ctx->nsys == 0 && a.lalpha2 == 0.f)
+              ? std::max(1, std::min(p.supergroup, kMaxK))
+              : 1;
     if (a.K > 1) {
         const int S = p.usequad ? 3 : 1;
         a.shcap = (int)std::max<int64_t>(ctx->fcap / 4, 1024);

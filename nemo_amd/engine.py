@@ -35,14 +35,16 @@ class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("eps", C.c_double), ("usequad", C.c_int), ("mode", C.c_int),
                 ("rsize", C.c_double), ("rmin", C.c_double * 3), ("rmin_given", C.c_int),
                 ("group_size", C.c_int), ("learned", C.c_int), ("keep_box", C.c_int), ("nleaf", C.c_int),
-                ("criterion", C.c_int), ("supergroup", C.c_int)]
+                ("criterion", C.c_int), ("supergroup", C.c_int),
+                ("local_alpha", C.c_double)]
 
 
 class Stats(C.Structure):
     _fields_ = [("n2btot", C.c_int64), ("nbctot", C.c_int64), ("ncell", C.c_int64), ("deep_buckets", C.c_int64),
                 ("rmin", C.c_double * 3), ("rsize", C.c_double), ("t_tree_ms", C.c_double),
                 ("t_walk_ms", C.c_double), ("n_mlp", C.c_int64), ("t_walk_kernel_ms", C.c_double),
-                ("t_sort_ms", C.c_double), ("ngroups", C.c_int64), ("t_mlp_ms", C.c_double)]
+                ("t_sort_ms", C.c_double), ("ngroups", C.c_int64), ("t_mlp_ms", C.c_double),
+                ("n_local", C.c_int64)]
 
     def as_dict(self):
         return {k: (list(getattr(self, k)) if k == "rmin" else getattr(self, k)) for k, _ in self._fields_}

@@ -77,6 +77,10 @@ def lib():
         L.lc_features.restype = C.c_double
         L.lc_apply.argtypes = [C.c_int, _d, _d, _d, _d, _d]
         L.lc_apply.restype = None
+        L.le_coeffs.argtypes = [C.c_int, _d, _d, _d, C.c_double, C.c_double, _d]
+        L.le_coeffs.restype = None
+        L.le_eval.argtypes = [_d, C.c_double, _d, C.c_int, _d, _d, _d]
+        L.le_eval.restype = None
         _LIB = L
     return _LIB
 
@@ -368,3 +372,28 @@ def apply_first_order(x, centre, R, j6):
     out = np.zeros_like(x)
     lib().lc_apply(len(x), _p(x), _p(c), _p(R), _p(j6), _p(out))
     return out
+
+
+# ---- group local expansion of far cells (le_oracle.c) -------------------------
+
+def local_coeffs(m, s, centre, eps, ell):
+    """Summed coefficients G[20] of the second-order local expansion about
+    centre of the softened monopoles (masses m, c.o.m. s[k][3]), tensors
+    scaled by powers of ell (le_oracle.c's order)."""
+    m = np.ascontiguousarray(m, np.float64)
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    c = np.ascontiguousarray(centre, np.float64)
+    G = np.zeros(20)
+    lib().le_coeffs(len(m), _p(m), _p(s), _p(c), float(eps), float(ell), _p(G))
+    return G
+
+
+def local_eval(G, ell, centre, x):
+    """Potential (NEMO's sign) and field of the expansion G at points x."""
+    G = np.ascontiguousarray(G, np.float64)
+    c = np.ascontiguousarray(centre, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    phi = np.zeros(len(x))
+    acc = np.zeros_like(x)
+    lib().le_eval(_p(G), float(ell), _p(c), len(x), _p(x), _p(phi), _p(acc))
+    return phi, acc

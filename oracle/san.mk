@@ -5,6 +5,6 @@ CC      ?= gcc
 CFLAGS  = -O1 -g -fno-omit-frame-pointer -fPIC -ffp-contract=off -fno-fast-math -std=gnu11 \
           -fsanitize=address,undefined -fno-sanitize-recover=undefined
 
-_san/liboracle.so: bh_oracle.c tc_oracle.c lc_oracle.c
+_san/liboracle.so: bh_oracle.c tc_oracle.c lc_oracle.c le_oracle.c
 	mkdir -p _san
-	$(CC) $(CFLAGS) -shared -o $@ bh_oracle.c tc_oracle.c lc_oracle.c -lm -lpthread
+	$(CC) $(CFLAGS) -shared -o $@ bh_oracle.c tc_oracle.c lc_oracle.c le_oracle.c -lm -lpthread
