@@ -420,6 +420,7 @@ def main():
     ap.add_argument("--local", default="0.12,0.2",
                     help="plain monopole tree with group local expansions at these local_alpha values "
                          "(comma-separated; 'none' skips)")
+    ap.add_argument("--local-only", action="store_true", help="plain tree: only the local-expansion variants")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-tol", type=float, default=None,
                     help="skip the learned calibration and use this tol (profiling runs)")
@@ -533,7 +534,7 @@ def main():
                 "t_walk_phase_ms": float(np.mean([s.t_walk_ms for s in stats]))}
 
     plain = []
-    for usequad in ((0, 1) if args.quad < 0 else (args.quad,)):
+    for usequad in (() if args.local_only else (0, 1) if args.quad < 0 else (args.quad,)):
         v = variant(dict(eps=args.eps, usequad=usequad, group_size=args.group, nleaf=args.nleaf), args.tol)
         log("plain tree usequad=%d: group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  %.3f ms/step" % (
             usequad, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"], v["p99_rel_err"], v["ms_per_step"]))

@@ -18,8 +18,8 @@
  * reference's test model (uniform ball, or M(r) = r with cencon=) drawn
  * from this program's own generator seeded by seed=.  Keywords added:
  * usequad= (default: true when invoked as hackcode1_qp), mode=fast|strict,
- * group=, nleaf=, criterion=hack|rcrit|bh86|sw94 (treecode1's opening tests),
- * device=.  fcells= is accepted and ignored.
+ * group=, nleaf=, local=, criterion=hack|rcrit|bh86|sw94 (treecode1's opening
+ * tests), device=.  fcells= is accepted and ignored.
  */
 #include <math.h>
 #include <stdint.h>
@@ -58,6 +58,7 @@ static gp_keyword keys[] = {
     {"mode", "fast", "fast (FP32 group walk) or strict (FP64, reference order)", NULL},
     {"group", "0", "particles per group in fast mode (0 = default, 64, 128, 192 or 256)", NULL},
     {"nleaf", "-1", "cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
+    {"local", "0", "fast mode, monopole cells: far cells (group radius < local x distance) by group local expansion (0 = off, <= 0.5)", NULL},
     {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},
     {"VERSION", VERSION, "", NULL},
@@ -462,6 +463,7 @@ int main(int argc, char **argv)
     else gp_fatal("Parameter mode=%s: fast or strict", mode);
     p.group_size = gp_iparam("group");
     p.nleaf = gp_iparam("nleaf");
+    p.local_alpha = gp_dparam("local");
     p.criterion = gp_choice("criterion", criteria, 4);
     p.rsize = rsize;
     p.rmin_given = 1;

@@ -9,7 +9,7 @@
  * hackforce_qp (Makefile:86-91 of the reference) it uses quadrupole cells.
  *
  * Keywords added here: usequad= (default: by program name),
- * mode=fast|strict|direct, group=, nleaf=, criterion=hack|rcrit|bh86|sw94
+ * mode=fast|strict|direct, group=, nleaf=, local=, criterion=hack|rcrit|bh86|sw94
  * (bh_criterion: treecode1's opening tests, treeload.c:232-252), device=.  fcells= is accepted and
  * ignored: the device tree sizes itself.  Errors print
  * "### Fatal error [hackforce]: ..." and exit with status 255
@@ -42,6 +42,7 @@ static gp_keyword keys[] = {
     {"mode", "fast", "fast (FP32 group walk), strict (FP64, reference order), direct (FP64 direct sum)", NULL},
     {"group", "0", "Particles per group in fast mode (0 = default, 64, 128, 192 or 256)", NULL},
     {"nleaf", "-1", "Cells with <= nleaf bodies interact body by body in fast mode (-1 = default)", NULL},
+    {"local", "0", "Fast mode, monopole cells: far cells (group radius < local x distance) by group local expansion (0 = off, <= 0.5)", NULL},
     {"criterion", "hack", "Cell opening: hack (hackcode1), rcrit, bh86 or sw94 (treecode1's criteria)", NULL},
     {"device", "0", "GPU ordinal", NULL},
     {"VERSION", VERSION, "", NULL},
@@ -80,6 +81,7 @@ int main(int argc, char **argv)
     p.usequad = gp_bparam("usequad", qp_name);
     p.group_size = (int)dparam("group");
     p.nleaf = (int)dparam("nleaf");
+    p.local_alpha = dparam("local");
     p.criterion = gp_choice("criterion", criteria, 4);
     const char *mode = param("mode");
     int direct = 0;

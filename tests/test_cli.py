@@ -227,6 +227,30 @@ def test_hackforce_fast_and_direct_modes(tmp_path):
     assert np.median(e) < 1e-3 and np.percentile(e, 99) < 1e-2
 
 
+@gpu
+def test_hackforce_local_keyword_equals_library(tmp_path):
+    """local= sets bh_params.local_alpha: the program's output equals
+    bh_force with the same parameters bit for bit; an out-of-range value is
+    the library's argument error."""
+    from nemo_amd import engine as E
+
+    m, x, v = E.mkplummer(30000, seed=12)
+    write_stream(str(tmp_path / "in"), [(m, x, v, 0.0)])
+    r = run("hackforce", "in", "out", "tol=0.7", "eps=0.01", "group=192", "nleaf=1", "local=0.2", cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    got = snapshot.read(str(tmp_path / "out"))[0]
+    e = E.Engine(0)
+    try:
+        phi, acc, st = e.force(m, x, E.default_params(tol=0.7, eps=0.01, rsize=4.0, group_size=192, nleaf=1,
+                                                       local_alpha=0.2))
+    finally:
+        e.close()
+    assert st.n_local > 0
+    assert np.array_equal(got.extra["Potential"], phi) and np.array_equal(got.extra["Acceleration"], acc)
+    r = run("hackforce", "in", "out2", "local=0.7", cwd=str(tmp_path))
+    assert r.returncode == 255 and b"local_alpha" in r.stderr, r.stderr
+
+
 # ---- hackcode1 -------------------------------------------------------------------------
 
 def test_hackcode1_help_and_argument_errors(tmp_path):
