@@ -305,6 +305,21 @@ void batch_upload(bh_ctx *ctx, int nsys, const int64_t *off, const double *mass,
     BH_HIP(hipMemcpyAsync(dsp, sp.data(), sp.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     bh::ensure<double4>(ctx->sysbox, nsys);
     bh::ensure<int>(ctx->sysroot, nsys);
+    // prefix-sum moments: every system starts a tile, so its sums round as
+    // in a run of that system alone
+    std::vector<int2> tiles;
+    std::vector<int> seg(nsys + 1);
+    for (int sy = 0; sy < nsys; sy++) {
+        seg[sy] = (int)tiles.size();
+        for (int64_t b = off[sy]; b < off[sy + 1]; b += bh::kPreTile)
+            tiles.push_back(make_int2((int)b, (int)std::min<int64_t>(b + bh::kPreTile, off[sy + 1])));
+    }
+    seg[nsys] = (int)tiles.size();
+    int2 *dt = bh::ensure<int2>(ctx->ptile, tiles.size());
+    int *dseg = bh::ensure<int>(ctx->pseg, nsys + 1);
+    BH_HIP(hipMemcpyAsync(dt, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    BH_HIP(hipMemcpyAsync(dseg, seg.data(), seg.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ctx->nptile = (int)tiles.size();
     BH_HIP(hipStreamSynchronize(ctx->stream));
     ctx->nsys = nsys;
     ctx->sys_eps0 = eps0;
@@ -404,7 +419,7 @@ void bh_destroy(bh_ctx *ctx)
                           &ctx->blo,    &ctx->thi,    &ctx->tlo,    &ctx->td,      &ctx->tperm,  &ctx->tself, &ctx->tinv, &ctx->tmap,
                           &ctx->cpl,    &ctx->ccnt,   &ctx->cbase,  &ctx->cmin1,   &ctx->cmin2,  &ctx->cfirst,
                           &ctx->clast,  &ctx->clevel, &ctx->csub,   &ctx->cnkids,  &ctx->crc2,
-                          &ctx->cskip,  &ctx->pre,    &ctx->cmo,
+                          &ctx->cskip,  &ctx->pre,    &ctx->cmo,    &ctx->mpre,    &ctx->mpart,  &ctx->ptile, &ctx->pseg,
                           &ctx->cA,     &ctx->cB,     &ctx->cQ0,    &ctx->cQ1,     &ctx->counters, &ctx->frontier,
                           &ctx->gctr,   &ctx->werr, &ctx->shlist, &ctx->shid,   &ctx->phi,    &ctx->acc,    &ctx->gfirst, &ctx->gcount,
                           &ctx->gscan,  &ctx->goff,   &ctx->gcpl,   &ctx->gccnt,   &ctx->gcbase, &ctx->gm1,

@@ -58,6 +58,7 @@ struct bh_ctx {
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cskip, pre, cowner, gowner;
     bh::DevBuf crc2;            // treecode1 Rcrit2 per cell (criterion != BH_CRIT_HACK)
     bh::DevBuf cmo, cfnb, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
+    bh::DevBuf mpre, mpart;     // fast monopole trees: prefix sums of (m x, m y, m z, m) over the sorted bodies
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
     bh::DevBuf gfirst, gcount, gscan, goff;
@@ -96,6 +97,8 @@ struct bh_ctx {
     int sysbits = 0;
     bool sys_eps0 = false;      // some system has eps = 0
     bh::DevBuf boff, sysbox, sysp, sysroot, sysdiag;
+    bh::DevBuf ptile, pseg;     // prefix-sum moments of a batch: (first, end) body per tile, first tile per system
+    int nptile = 0;
     bh::Tree tree;
     int64_t nb = 0;             // massive bodies (counted at upload)
     int cell_cap = 0;           // cell capacity of the tree arrays

@@ -11,6 +11,7 @@
 
 namespace bh {
 
+constexpr int kPreTile = 2048;         // bodies per tile of the prefix-sum moments (a system starts a tile)
 constexpr int kMaxLevel = 21;           // 63-bit Morton keys: 21 levels below the root
 constexpr int kWaveSize = 64;
 

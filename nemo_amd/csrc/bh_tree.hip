@@ -1198,6 +1198,161 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
     }
 }
 
+// ---- fast monopole trees: moments from prefix sums --------------------------
+//
+// A cell is a contiguous range [first, last] of the sorted bodies, so its mass
+// and first moments are differences of the FP64 prefix sums of (m x, m y,
+// m z, m): one pass over the bodies (beside the topology, on the second
+// stream) and one O(1) step per cell, instead of 22 dependent level launches.
+// The sums round differently from hackcofm's child-by-child order, so this
+// path serves the fast walk with monopole cells only; strict mode and
+// quadrupole or learned builds keep the level-by-level pass.  The difference
+// of two prefixes carries an absolute error of ~1e-16 of the running sums:
+// relative 1e-9 of a two-body cell's mass at N = 4M, far below the FP32 of
+// the walk.
+constexpr int kPreThr = 256, kPrePer = 8;
+static_assert(kPreThr * kPrePer == kPreTile, "prefix tile");
+
+__device__ __forceinline__ double4 d4add(double4 a, double4 b) { return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+__device__ __forceinline__ double4 body_mom(const double4 b) { return make_double4(b.x * b.w, b.y * b.w, b.z * b.w, b.w); }
+
+// Bodies [lo, hi) of tile t: kPreTile-body tiles from 0 (one system), or the
+// batch's tiles, which restart at every system's first body.
+__device__ __forceinline__ int2 pre_tile(const int2 *__restrict__ tiles, int t, int n)
+{
+    return tiles ? tiles[t] : make_int2(t * kPreTile, min(n, (t + 1) * kPreTile));
+}
+
+// Block-wide exclusive scan of one double4 per thread (kPreThr threads).
+__device__ double4 block_excl_scan(double4 v, double4 *tot)
+{
+    __shared__ double4 sw[kPreThr / 64];
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    double4 x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        double4 t;
+        t.x = __shfl_up(x.x, d, 64);
+        t.y = __shfl_up(x.y, d, 64);
+        t.z = __shfl_up(x.z, d, 64);
+        t.w = __shfl_up(x.w, d, 64);
+        if (lane >= d) x = d4add(x, t);
+    }
+    if (lane == 63) sw[w] = x;
+    __syncthreads();
+    double4 off = make_double4(0, 0, 0, 0), all = make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kPreThr / 64; k++) {
+        if (k < w) off = d4add(off, sw[k]);
+        all = d4add(all, sw[k]);
+    }
+    __syncthreads();
+    if (tot) *tot = all;
+    return d4add(off, make_double4(x.x - v.x, x.y - v.y, x.z - v.z, x.w - v.w));
+}
+
+// Per tile: the sum of its bodies' (m x, m y, m z, m).
+__global__ __launch_bounds__(kPreThr) void k_pre_reduce(const double4 *__restrict__ bd, int n,
+                                                        const int2 *__restrict__ tiles, double4 *part)
+{
+    const int2 r = pre_tile(tiles, blockIdx.x, n);
+    const int base = r.x + threadIdx.x * kPrePer;
+    double4 s = make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kPrePer; k++)
+        if (base + k < r.y) s = d4add(s, body_mom(bd[base + k]));
+    double4 tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Exclusive scan of the tile sums of one system per block.
+__global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *__restrict__ seg, int ntile)
+{
+    const int a = seg ? seg[blockIdx.x] : 0, b = seg ? seg[blockIdx.x + 1] : ntile;
+    double4 carry = make_double4(0, 0, 0, 0);
+    for (int base = a; base < b; base += kPreThr) {
+        const int i = base + threadIdx.x;
+        const double4 v = i < b ? part[i] : make_double4(0, 0, 0, 0);
+        double4 tot;
+        const double4 e = block_excl_scan(v, &tot);
+        if (i < b) part[i] = d4add(carry, e);
+        carry = d4add(carry, tot);
+    }
+}
+
+// inc[i] = the sum over the bodies of i's system up to and including i.
+__global__ __launch_bounds__(kPreThr) void k_pre_down(const double4 *__restrict__ bd, int n,
+                                                      const int2 *__restrict__ tiles,
+                                                      const double4 *__restrict__ part, double4 *inc)
+{
+    const int2 r = pre_tile(tiles, blockIdx.x, n);
+    const int base = r.x + threadIdx.x * kPrePer;
+    double4 v[kPrePer];
+    double4 s = make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kPrePer; k++) {
+        v[k] = base + k < r.y ? body_mom(bd[base + k]) : make_double4(0, 0, 0, 0);
+        s = d4add(s, v[k]);
+    }
+    double4 run = d4add(part[blockIdx.x], block_excl_scan(s, nullptr));
+#pragma unroll
+    for (int k = 0; k < kPrePer; k++) {
+        run = d4add(run, v[k]);
+        if (base + k < r.y) inc[base + k] = run;
+    }
+}
+
+// Mass, c.o.m. and the FP32 walk records of every cell from the prefixes
+// (the records cell_moments writes for a monopole build).  In a batch, the
+// cells above the systems' roots are never walked and get no moments.
+__global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__ cfirst, const int *__restrict__ clast,
+                                                         const uint8_t *__restrict__ clevel,
+                                                         const unsigned long long *ncellp,
+                                                         const double4 *__restrict__ inc, const uint64_t *__restrict__ keys,
+                                                         double *mo, float4 *cA, float4 *cB, const double4 *__restrict__ box,
+                                                         int lsys, int sysbits, const double4 *__restrict__ sysbox,
+                                                         const int64_t *__restrict__ boff)
+{
+    const int ncell = (int)*ncellp;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        const int s = cfirst[c], e = clast[c];
+        const int L = clevel[c];
+        int Lg = L, s0 = 0;
+        double rsize = box->w;
+        if (sysbox) {
+            const int sys = sysbits ? (int)(keys[s] >> (63 - sysbits)) : 0;
+            rsize = sysbox[sys].w;
+            Lg = L - lsys;
+            s0 = (int)boff[sys];
+        }
+        double *me = mo + 10 * (int64_t)c;
+        if (Lg < 0) {
+            me[0] = me[1] = me[2] = me[3] = 0.0;
+            cA[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            cB[c] = make_float4(0.f, 0.f, 0.f, 3.0e38f);
+            continue;
+        }
+        const double4 b = inc[e];
+        const double4 a = s > s0 ? inc[s - 1] : make_double4(0, 0, 0, 0);
+        const double M = b.w - a.w;
+        const double px = (b.x - a.x) / M, py = (b.y - a.y) / M, pz = (b.z - a.z) / M;
+        me[0] = M;
+        me[1] = px;
+        me[2] = py;
+        me[3] = pz;
+        float4 A, B;
+        split2(px, A.x, B.x);
+        split2(py, A.y, B.y);
+        split2(pz, A.z, B.z);
+        A.w = (float)M;
+        B.w = (float)(rsize * rsize * ldexp(1.0, -2 * Lg));
+        cA[c] = A;
+        cB[c] = B;
+    }
+}
+
 // Cells bucketed by level so the moments can be computed level by level,
 // deepest first, with a kernel boundary between levels: a counting sort with
 // one global atomic per (block, level).  The order of cells inside a level
@@ -1563,8 +1718,21 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
     // the gather (HBM-bound) runs beside the topology (latency-bound); only
     // the moments and the walk read its records
+    // fast monopole builds take the moments from prefix sums of the gathered
+    // bodies (on the side stream, beside the topology)
+    const bool feat_b = p.learned != 0 || ctx->mlp_sampling;
+    const bool prefix_moments = p.mode == BH_MODE_FAST && !p.usequad && !feat_b && nb > 1;
+    double4 *mpre = prefix_moments ? ensure<double4>(ctx->mpre, (size_t)nb) : nullptr;
     side_fork(ctx, kForkSort);
     if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, ctx->side>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo);
+    if (prefix_moments) {
+        const int2 *tiles = nsys ? ptr<int2>(ctx->ptile) : nullptr;
+        const int ntile = nsys ? ctx->nptile : (int)((nb + kPreTile - 1) / kPreTile);
+        double4 *part = ensure<double4>(ctx->mpart, ntile);
+        k_pre_reduce<<<ntile, kPreThr, 0, ctx->side>>>(bd, nb, tiles, part);
+        k_pre_top<<<nsys ? nsys : 1, kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr, ntile);
+        k_pre_down<<<ntile, kPreThr, 0, ctx->side>>>(bd, nb, tiles, part, mpre);
+    }
     side_mark(ctx, kMarkGather);
     T.bd = bd;
     T.bhi = bhi;
@@ -1653,8 +1821,10 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     int *llist = ensure<int>(ctx->llist, cap);
     side_fork(ctx, kForkTopo);
     hipStream_t sd = ctx->side;
-    k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
-    k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount, loffd, lcursor, llist);
+    if (!prefix_moments) {
+        k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
+        k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount, loffd, lcursor, llist);
+    }
     side_mark(ctx, kMarkLevels);
     k_children<<<capgrid(ctx, cap), kTpb, 0, st>>>(ks, cpl, cbase, cfirst, clast, clevel, tt.ncellp, csub, cnkids);
     T.first = cfirst;
@@ -1682,7 +1852,11 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     // the learned features carry the quadrupole (the leading error of a
     // monopole cell), so a learned or sampling build computes it whatever
     // the walk's cells are
-    for (int L = kMaxLevel; L >= 0; L--) {
+    if (prefix_moments)
+        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, st>>>(cfirst, clast, clevel, tt.ncellp, mpre, ks, mo, cA, cB,
+                                                            d_box, lsys, ctx->sysbits, sysbox,
+                                                            nsys ? ptr<int64_t>(ctx->boff) : nullptr);
+    for (int L = prefix_moments ? -1 : kMaxLevel; L >= 0; L--) {
         // level L holds at most 8^L cells: the top levels need a block or two
         const int lgrid = L <= 3 ? nblocks((int64_t)1 << (3 * L)) : mgrid * 2;
         if (p.usequad || feat)

@@ -370,10 +370,10 @@ struct WalkArgs {
 // c.o.m. lies more than s beyond U.  Every body's own sink cube lies in U, so
 // each body's treecode1 test passes whenever the group's does.  U is formed
 // in integer coordinates of the finest level (2^21 per side).
-// The local-expansion variants ask for 4 waves per SIMD (128 VGPRs): left
-// to itself the compiler gives them ~160 and 3 waves.
+// The monopole variants with three points per lane ask for 4 waves per SIMD
+// (128 VGPRs): left to itself the compiler gives them ~160 and 3 waves.
 template <int NPL, bool QUAD, bool SOFT0, bool LREC, bool CRIT, bool LOCAL>
-__global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && NPL == 3 ? 4 : 1))) void k_walk(WalkArgs a)
+__global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(!QUAD && !LREC && NPL == 3 ? 4 : 1))) void k_walk(WalkArgs a)
 {
     static_assert(!(LOCAL && (QUAD || LREC)), "local expansions take monopole cells, no records");
     constexpr int G = 64 * NPL;
@@ -383,11 +383,16 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
     __shared__ unsigned char sSelf[kWaves][G];
     __shared__ float4 sF[LOCAL ? kWaves : 1][LOCAL ? kListCap : 1];    // far cells awaiting the expansion
     __shared__ float sG[LOCAL ? kWaves : 1][kLoc];                     // the group's coefficients
+    // the first kLFr entries of each frontier level live in LDS, the rest in
+    // the wave's global frontier (index - kLFr): most levels never leave LDS
+    constexpr int kLFr = QUAD ? 192 : 384;
+    __shared__ int sFr[kWaves][2][kLFr];
     const int w = threadIdx.x >> 6, lane = lane_id();
     float4 *P = sP[w];
     float4 *Cl = sC[w];
     int *fr0 = a.frontier + (int64_t)(blockIdx.x * kWaves + w) * 2 * a.fcap;
     int *fr1 = fr0 + a.fcap;
+    int *lf0 = sFr[w][0], *lf1 = sFr[w][1];
 
     int lused = 1;                       // used ints of this wave's record region
     for (;;) {
@@ -506,13 +511,13 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
             }
         }
         if (sys >= 0) {
-            if (lane == 0) fr0[0] = a.sysroot[sys];
+            if (lane == 0) lf0[0] = a.sysroot[sys];
             nf = 1;
         } else if (*a.ncellp == 0) {
-            if (a.nb == 1 && lane == 0) fr0[0] = 0;
+            if (a.nb == 1 && lane == 0) lf0[0] = 0;
             nf = a.nb == 1 ? 1 : 0;
         } else {
-            if (lane == 0) fr0[0] = ~0;
+            if (lane == 0) lf0[0] = ~0;
             nf = 1;
         }
         bool overflow = false;
@@ -522,15 +527,18 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
                 wave_lds_sync();
                 const int k = base + lane;
                 const bool v = k < nf;
-                const int node = v ? fr0[k] : 0;
+                const int node = v ? (k < kLFr ? lf0[k] : fr0[k - kLFr]) : 0;
                 const bool isBody = v && node >= 0;
                 const bool isCell = v && node < 0;
                 bool open = false, accept = false;
                 int c = 0, nk = 0, bfirst = -1;
                 float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
                 float rx = 0.f, ry = 0.f, rz = 0.f;
+                int4 s0 = make_int4(kNoChild, kNoChild, kNoChild, kNoChild), s1 = s0;
                 if (isCell) {
                     c = ~node;
+                    s0 = reinterpret_cast<const int4 *>(a.sub)[2 * (int64_t)c];
+                    s1 = reinterpret_cast<const int4 *>(a.sub)[2 * (int64_t)c + 1];
                     A = a.cA[c];
                     const float4 B = a.cB[c];
                     rx = (A.x - cx) + B.x;
@@ -560,7 +568,13 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
                         open = tolsq * d2 < B.w || fn.y <= a.nleaf;
                     }
                     accept = !open;
-                    if (open) nk = small ? fn.y : a.nkids[c];
+                    if (open) {
+                        int cnt = 0;
+                        const int so[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+                        for (int o = 0; o < 8; o++) cnt += so[o] != kNoChild;
+                        nk = small ? fn.y : cnt;
+                    }
                     bfirst = small ? fn.x : -1;
                 }
                 // far accepted cells -> the local expansion's list
@@ -619,21 +633,25 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
                 {
                     int tot;
                     const int off = wave_excl_scan(nk, &tot);
-                    if (nn + tot > a.fcap) {
+                    if (nn + tot > kLFr + a.fcap) {
                         overflow = true;
                         break;
                     }
                     if (open) {
-                        int *dst = fr1 + nn + off;
+                        const int at = nn + off;
+                        auto put = [&](int j, int r) {
+                            if (at + j < kLFr) lf1[at + j] = r;
+                            else fr1[at + j - kLFr] = r;
+                        };
                         if (bfirst >= 0) {
-                            for (int j = 0; j < nk; j++) dst[j] = bfirst + j;
+                            for (int j = 0; j < nk; j++) put(j, bfirst + j);
                         } else {
-                            const int *sub = a.sub + 8 * (int64_t)c;
+                            const int so[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
                             int j = 0;
 #pragma unroll
                             for (int o = 0; o < 8; o++) {
-                                int r = sub[o];
-                                if (r != kNoChild) dst[j++] = r;
+                                int r = so[o];
+                                if (r != kNoChild) put(j++, r);
                             }
                         }
                     }
@@ -684,6 +702,9 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(LOCAL && N
             int *tmp = fr0;
             fr0 = fr1;
             fr1 = tmp;
+            tmp = lf0;
+            lf0 = lf1;
+            lf1 = tmp;
             nf = nn;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }

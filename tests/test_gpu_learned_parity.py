@@ -10,8 +10,9 @@ grav.c:42-73), to the FP32 rounding of the stored values.
 
 Correction: after a learned evaluation, every recorded entry's predicted
 field equals the MLP oracle's output on the oracle's features (BF16
-tolerance), and every group member's correction (learned minus plain walk,
-same lists) equals R + J (x - centre) from the group's summed entries."""
+tolerance), and every group member's correction (learned minus the same
+learned walk with a zero-output model: same tree, moments and lists) equals
+R + J (x - centre) from the group's summed entries."""
 import numpy as np
 import pytest
 
@@ -104,7 +105,8 @@ def test_learned_entries_and_group_correction_equal_oracle():
         gfirst, gcount = e.debug_groups()
         rmin, rsize = np.array(st.rmin), st.rsize
         assert st.n_mlp == len(ecell) > 10000
-        e.eval(E.default_params(**base))
+        e.mlp_set(np.zeros_like(P), norm)
+        e.eval(E.default_params(learned=1, **base))
         _, a0 = e.download()
         # every entry's predicted field: the MLP oracle on the oracle's features
         gc, gs = _tree_geometry(cells, first, count, perm, x, rmin, rsize)
