@@ -417,6 +417,8 @@ __global__ void k_scan_top(uint32_t *partial, int64_t nparts, uint32_t *out, int
     }
 }
 
+// Each thread owns kScanItems consecutive entries; when in and out are
+// 16-byte aligned, a thread's full run moves as 16-byte vectors.
 __global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n, const unsigned long long *dn,
                             const uint32_t *__restrict__ partial)
 {
@@ -424,21 +426,41 @@ __global__ void k_scan_down(const uint32_t *in, uint32_t *out, int64_t n, const 
     const int64_t base = (int64_t)blockIdx.x * kScanChunk;
     if (base >= n) return;
     const int64_t mine = base + (int64_t)threadIdx.x * kScanItems;
+    const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const bool full = vec && mine + kScanItems <= n;
     uint32_t v[kScanItems];
     uint32_t s = 0;
+    if (full) {
 #pragma unroll
-    for (int r = 0; r < kScanItems; r++) {
-        int64_t i = mine + r;
-        v[r] = i < n ? in[i] : 0;
-        s += v[r];
+        for (int r = 0; r < kScanItems / 4; r++) {
+            const uint4 q = reinterpret_cast<const uint4 *>(in + mine)[r];
+            v[4 * r] = q.x;
+            v[4 * r + 1] = q.y;
+            v[4 * r + 2] = q.z;
+            v[4 * r + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++) v[r] = mine + r < n ? in[mine + r] : 0;
     }
+#pragma unroll
+    for (int r = 0; r < kScanItems; r++) s += v[r];
     uint32_t tot;
     uint32_t ex = block_excl_scan(s, &tot) + partial[blockIdx.x];
+    uint32_t o[kScanItems];
 #pragma unroll
     for (int r = 0; r < kScanItems; r++) {
-        int64_t i = mine + r;
-        if (i < n) out[i] = ex;
+        o[r] = ex;
         ex += v[r];
+    }
+    if (full) {
+#pragma unroll
+        for (int r = 0; r < kScanItems / 4; r++)
+            reinterpret_cast<uint4 *>(out + mine)[r] = make_uint4(o[4 * r], o[4 * r + 1], o[4 * r + 2], o[4 * r + 3]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++)
+            if (mine + r < n) out[mine + r] = o[r];
     }
 }
 

@@ -84,6 +84,24 @@ def test_cells_match_oracle_tree_large(eng):
     assert np.array_equal(c["quad6"], t["quad"][:, [0, 1, 2, 4, 5, 8]])
 
 
+def test_fast_monopole_cells_from_prefix_sums(eng):
+    """Fast monopole builds take each cell's mass and c.o.m. from FP64 prefix
+    sums over the sorted bodies: equal to the oracle's tree (hackcofm's
+    child-by-child sums) to the rounding of the running sums."""
+    m, x, _ = E.mkplummer(600000, seed=6, with_vel=False)
+    eng.upload(m, x)
+    eng.eval(E.default_params(tol=0.7, eps=0.01, usequad=0, group_size=192, nleaf=1))
+    c = eng.debug_cells()
+    t = oracle.tree_cells(m, x, np.full(3, -2.0), 4.0, usequad=False)
+    assert np.array_equal(c["level"], t["level"]) and np.array_equal(c["nbody"], t["nbody"])
+    dm = np.abs(c["mass"] - t["mass"]) / t["mass"]
+    dp = np.abs(c["pos"] - t["pos"]).max(1) / np.maximum(np.abs(t["pos"]).max(1), 1e-3)
+    print("prefix-sum moments: max relative mass error %.2e, c.o.m. %.2e" % (dm.max(), dp.max()))
+    assert dm.max() < 1e-9 and dp.max() < 1e-9
+    # the root's mass is the total
+    assert abs(c["mass"][0] - m.sum()) < 1e-12 * m.sum()
+
+
 def test_hack128_fixture_strict(eng, golden):
     """The reference's own t=0 frame: tree forces at tol=1 (box -2..2) give the
     stored FP64 potential energy and PE tensor exactly."""
