@@ -384,15 +384,24 @@ __device__ __forceinline__ int64_t scan_len(int64_t n, const unsigned long long 
     return dn ? min(n, (int64_t)*dn) : n;
 }
 
+// Sum of each chunk; a thread adds its kScanItems consecutive entries (16-byte
+// vectors when aligned and in range), as k_scan_down reads them.
 __global__ void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n, const unsigned long long *dn,
                               uint32_t *partial)
 {
     n = scan_len(n, dn);
-    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    const int64_t mine = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
     uint32_t s = 0;
-    for (int r = 0; r < kScanItems; r++) {
-        int64_t i = base + (int64_t)r * kTpb + threadIdx.x;
-        if (i < n) s += in[i];
+    if ((reinterpret_cast<uintptr_t>(in) & 15) == 0 && mine + kScanItems <= n) {
+#pragma unroll
+        for (int r = 0; r < kScanItems / 4; r++) {
+            const uint4 q = reinterpret_cast<const uint4 *>(in + mine)[r];
+            s += (q.x + q.y) + (q.z + q.w);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < kScanItems; r++)
+            if (mine + r < n) s += in[mine + r];
     }
     uint32_t tot;
     block_excl_scan(s, &tot);
@@ -678,14 +687,27 @@ __global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, ui
 // Block minima over 64 entries of a byte array in the cq domain (cpl + 1,
 // values 0..22): level 1 over cq, levels 2 and 3 over the level below, for
 // the range-end search.
+// out[b] = min of in[64 b .. 64 b + 63] (entries past nin ignored): eight
+// lanes per output, one 8-byte load each (in is 8-byte aligned), then a min
+// across the eight.  Launch with 8 * nout threads.
 __global__ void k_cmin(const uint8_t *__restrict__ in, int nin, uint8_t *out, int nout)
 {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nout) return;
-    int lo = b * 64, hi = min(nin, lo + 64);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = t >> 3, lo = b * 64 + 8 * (t & 7);
     int m = 255;
-    for (int j = lo; j < hi; j++) m = min(m, (int)in[j]);
-    out[b] = (uint8_t)m;
+    if (b < nout) {
+        if (lo + 8 <= nin) {
+            const uint64_t w = *reinterpret_cast<const uint64_t *>(in + lo);
+#pragma unroll
+            for (int j = 0; j < 8; j++) m = min(m, (int)((w >> (8 * j)) & 255));
+        } else {
+            for (int j = lo; j < nin; j++) m = min(m, (int)in[j]);
+        }
+    }
+    m = min(m, __shfl_xor(m, 1, 64));
+    m = min(m, __shfl_xor(m, 2, 64));
+    m = min(m, __shfl_xor(m, 4, 64));
+    if (b < nout && (t & 7) == 0) out[b] = (uint8_t)m;
 }
 
 // Flags (the high bit of a byte) of the entries k, k+1, .. of the aligned
@@ -1556,9 +1578,9 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     auto words = [](int v) { return (v + 15) & ~7; };
     uint8_t *m1 = reinterpret_cast<uint8_t *>(ensure<int8_t>(bm1, words(n1) + words(n2) + words(n3)));
     uint8_t *m2 = m1 + words(n1), *m3 = m2 + words(n2);
-    k_cmin<<<nblocks(n1), kTpb, 0, st>>>(cq, n, m1, n1);
-    k_cmin<<<nblocks(n2), kTpb, 0, st>>>(m1, n1, m2, n2);
-    k_cmin<<<nblocks(n3), kTpb, 0, st>>>(m2, n2, m3, n3);
+    k_cmin<<<nblocks(8 * (int64_t)n1), kTpb, 0, st>>>(cq, n, m1, n1);
+    k_cmin<<<nblocks(8 * (int64_t)n2), kTpb, 0, st>>>(m1, n1, m2, n2);
+    k_cmin<<<nblocks(8 * (int64_t)n3), kTpb, 0, st>>>(m2, n2, m3, n3);
     t.first = ensure<int>(bfirst, cap);
     t.last = ensure<int>(blast, cap);
     t.level = ensure<uint8_t>(blevel, cap);
@@ -1841,9 +1863,16 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     // cells bucketed by level (side stream) beside the child slots (main)
     uint32_t *lcount = lwork, *loffd = lwork + (kLevels + 1), *lcursor = lwork + 2 * (kLevels + 1);
     int *llist = ensure<int>(ctx->llist, cap);
+    const int lsys = ctx->sysbits / 3;
     side_fork(ctx, kForkTopo);
     hipStream_t sd = ctx->side;
-    if (!prefix_moments) {
+    if (prefix_moments) {
+        // the moments on the second stream (after its prefix scan), beside
+        // the child slots and the groups on the main stream
+        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, sd>>>(cfirst, clast, clevel, tt.ncellp, mpre, ks, mo, cA, cB,
+                                                            d_box, lsys, ctx->sysbits, sysbox,
+                                                            nsys ? ptr<int64_t>(ctx->boff) : nullptr);
+    } else {
         k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
         k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount, loffd, lcursor, llist);
     }
@@ -1856,28 +1885,27 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.skip = ptr<int>(ctx->cskip);
     T.sub = csub;
     T.nkids = cnkids;
-    // the groups (side stream, after the level lists there) need the child
-    // slots and the topology only; the moments need the gathered bodies and
-    // the level lists
+    // the groups need the child slots and the topology only: on the second
+    // stream (after the level lists there) beside the level-by-level
+    // moments, or on the main stream while the second computes the moments
+    // from the prefix sums
     const bool groups = p.mode == BH_MODE_FAST;
-    if (groups) {
+    const bool groups_side = groups && !prefix_moments;
+    if (groups_side) {
         side_fork(ctx, kForkKids);
         build_groups_side(ctx, p, &tt, ks, tks_for_groups);
         side_mark(ctx, kMarkGroups);
+    } else if (groups) {
+        build_groups(ctx, p, &tt, ks, tks_for_groups);
     }
     main_wait(ctx, kMarkGather);
     main_wait(ctx, kMarkLevels);
     const int mgrid = ctx->ncu * 4;
-    const int lsys = ctx->sysbits / 3;
     // the learned features are written only for the learned walk and sampling
     const bool feat = p.learned != 0 || ctx->mlp_sampling;
     // the learned features carry the quadrupole (the leading error of a
     // monopole cell), so a learned or sampling build computes it whatever
     // the walk's cells are
-    if (prefix_moments)
-        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, st>>>(cfirst, clast, clevel, tt.ncellp, mpre, ks, mo, cA, cB,
-                                                            d_box, lsys, ctx->sysbits, sysbox,
-                                                            nsys ? ptr<int64_t>(ctx->boff) : nullptr);
     for (int L = prefix_moments ? -1 : kMaxLevel; L >= 0; L--) {
         // level L holds at most 8^L cells: the top levels need a block or two
         const int lgrid = L <= 3 ? nblocks((int64_t)1 << (3 * L)) : mgrid * 2;
@@ -1904,7 +1932,7 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     T.cQ1 = cQ1;
     T.cF0 = cF0;
     T.cF1 = cF1;
-    if (groups) main_wait(ctx, kMarkGroups);
+    if (groups_side) main_wait(ctx, kMarkGroups);
 }
 
 }  // namespace bh
