@@ -196,17 +196,31 @@ def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.9, 1.1, 1.3), stri
             "t_collect_s": t1 - t0, "t_train_s": t2 - t1}
 
 
-def profiled_traffic(kernel, profile="walk_plummer4M_plain_mono.md"):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a device copy of known
-    size), when that summary's dominant kernel is this kernel; else None.
-    Counters cannot be read from inside the timed process, so the figure is
-    the one measured on the same workload by the profile's command."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", profile)
+def profiled_traffic(kernel, profiles=None):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    summaries of the N = 4M walks (profiles/walk_plummer4M_*.md: FETCH_SIZE
+    and WRITE_SIZE runs, each scaled by a device copy of known size), from the
+    first summary whose dominant kernel is this kernel; else None.  Counters
+    cannot be read from inside the timed process, so the figure is the one
+    measured on the same workload by the profile's command."""
+    pdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if profiles is None:
+        try:
+            profiles = sorted(f for f in os.listdir(pdir) if f.startswith("walk_plummer4M_") and f.endswith(".md"))
+        except OSError:
+            return None, None
+    for prof in profiles:
+        got = _profile_traffic(os.path.join(pdir, prof), kernel)
+        if got is not None:
+            return got, "profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; scaled)" % prof
+    return None, None
+
+
+def _profile_traffic(path, kernel):
     try:
         text = open(path).read()
     except OSError:
-        return None, None
+        return None
     want = kernel.replace(" ", "")
     dom, in_traffic = None, False
     for line in text.splitlines():
@@ -218,10 +232,10 @@ def profiled_traffic(kernel, profile="walk_plummer4M_plain_mono.md"):
                 and line.split("`")[1].replace(" ", "") == want):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:
-                return float(cells[-1]), "profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; scaled)" % profile
+                return float(cells[-1])
             except ValueError:
-                return None, None
-    return None, None
+                return None
+    return None
 
 
 def walk_roofline(n_int, t_walk_s, kernel):
@@ -614,8 +628,7 @@ def main():
             "interactions_by_expansion": int(head_v["n_local"]),
         },
     }
-    traffic, src = profiled_traffic(out["roofline"]["kernel"],
-                                    "walk_plummer4M_local_mono.md" if head_local else "walk_plummer4M_plain_mono.md")
+    traffic, src = profiled_traffic(out["roofline"]["kernel"])
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_source"] = src
