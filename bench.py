@@ -626,6 +626,9 @@ def main():
                                "kernel; cell interactions taken by group local expansions are not counted)",
             "interactions_evaluated": int(tile_int),
             "interactions_by_expansion": int(head_v["n_local"]),
+            # the rate a walk evaluating every interaction one by one would need
+            # to match this one (not an achieved flop rate)
+            "equivalent_tflops_all_interactions": FLOP_PER_INTERACTION * head_v["interactions_per_step"] / t_walk / 1e12,
         },
     }
     traffic, src = profiled_traffic(out["roofline"]["kernel"])
