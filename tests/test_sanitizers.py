@@ -60,6 +60,7 @@ def test_oracle_under_sanitizers(san_env):
     # preloaded, and leak checking is left to the C programs above
     env = dict(san_env, NEMO_ORACLE_LIB=os.path.join(ROOT, "oracle", "_san", "liboracle.so"), LD_PRELOAD=_libasan(),
                ASAN_OPTIONS=san_env["ASAN_OPTIONS"] + ":detect_leaks=0")
-    r = _pytest(env, "tests/test_oracle.py", "tests/test_tc_oracle.py", "tests/test_mlp_oracle.py")
+    r = _pytest(env, "tests/test_oracle.py", "tests/test_tc_oracle.py", "tests/test_mlp_oracle.py",
+                "tests/test_lc_oracle.py", "tests/test_le_oracle.py")
     assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
     assert "passed" in r.stdout and "runtime error" not in r.stderr
