@@ -66,20 +66,33 @@ def test_graph_replay_equals_eager(eng, mode, criterion):
 
 
 def test_fast_leapfrog_energy_and_oracle(eng):
-    """Fast walk + leapfrog: close to the oracle's trajectory and energy."""
+    """Fast walk + leapfrog against the oracle's FP64 per-body walk: energy
+    and, for positions, a bound derived from the two walks' force difference.
+    A constant force difference da displaces a leapfrog orbit by about
+    n^2 dt^2 / 2 * da after n steps; the test allows twice that (n^2 dt^2 da)
+    on the median body after 4 and after 33 steps, with da measured at the
+    start, where both walks see the same positions."""
     m, x, v = E.mkplummer(4000, seed=21)
-    lf = E.Leapfrog(eng, m, x, v, 1.0 / 32.0, tol=0.7, eps=0.05, usequad=1)
-    d, n2b, nbc = lf.run(33, graph=True)
+    dt = 1.0 / 32.0
+    lf = E.Leapfrog(eng, m, x, v, dt, tol=0.7, eps=0.05, usequad=1)
     ref = oracle.Leapfrog(m, x, v, tol=0.7, eps=0.05, usequad=True, fcells=2.0)
-    for _ in range(33):
-        rd, _, _ = ref.step()
-    e0, e1 = d[0, 0], d[-1, 0]
+    d0, _, _ = lf.run(1, graph=True)
+    ref.step()
+    a0 = lf.state()[2]
+    da = np.median(np.linalg.norm(a0 - ref.acc, axis=1))
+    # the two walks differ at the level of their force errors, not more
+    assert 0 < da < 2e-3 * np.median(np.linalg.norm(ref.acc, axis=1))
+    done = 1
+    for n in (4, 33):
+        d, _, _ = lf.run(n - done, graph=True)
+        for _ in range(n - done):
+            rd, _, _ = ref.step()
+        done = n
+        dx = np.median(np.linalg.norm(lf.state()[0] - ref.pos, axis=1))
+        assert dx < n * n * dt * dt * da, (n, dx, da)
+    e0, e1 = d0[0, 0], d[-1, 0]
     assert abs(e1 - e0) / abs(e0) < 2e-3
     assert abs(d[-1, 0] - rd[0]) / abs(rd[0]) < 1e-3
-    pos = lf.state()[0]
-    # different force errors (FP32 group walk vs FP64 per-particle walk) in a
-    # chaotic system: the trajectories agree to the force-error level
-    assert np.median(np.linalg.norm(pos - ref.pos, axis=1)) < 5e-3
 
 
 def test_upload_ends_the_leapfrog_state():
