@@ -58,7 +58,7 @@ struct bh_ctx {
     bh::DevBuf cfirst, clast, clevel, csub, cnkids, cskip, pre, cowner, gowner;
     bh::DevBuf crc2;            // treecode1 Rcrit2 per cell (criterion != BH_CRIT_HACK)
     bh::DevBuf cmo, cfnb, cA, cB, cQ0, cQ1, lhist, llist, lkeys0, lkeys1, lvals0, lvals1;
-    bh::DevBuf mpre, mpart;     // fast monopole trees: prefix sums of (m x, m y, m z, m) over the sorted bodies
+    bh::DevBuf mpre, mpart;     // fast monopole trees: tile-local prefix sums of (m x, m y, m z, m), tile prefixes
     bh::DevBuf counters;        // u64: n2b, nbc, deep, n_mlp, err flags ...
     // groups of the test set (fast walk)
     bh::DevBuf gfirst, gcount, gscan, goff;

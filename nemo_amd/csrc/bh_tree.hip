@@ -610,32 +610,6 @@ __device__ __forceinline__ void split2(double x, float &hi, float &lo)
     lo = (float)(x - (double)hi);
 }
 
-// pk: packed (x, y, z, m) records in input order, or nullptr to read pos/mass.
-__global__ void k_gather_tree(const uint32_t *__restrict__ perm, int nb, const double *__restrict__ pos,
-                              const double *__restrict__ mass, const double4 *__restrict__ pk, double4 *bd,
-                              float4 *bhi, float4 *blo)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb) return;
-    int64_t o = perm[s];
-    double x, y, z, m;
-    if (pk) {
-        const double4 r = pk[o];
-        x = r.x, y = r.y, z = r.z, m = r.w;
-    } else {
-        x = pos[3 * o], y = pos[3 * o + 1], z = pos[3 * o + 2], m = mass[o];
-    }
-    bd[s] = make_double4(x, y, z, m);
-    float4 h, l;
-    split2(x, h.x, l.x);
-    split2(y, h.y, l.y);
-    split2(z, h.z, l.z);
-    h.w = (float)m;
-    l.w = 0.f;
-    bhi[s] = h;
-    blo[s] = l;
-}
-
 // Test set in its sorted order.  tmap (input index -> tree body or -1, NULL
 // when no test point is a tree body) gives tself[s], the tree body of test
 // point s, and tinv, the test index of each tree body.
@@ -674,14 +648,33 @@ __global__ void k_tree_index(const uint32_t *__restrict__ vs, int64_t n, int nb,
 // cpl and its copy cq = cpl + 1 (0..22, for the 8-entry range scans).
 __global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, uint8_t *cq, uint32_t *ccnt)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb) return;
-    int c = -1, cp = -1;
-    if (s + 1 < nb) c = min(kMaxLevel, (__clzll(keys[s] ^ keys[s + 1]) - 1) / 3);
-    if (s > 0) cp = min(kMaxLevel, (__clzll(keys[s - 1] ^ keys[s]) - 1) / 3);
-    cpl[s] = (int8_t)c;
-    cq[s] = (uint8_t)(c + 1);
-    ccnt[s] = (uint32_t)max(0, c - cp);
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += gridDim.x * blockDim.x) {
+        int c = -1, cp = -1;
+        if (s + 1 < nb) c = min(kMaxLevel, (__clzll(keys[s] ^ keys[s + 1]) - 1) / 3);
+        if (s > 0) cp = min(kMaxLevel, (__clzll(keys[s - 1] ^ keys[s]) - 1) / 3);
+        cpl[s] = (int8_t)c;
+        cq[s] = (uint8_t)(c + 1);
+        ccnt[s] = (uint32_t)max(0, c - cp);
+    }
+}
+
+// Cells needed = cbase[n]; the count used is clamped to the capacity and an
+// overflow is flagged (the host then grows the arrays and evaluates again).
+struct CellClamp {
+    const uint32_t *cbase;
+    int n;
+    unsigned long long cap;
+    unsigned long long *stat;
+    int slot, slot_raw;
+    unsigned long long flag;
+};
+
+__device__ __forceinline__ void clamp_cells(const CellClamp &c)
+{
+    const unsigned long long raw = c.n >= 2 ? c.cbase[c.n] : 0;
+    c.stat[c.slot_raw] = raw;
+    c.stat[c.slot] = raw < c.cap ? raw : c.cap;
+    if (raw > c.cap) atomicOr(&c.stat[S_FLAGS], c.flag);
 }
 
 // Block minima over 64 entries of a byte array in the cq domain (cpl + 1,
@@ -689,10 +682,12 @@ __global__ void k_cpl(const uint64_t *__restrict__ keys, int nb, int8_t *cpl, ui
 // the range-end search.
 // out[b] = min of in[64 b .. 64 b + 63] (entries past nin ignored): eight
 // lanes per output, one 8-byte load each (in is 8-byte aligned), then a min
-// across the eight.  Launch with 8 * nout threads.
-__global__ void k_cmin(const uint8_t *__restrict__ in, int nin, uint8_t *out, int nout)
+// across the eight.  Launch with 8 * nout threads.  cc (first pyramid level
+// only): thread 0 also sets the cell count.
+__global__ void k_cmin(const uint8_t *__restrict__ in, int nin, uint8_t *out, int nout, CellClamp cc = CellClamp{})
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cc.stat && t == 0) clamp_cells(cc);
     const int b = t >> 3, lo = b * 64 + 8 * (t & 7);
     int m = 255;
     if (b < nout) {
@@ -765,10 +760,11 @@ __device__ int range_end(const uint8_t *__restrict__ cq, const uint8_t *__restri
 // Owner body of every cell: body s starts the cells cbase[s] .. cbase[s+1]-1.
 __global__ void k_owner(const uint32_t *__restrict__ cbase, int nb, const unsigned long long *ncellp, int *owner)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb) return;
-    const int id1 = min((int)cbase[s + 1], (int)*ncellp);
-    for (int id = (int)cbase[s]; id < id1; id++) owner[id] = s;
+    const int ncell = (int)*ncellp;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += gridDim.x * blockDim.x) {
+        const int id1 = min((int)cbase[s + 1], ncell);
+        for (int id = (int)cbase[s]; id < id1; id++) owner[id] = s;
+    }
 }
 
 // One thread per cell (grid-stride over the device cell count).  The cells
@@ -807,20 +803,9 @@ __global__ void k_cells(const int8_t *__restrict__ cpl, const uint8_t *__restric
 // Pre-order position of body s: after the cells that start at s.
 __global__ void k_body_pre(const uint32_t *__restrict__ cbase, int nb, const unsigned long long *ncellp, int *pre)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb || cbase[s + 1] > (uint32_t)*ncellp) return;
-    pre[s + cbase[s + 1]] = s;
-}
-
-// Cells needed = cbase[n]; the count used is clamped to the capacity and an
-// overflow is flagged (the host then grows the arrays and evaluates again).
-__global__ void k_clamp_cells(const uint32_t *__restrict__ cbase, int n, unsigned long long cap,
-                              unsigned long long *stat, int slot, int slot_raw, unsigned long long flag)
-{
-    unsigned long long raw = n >= 2 ? cbase[n] : 0;
-    stat[slot_raw] = raw;
-    stat[slot] = raw < cap ? raw : cap;
-    if (raw > cap) atomicOr(&stat[S_FLAGS], flag);
+    const uint32_t ncell = (uint32_t)*ncellp;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nb; s += gridDim.x * blockDim.x)
+        if (cbase[s + 1] <= ncell) pre[s + cbase[s + 1]] = s;
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -1296,19 +1281,62 @@ __device__ double4 block_excl_scan(double4 v, double4 *tot)
     return d4add(off, make_double4(x.x - v.x, x.y - v.y, x.z - v.z, x.w - v.w));
 }
 
-// Per tile: the sum of its bodies' (m x, m y, m z, m).
-__global__ __launch_bounds__(kPreThr) void k_pre_reduce(const double4 *__restrict__ bd, int n,
-                                                        const int2 *__restrict__ tiles, double4 *part)
+// Gather of the mass set into sorted order: bd (FP64 x, y, z, m) and the
+// split FP32 records bhi/blo.  pk: packed (x, y, z, m) records in input
+// order, or nullptr to read pos/mass.  One block per tile of kPreTile sorted
+// bodies (blockIdx.x, blockIdx.x + gridDim.x, ...); in round k thread j takes
+// body k * kPreThr + j of the tile (coalesced), all rounds' loads issued
+// together.  loc (prefix-moment builds): the tile-local inclusive sums of
+// (m x, m y, m z, m), one block scan per round, and part[t] the tile's total,
+// from the records just gathered (no second pass over bd).
+__global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restrict__ perm, int nb,
+                                                         const double *__restrict__ pos, const double *__restrict__ mass,
+                                                         const double4 *__restrict__ pk, double4 *bd, float4 *bhi,
+                                                         float4 *blo, const int2 *__restrict__ tiles, int ntile,
+                                                         double4 *loc, double4 *part)
 {
-    const int2 r = pre_tile(tiles, blockIdx.x, n);
-    const int base = r.x + threadIdx.x * kPrePer;
-    double4 s = make_double4(0, 0, 0, 0);
+    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+        const int2 r = pre_tile(tiles, t, nb);
+        double4 v[kPrePer];
 #pragma unroll
-    for (int k = 0; k < kPrePer; k++)
-        if (base + k < r.y) s = d4add(s, body_mom(bd[base + k]));
-    double4 tot;
-    block_excl_scan(s, &tot);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+        for (int k = 0; k < kPrePer; k++) {
+            const int i = r.x + k * kPreThr + (int)threadIdx.x;
+            v[k] = make_double4(0, 0, 0, 0);
+            if (i < r.y) {
+                const int64_t o = perm[i];
+                if (pk) {
+                    v[k] = pk[o];
+                } else {
+                    v[k] = make_double4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
+                }
+            }
+        }
+        double4 carry = make_double4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < kPrePer; k++) {
+            const int i = r.x + k * kPreThr + (int)threadIdx.x;
+            const double4 b = v[k];
+            if (i < r.y) {
+                bd[i] = b;
+                float4 h, l;
+                split2(b.x, h.x, l.x);
+                split2(b.y, h.y, l.y);
+                split2(b.z, h.z, l.z);
+                h.w = (float)b.w;
+                l.w = 0.f;
+                bhi[i] = h;
+                blo[i] = l;
+            }
+            if (loc) {
+                const double4 mm = body_mom(b);
+                double4 tot;
+                const double4 ex = block_excl_scan(mm, &tot);
+                if (i < r.y) loc[i] = d4add(d4add(carry, ex), mm);
+                carry = d4add(carry, tot);
+            }
+        }
+        if (loc && threadIdx.x == 0) part[t] = carry;
+    }
 }
 
 // Exclusive scan of the tile sums of one system per block.
@@ -1326,35 +1354,20 @@ __global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *_
     }
 }
 
-// inc[i] = the sum over the bodies of i's system up to and including i.
-__global__ __launch_bounds__(kPreThr) void k_pre_down(const double4 *__restrict__ bd, int n,
-                                                      const int2 *__restrict__ tiles,
-                                                      const double4 *__restrict__ part, double4 *inc)
-{
-    const int2 r = pre_tile(tiles, blockIdx.x, n);
-    const int base = r.x + threadIdx.x * kPrePer;
-    double4 v[kPrePer];
-    double4 s = make_double4(0, 0, 0, 0);
-#pragma unroll
-    for (int k = 0; k < kPrePer; k++) {
-        v[k] = base + k < r.y ? body_mom(bd[base + k]) : make_double4(0, 0, 0, 0);
-        s = d4add(s, v[k]);
-    }
-    double4 run = d4add(part[blockIdx.x], block_excl_scan(s, nullptr));
-#pragma unroll
-    for (int k = 0; k < kPrePer; k++) {
-        run = d4add(run, v[k]);
-        if (base + k < r.y) inc[base + k] = run;
-    }
-}
+__device__ __forceinline__ double4 d4sub(double4 a, double4 b) { return make_double4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
 
 // Mass, c.o.m. and the FP32 walk records of every cell from the prefixes
-// (the records cell_moments writes for a monopole build).  In a batch, the
-// cells above the systems' roots are never walked and get no moments.
+// (the records cell_moments writes for a monopole build).  The sums over the
+// cell's bodies [s, e] are loc[e] - loc[s-1] inside one tile, else the
+// difference of the tiles' exclusive prefixes part[] plus that of the
+// tile-local sums.  Tile of body i: seg[sys] + (i - s0) / kPreTile, s0 the
+// system's first body (one system: i / kPreTile).  In a batch, the cells
+// above the systems' roots are never walked and get no moments.
 __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__ cfirst, const int *__restrict__ clast,
                                                          const uint8_t *__restrict__ clevel,
                                                          const unsigned long long *ncellp,
-                                                         const double4 *__restrict__ inc, const uint64_t *__restrict__ keys,
+                                                         const double4 *__restrict__ loc, const double4 *__restrict__ part,
+                                                         const int *__restrict__ seg, const uint64_t *__restrict__ keys,
                                                          double *mo, float4 *cA, float4 *cB, const double4 *__restrict__ box,
                                                          int lsys, int sysbits, const double4 *__restrict__ sysbox,
                                                          const int64_t *__restrict__ boff)
@@ -1363,13 +1376,14 @@ __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         const int s = cfirst[c], e = clast[c];
         const int L = clevel[c];
-        int Lg = L, s0 = 0;
+        int Lg = L, s0 = 0, t0 = 0;
         double rsize = box->w;
         if (sysbox) {
             const int sys = sysbits ? (int)(keys[s] >> (63 - sysbits)) : 0;
             rsize = sysbox[sys].w;
             Lg = L - lsys;
             s0 = (int)boff[sys];
+            t0 = seg[sys];
         }
         double *me = mo + 10 * (int64_t)c;
         if (Lg < 0) {
@@ -1378,10 +1392,17 @@ __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__
             cB[c] = make_float4(0.f, 0.f, 0.f, 3.0e38f);
             continue;
         }
-        const double4 b = inc[e];
-        const double4 a = s > s0 ? inc[s - 1] : make_double4(0, 0, 0, 0);
-        const double M = b.w - a.w;
-        const double px = (b.x - a.x) / M, py = (b.y - a.y) / M, pz = (b.z - a.z) / M;
+        const int tb = t0 + (e - s0) / kPreTile;
+        double4 d = loc[e];
+        if (s > s0) {
+            const int ta = t0 + (s - 1 - s0) / kPreTile;
+            const double4 la = loc[s - 1];
+            d = ta == tb ? d4sub(d, la) : d4add(d4sub(part[tb], part[ta]), d4sub(d, la));
+        } else if (tb != t0) {
+            d = d4add(part[tb], d);
+        }
+        const double M = d.w;
+        const double px = d.x / M, py = d.y / M, pz = d.z / M;
         me[0] = M;
         me[1] = px;
         me[2] = py;
@@ -1506,7 +1527,13 @@ __global__ __launch_bounds__(kTpb) void k_moments_level(const int *__restrict__ 
 
 inline int nblocks(int64_t n, int tpb = kTpb) { return (int)std::max<int64_t>(1, (n + tpb - 1) / tpb); }
 // Grid of the kernels that stride over a device-side count bounded by cap.
-inline int capgrid(const bh_ctx *ctx, int64_t cap) { return (int)std::min<int64_t>(nblocks(cap), (int64_t)ctx->ncu * 8); }
+constexpr int kGatherBlocksPerCU = 8;   // grid of the tree build's gather (tiles of kPreTile bodies), per CU
+constexpr int kBodyBlocksPerCU = 16;    // grid of the per-body topology kernels (grid-stride), per CU
+constexpr int kCapBlocksPerCU = 8;      // grid of the per-cell kernels (grid-stride over the device count), per CU
+inline int capgrid(const bh_ctx *ctx, int64_t cap)
+{
+    return (int)std::min<int64_t>(nblocks(cap), (int64_t)ctx->ncu * kCapBlocksPerCU);
+}
 
 }  // namespace
 
@@ -1569,16 +1596,17 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     uint8_t *cq = reinterpret_cast<uint8_t *>(t.cpl) + ((n + 15) & ~7);   // 8-aligned, n + 8 bytes
     uint32_t *ccnt = ensure<uint32_t>(bcnt, n + 1);
     t.cbase = ensure<uint32_t>(bbase, n + 1);
-    k_cpl<<<nblocks(n), kTpb, 0, st>>>(keys, n, t.cpl, cq, ccnt);
+    const int bgrid = (int)std::min<int64_t>(nblocks(n), (int64_t)ctx->ncu * kBodyBlocksPerCU);
+    k_cpl<<<bgrid, kTpb, 0, st>>>(keys, n, t.cpl, cq, ccnt);
     exclusive_scan_u32(ctx, ccnt, t.cbase, n);
-    k_clamp_cells<<<1, 1, 0, st>>>(t.cbase, n, (unsigned long long)cap, stat, slot, slot_raw, flag);
     int n1 = (n + 63) / 64, n2 = (n1 + 63) / 64, n3 = (n2 + 63) / 64;
     // each level 8-byte aligned with at least one word of padding (the
     // search reads whole words; it stops at the entry holding cpl[n-1] = -1)
     auto words = [](int v) { return (v + 15) & ~7; };
     uint8_t *m1 = reinterpret_cast<uint8_t *>(ensure<int8_t>(bm1, words(n1) + words(n2) + words(n3)));
     uint8_t *m2 = m1 + words(n1), *m3 = m2 + words(n2);
-    k_cmin<<<nblocks(8 * (int64_t)n1), kTpb, 0, st>>>(cq, n, m1, n1);
+    const CellClamp cc{t.cbase, n, (unsigned long long)cap, stat, slot, slot_raw, flag};
+    k_cmin<<<nblocks(8 * (int64_t)n1), kTpb, 0, st>>>(cq, n, m1, n1, cc);
     k_cmin<<<nblocks(8 * (int64_t)n2), kTpb, 0, st>>>(m1, n1, m2, n2);
     k_cmin<<<nblocks(8 * (int64_t)n3), kTpb, 0, st>>>(m2, n2, m3, n3);
     t.first = ensure<int>(bfirst, cap);
@@ -1588,10 +1616,10 @@ static Topo topology(bh_ctx *ctx, const uint64_t *keys, int n, int cap, DevBuf &
     int *pre = bpre ? ensure<int>(*bpre, (size_t)n + cap) : nullptr;
     int2 *fnb = bfnb ? ensure<int2>(*bfnb, cap) : nullptr;
     int *owner = ensure<int>(bowner, cap);
-    k_owner<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, owner);
+    k_owner<<<bgrid, kTpb, 0, st>>>(t.cbase, n, t.ncellp, owner);
     k_cells<<<capgrid(ctx, cap), kTpb, 0, st>>>(t.cpl, cq, t.cbase, owner, m1, m2, m3, t.ncellp, t.first, t.last, t.level,
                                                 cskip, pre, fnb, bskip ? stat + S_DEEP : nullptr);
-    if (pre) k_body_pre<<<nblocks(n), kTpb, 0, st>>>(t.cbase, n, t.ncellp, pre);
+    if (pre) k_body_pre<<<bgrid, kTpb, 0, st>>>(t.cbase, n, t.ncellp, pre);
     BH_HIP(hipGetLastError());
     return t;
 }
@@ -1760,23 +1788,25 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
     float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
-    // the gather (HBM-bound) runs beside the topology (latency-bound); only
-    // the moments and the walk read its records
     // fast monopole builds take the moments from prefix sums of the gathered
-    // bodies (on the side stream, beside the topology)
+    // bodies (on the second stream, beside the topology)
     const bool feat_b = p.learned != 0 || ctx->mlp_sampling;
     const bool prefix_moments = p.mode == BH_MODE_FAST && !p.usequad && !feat_b && nb > 1;
     double4 *mpre = prefix_moments ? ensure<double4>(ctx->mpre, (size_t)nb) : nullptr;
+    // the gather (HBM-bound; with, for prefix-moment builds, the tile-local
+    // sums) on the second stream beside the topology (latency-bound); only
+    // the moments and the walk read its records.  A bounded grid, so that
+    // the main stream's small launches beside it find free slots.  A batch's
+    // prefix tiles restart at every system.
+    const int2 *tiles = prefix_moments && nsys ? ptr<int2>(ctx->ptile) : nullptr;
+    const int ntile = tiles ? ctx->nptile : (int)((nb + kPreTile - 1) / kPreTile);
+    double4 *part = prefix_moments ? ensure<double4>(ctx->mpart, ntile) : nullptr;
     side_fork(ctx, kForkSort);
-    if (nb > 0) k_gather_tree<<<nblocks(nb), kTpb, 0, ctx->side>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo);
-    if (prefix_moments) {
-        const int2 *tiles = nsys ? ptr<int2>(ctx->ptile) : nullptr;
-        const int ntile = nsys ? ctx->nptile : (int)((nb + kPreTile - 1) / kPreTile);
-        double4 *part = ensure<double4>(ctx->mpart, ntile);
-        k_pre_reduce<<<ntile, kPreThr, 0, ctx->side>>>(bd, nb, tiles, part);
+    if (nb > 0)
+        k_gather_tree<<<std::min(ntile, ctx->ncu * kGatherBlocksPerCU), kPreThr, 0, ctx->side>>>(
+            vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile, mpre, part);
+    if (prefix_moments)
         k_pre_top<<<nsys ? nsys : 1, kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr, ntile);
-        k_pre_down<<<ntile, kPreThr, 0, ctx->side>>>(bd, nb, tiles, part, mpre);
-    }
     side_mark(ctx, kMarkGather);
     T.bd = bd;
     T.bhi = bhi;
@@ -1869,7 +1899,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     if (prefix_moments) {
         // the moments on the second stream (after its prefix scan), beside
         // the child slots and the groups on the main stream
-        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, sd>>>(cfirst, clast, clevel, tt.ncellp, mpre, ks, mo, cA, cB,
+        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, sd>>>(cfirst, clast, clevel, tt.ncellp, mpre, part,
+                                                            nsys ? ptr<int>(ctx->pseg) : nullptr, ks, mo, cA, cB,
                                                             d_box, lsys, ctx->sysbits, sysbox,
                                                             nsys ? ptr<int64_t>(ctx->boff) : nullptr);
     } else {
