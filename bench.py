@@ -17,8 +17,11 @@ residual samples of three other Plummer snapshots (seeds 41, 43 and 47, tol
 0.9/1.1/1.3, cells with size/r in [0.45, 1.5]; the targets are the exact
 cell force minus that walk's multipole force at the group centre and its
 Jacobian, applied to each member to first order), then that learned walk is
-calibrated to the same target on the seed-123 snapshot; the faster of the
-two is the learned figure (both are in `config.learned_variants`).
+calibrated to the same target on the seed-123 snapshot; the walk with
+monopole cells also runs with the group local expansion of far cells
+(local_alpha in --learned-local, same model: the correction covers the same
+recorded cells); the fastest is the learned figure (all are in
+`config.learned_variants`).
 `value` is the evals/s of the faster of the two (`config.value_is` names it;
 both variants and their ratio `learned_over_plain` are in `config`); with
 --no-learned it is the plain figure.
@@ -441,6 +444,9 @@ def main():
     ap.add_argument("--learned-quad", type=int, default=-1,
                     help="learned walk cells: 1 quadrupole, 0 monopole, -1 both (each with its own model; the faster "
                          "is the learned figure)")
+    ap.add_argument("--learned-local", default="0.2",
+                    help="learned walk with monopole cells: also with group local expansions at these local_alpha "
+                         "values (comma-separated; 'none' skips), same model")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
@@ -576,6 +582,18 @@ def main():
                 "%.3f ms, %d evaluations)" % (lq, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"],
                                               v["p99_rel_err"], v["ms_per_step"], v["t_mlp_kernel_ms"], v["n_mlp"]))
             learned_variants.append(v)
+            lalphas = [] if lq or args.learned_local.strip().lower() in ("", "none") else \
+                [float(a) for a in args.learned_local.split(",")]
+            for alpha in lalphas:
+                v = variant(dict(eps=args.eps, usequad=0, group_size=args.group, nleaf=args.nleaf, learned=1,
+                                 local_alpha=alpha), args.learned_tol)
+                v["training"] = tr
+                log("learned walk + local expansions (alpha %.3f): group %d nleaf %d tol=%.4f median=%.3g p99=%.3g  "
+                    "%.3f ms/step (MLP kernel %.3f ms, %d evaluations; %.1f %% of the cell interactions by "
+                    "expansion)" % (alpha, v["group_size"], v["nleaf"], v["tol"], v["median_rel_err"],
+                                    v["p99_rel_err"], v["ms_per_step"], v["t_mlp_kernel_ms"], v["n_mlp"],
+                                    100.0 * v["n_local"] / max(v["nbc"], 1)))
+                learned_variants.append(v)
     learned = min(learned_variants, key=lambda v: v["ms_per_step"]) if learned_variants else None
     learned_train = learned["training"] if learned else None
     # value: the faster of the two at the target median error (both are in

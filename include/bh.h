@@ -83,7 +83,8 @@ typedef struct {
     int supergroup;      /* fast mode: K consecutive groups share one traversal (1 = off, 2..4;
                             0 = tuned default); the interactions are those of K = 1.  Used by the
                             plain single-system walk with the hackcode1 criterion */
-    double local_alpha;  /* fast mode, monopole cells, no learned correction: an accepted cell whose
+    double local_alpha;  /* fast mode, monopole cells (also under the learned correction, whose
+                            residuals are added for the cells in its range either way): an accepted cell whose
                             c.o.m. lies farther than R/local_alpha from the group centre (R: the
                             group's radius about that centre) enters the group's second-order local
                             expansion (the cell's softened monopole potential expanded to third

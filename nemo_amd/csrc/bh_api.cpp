@@ -105,8 +105,8 @@ void check_params(const bh_params *p)
         throw ApiFail(BH_ERR_ARG, "the learned correction is trained for the hackcode1 criterion only");
     if (!(p->local_alpha >= 0.0 && p->local_alpha <= 0.5))
         throw ApiFail(BH_ERR_ARG, "local_alpha must lie in [0, 0.5]");
-    if (p->local_alpha > 0.0 && p->mode == BH_MODE_FAST && (p->usequad || p->learned))
-        throw ApiFail(BH_ERR_ARG, "local expansions take monopole cells without the learned correction");
+    if (p->local_alpha > 0.0 && p->mode == BH_MODE_FAST && p->usequad)
+        throw ApiFail(BH_ERR_ARG, "local expansions take monopole cells");
 }
 
 void lf_drop_graph(bh_ctx *ctx)

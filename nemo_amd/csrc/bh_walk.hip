@@ -375,7 +375,7 @@ struct WalkArgs {
 template <int NPL, bool QUAD, bool SOFT0, bool LREC, bool CRIT, bool LOCAL>
 __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(!QUAD && !LREC && NPL == 3 ? 4 : 1))) void k_walk(WalkArgs a)
 {
-    static_assert(!(LOCAL && (QUAD || LREC)), "local expansions take monopole cells, no records");
+    static_assert(!(LOCAL && QUAD), "local expansions take monopole cells");
     constexpr int G = 64 * NPL;
     constexpr int S = QUAD ? 3 : 1;
     __shared__ float4 sP[kWaves][kListCap];
@@ -577,6 +577,10 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(!QUAD && !
                     }
                     bfirst = small ? fn.x : -1;
                 }
+                // the learned record keeps every accepted cell in its range,
+                // also those the expansion takes (their residual is added on
+                // top of whichever evaluates their monopole)
+                const bool accepted = accept;
                 // far accepted cells -> the local expansion's list
                 if (LOCAL) {
                     const bool far = accept && lrg2 < a.lalpha2 * fmaf(rx, rx, fmaf(ry, ry, rz * rz));
@@ -605,8 +609,8 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(!QUAD && !
                 // accepted cells in the learned range -> this group's record
                 if (LREC && lrecord) {
                     const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-                    const float q = accept ? a.cB[c].w : 0.f;
-                    const bool take = accept && q >= a.lr_lo2 * r2 && q <= a.lr_hi2 * r2;
+                    const float q = accepted ? a.cB[c].w : 0.f;
+                    const bool take = accepted && q >= a.lr_lo2 * r2 && q <= a.lr_hi2 * r2;
                     const unsigned long long m = __ballot(take);
                     const int at = lhead + 5 + lcnt;
                     if (at + 64 <= a.lcap) {
@@ -1238,7 +1242,8 @@ template <int NPL, bool QUAD, bool SOFT0>
 void launch(bh_ctx *ctx, WalkArgs &a, int grid, bool crit)
 {
     if (a.lalpha2 > 0.f && !QUAD) {
-        if (crit) k_walk<NPL, false, SOFT0, false, true, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+        if (a.lmode) k_walk<NPL, false, SOFT0, true, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
+        else if (crit) k_walk<NPL, false, SOFT0, false, true, true><<<grid, kTpb, 0, ctx->stream>>>(a);
         else k_walk<NPL, false, SOFT0, false, false, true><<<grid, kTpb, 0, ctx->stream>>>(a);
     } else if (crit) k_walk<NPL, QUAD, SOFT0, false, true, false><<<grid, kTpb, 0, ctx->stream>>>(a);
     else if (a.lmode) k_walk<NPL, QUAD, SOFT0, true, false, false><<<grid, kTpb, 0, ctx->stream>>>(a);
@@ -1312,7 +1317,9 @@ void walk_fast(bh_ctx *ctx, const bh_params &p, int lmode, int lstride)
     // super-groups: the plain walk of a single system with the hackcode1
     // criterion (learned recordings, treecode1 criteria, batches and local
     // expansions walk group by group)
-    a.lalpha2 = (lmode == 0 && !p.usequad) ? (float)(p.local_alpha * p.local_alpha) : 0.f;
+    // local expansions: the plain walk and the learned walk (lmode 1), not
+    // the sample walks
+    a.lalpha2 = (lmode <= 1 && !p.usequad) ? (float)(p.local_alpha * p.local_alpha) : 0.f;
     a.K = (lmode == 0 && p.criterion == BH_CRIT_HACK && ctx->nsys == 0 && a.lalpha2 == 0.f)
               ? std::max(1, std::min(p.supergroup, kMaxK))
               : 1;

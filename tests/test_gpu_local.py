@@ -203,7 +203,7 @@ def test_local_parameter_checks(eng):
     m, x, _ = E.mkplummer(2000, seed=66, with_vel=False)
     eng.upload(m, x)
     for bad in (dict(local_alpha=-0.1), dict(local_alpha=0.6), dict(local_alpha=float("nan")),
-                dict(local_alpha=0.2, usequad=1), dict(local_alpha=0.2, learned=1)):
+                dict(local_alpha=0.2, usequad=1)):
         with pytest.raises(E.BHError) as ei:
             eng.eval(E.default_params(tol=0.7, eps=0.01, **bad))
         assert ei.value.code == 1, bad
@@ -215,3 +215,47 @@ def test_local_parameter_checks(eng):
     eng.upload(*E.mkplummer(20000, seed=67, with_vel=False)[:2])
     best, _ = eng.autotune(E.default_params(tol=0.7, eps=0.01, local_alpha=0.15), reps=1)
     assert best.supergroup == 1 and best.local_alpha == 0.15
+
+
+def test_learned_walk_with_local_expansions(eng):
+    """The learned walk with local expansions: the record (the cells the
+    model corrects) and the model's correction are those of the learned walk
+    without them, while the far cells' monopole part is the expansion's (the
+    zero-output model isolates it, against the plain local walk's error)."""
+    n, eps = 30000, 0.01
+    m, x, _ = E.mkplummer(n, seed=53, with_vel=False)
+    eng.upload(m, x)
+    eng.mlp_init(11)
+    P, _ = eng.mlp_get()
+    norm = np.array([2e-3, 0.3, 1.5, 3e-3], np.float32)
+    base = dict(tol=1.0, eps=eps, usequad=0, group_size=128, nleaf=1, learned=1)
+    out = {}
+    for alpha in (0.0, 0.2):
+        eng.mlp_set(P, norm)
+        st = eng.eval(E.default_params(local_alpha=alpha, **base))
+        _, a1 = eng.download()
+        ecell, egrp, _, _ = eng.debug_learned()
+        eng.mlp_set(np.zeros_like(P), norm)
+        st0 = eng.eval(E.default_params(local_alpha=alpha, **base))
+        _, a0 = eng.download()
+        assert (st.n2btot, st.nbctot, st.n_mlp) == (st0.n2btot, st0.nbctot, st0.n_mlp)
+        out[alpha] = (st, a1, a0, ecell, egrp)
+    s0, a1p, a0p, c0, g0 = out[0.0]
+    s1, a1l, a0l, c1, g1 = out[0.2]
+    assert s0.n_local == 0 and s1.n_local > 0
+    assert (s0.n2btot, s0.nbctot, s0.n_mlp) == (s1.n2btot, s1.nbctot, s1.n_mlp) and s1.n_mlp > 10000
+    # the same (group, cell) entries (the order of the groups' regions
+    # follows the waves' dynamic scheduling)
+    k0, k1 = np.lexsort((c0, g0)), np.lexsort((c1, g1))
+    assert np.array_equal(c0[k0], c1[k1]) and np.array_equal(g0[k0], g1[k1])
+    # the same correction per member, to the rounding of the FP64 sums
+    scale = np.abs(a0p).max()
+    assert np.abs((a1l - a0l) - (a1p - a0p)).max() < 1e-10 * scale
+    assert np.abs(a1p - a0p).max() > 1e-6 * scale
+    # the expansion moves the result by less than the walk's own error
+    _, ad = eng.direct(eps, np.arange(0, n, 7))
+    ref = np.linalg.norm(ad, axis=1)
+    dl = np.linalg.norm(a0l[::7] - a0p[::7], axis=1) / ref
+    dw = np.linalg.norm(a0p[::7] - ad, axis=1) / ref
+    print("median shift by the expansion %.3e, median walk error %.3e" % (np.median(dl), np.median(dw)))
+    assert 0 < np.median(dl) < np.median(dw)
