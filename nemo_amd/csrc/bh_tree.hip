@@ -1074,6 +1074,59 @@ __device__ void children_of(int c, const uint64_t *__restrict__ keys, const int8
 // are processed bottom-up).  Accumulation follows
 // hackcofm's order: children in octant order, FP64, no FMA (this file is
 // compiled with -ffp-contract=off).
+// The records of cell c from its FP64 mass, c.o.m. and (QUAD) traceless
+// quadrupole q (xx xy xz yy yz zz): the FP64 moments (10 per cell), the split
+// FP32 c.o.m. with the mass, the size^2 (infinite above a batch's system
+// roots, Lg < 0), the FP32 quadrupole (cQ0 non-null) and the learned
+// features (cF0 non-null): c.o.m. offset from the geometric centre and the
+// quadrupole, in units of the cell size.
+template <bool QUAD>
+__device__ __forceinline__ void cell_records(int c, double M, double px, double py, double pz, const double *q, int L,
+                                             int Lg, double rsize, double4 box, const uint64_t *__restrict__ keys,
+                                             const int *__restrict__ cfirst, double *mo, float4 *cA, float4 *cB,
+                                             float4 *cQ0, float4 *cQ1, float4 *cF0, float4 *cF1)
+{
+    double *me = mo + 10 * (int64_t)c;
+    me[0] = M;
+    me[1] = px;
+    me[2] = py;
+    me[3] = pz;
+    if (QUAD) {
+#pragma unroll
+        for (int u = 0; u < 6; u++) me[4 + u] = q[u];
+    }
+    float4 A, B;
+    split2(px, A.x, B.x);
+    split2(py, A.y, B.y);
+    split2(pz, A.z, B.z);
+    A.w = (float)M;
+    B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
+    cA[c] = A;
+    cB[c] = B;
+    if (QUAD && cQ0) {
+        cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+        cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
+    }
+    if (cF0 && Lg >= 0) {
+        // geometric centre of the cell from the key prefix of its first body
+        const uint64_t key = keys[cfirst[c]];
+        uint64_t ix = 0, iy = 0, iz = 0;
+        for (int l = L - Lg; l < L; l++) {
+            const int sh = 60 - 3 * l;
+            ix = (ix << 1) | ((key >> (sh + 2)) & 1);
+            iy = (iy << 1) | ((key >> (sh + 1)) & 1);
+            iz = (iz << 1) | ((key >> sh) & 1);
+        }
+        const double size = rsize * ldexp(1.0, -Lg);
+        const double gx = box.x + ((double)ix + 0.5) * size, gy = box.y + ((double)iy + 0.5) * size,
+                     gz = box.z + ((double)iz + 0.5) * size;
+        const double qn = 1.0 / (M * size * size);
+        cF0[c] = make_float4((float)((px - gx) / size), (float)((py - gy) / size), (float)((pz - gz) / size),
+                             (float)(q[4] * qn));
+        cF1[c] = make_float4((float)(q[0] * qn), (float)(q[3] * qn), (float)(q[1] * qn), (float)(q[2] * qn));
+    }
+}
+
 template <bool QUAD>
 __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__restrict__ cfirst,
                              const int *__restrict__ clast, const uint8_t *__restrict__ clevel,
@@ -1186,45 +1239,7 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
             }
         }
     }
-    double *me = mo + 10 * (int64_t)c;
-    me[0] = M;
-    me[1] = px;
-    me[2] = py;
-    me[3] = pz;
-    if (QUAD) {
-#pragma unroll
-        for (int u = 0; u < 6; u++) me[4 + u] = q[u];
-    }
-    float4 A, B;
-    split2(px, A.x, B.x);
-    split2(py, A.y, B.y);
-    split2(pz, A.z, B.z);
-    A.w = (float)M;
-    B.w = Lg >= 0 ? (float)(rsize * rsize * ldexp(1.0, -2 * Lg)) : 3.0e38f;
-    cA[c] = A;
-    cB[c] = B;
-    if (QUAD && cQ0) {
-        cQ0[c] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
-        cQ1[c] = make_float4((float)q[4], (float)q[5], 0.f, 0.f);
-    }
-    if (cF0 && Lg >= 0) {
-        // geometric centre of the cell from the key prefix of its first body
-        const uint64_t key = keys[cfirst[c]];
-        uint64_t ix = 0, iy = 0, iz = 0;
-        for (int l = L - Lg; l < L; l++) {
-            const int sh = 60 - 3 * l;
-            ix = (ix << 1) | ((key >> (sh + 2)) & 1);
-            iy = (iy << 1) | ((key >> (sh + 1)) & 1);
-            iz = (iz << 1) | ((key >> sh) & 1);
-        }
-        const double size = rsize * ldexp(1.0, -Lg);
-        const double gx = box.x + ((double)ix + 0.5) * size, gy = box.y + ((double)iy + 0.5) * size,
-                     gz = box.z + ((double)iz + 0.5) * size;
-        const double qn = 1.0 / (M * size * size);
-        cF0[c] = make_float4((float)((px - gx) / size), (float)((py - gy) / size), (float)((pz - gz) / size),
-                             (float)(q[4] * qn));
-        cF1[c] = make_float4((float)(q[0] * qn), (float)(q[3] * qn), (float)(q[1] * qn), (float)(q[2] * qn));
-    }
+    cell_records<QUAD>(c, M, px, py, pz, q, L, Lg, rsize, box, keys, cfirst, mo, cA, cB, cQ0, cQ1, cF0, cF1);
 }
 
 // ---- fast monopole trees: moments from prefix sums --------------------------
@@ -1288,12 +1303,14 @@ __device__ double4 block_excl_scan(double4 v, double4 *tot)
 // body k * kPreThr + j of the tile (coalesced), all rounds' loads issued
 // together.  loc (prefix-moment builds): the tile-local inclusive sums of
 // (m x, m y, m z, m), one block scan per round, and part[t] the tile's total,
-// from the records just gathered (no second pass over bd).
+// from the records just gathered (no second pass over bd).  nvec = 3 (cells
+// with the quadrupole): also the second moments m (xx, xy, xz, yy) and
+// m (yz, zz, 0, 0), as vectors 1 and 2 (loc[v nb + i], part[v ntile + t]).
 __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restrict__ perm, int nb,
                                                          const double *__restrict__ pos, const double *__restrict__ mass,
                                                          const double4 *__restrict__ pk, double4 *bd, float4 *bhi,
                                                          float4 *blo, const int2 *__restrict__ tiles, int ntile,
-                                                         double4 *loc, double4 *part)
+                                                         double4 *loc, double4 *part, int nvec)
 {
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
         const int2 r = pre_tile(tiles, t, nb);
@@ -1311,7 +1328,7 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
                 }
             }
         }
-        double4 carry = make_double4(0, 0, 0, 0);
+        double4 carry = make_double4(0, 0, 0, 0), carry1 = carry, carry2 = carry;
 #pragma unroll
         for (int k = 0; k < kPrePer; k++) {
             const int i = r.x + k * kPreThr + (int)threadIdx.x;
@@ -1333,24 +1350,44 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
                 const double4 ex = block_excl_scan(mm, &tot);
                 if (i < r.y) loc[i] = d4add(d4add(carry, ex), mm);
                 carry = d4add(carry, tot);
+                if (nvec == 3) {
+                    const double4 m1 = make_double4(mm.x * b.x, mm.x * b.y, mm.x * b.z, mm.y * b.y);
+                    const double4 m2 = make_double4(mm.y * b.z, mm.z * b.z, 0.0, 0.0);
+                    const double4 e1 = block_excl_scan(m1, &tot);
+                    if (i < r.y) loc[nb + i] = d4add(d4add(carry1, e1), m1);
+                    carry1 = d4add(carry1, tot);
+                    const double4 e2 = block_excl_scan(m2, &tot);
+                    if (i < r.y) loc[2 * (int64_t)nb + i] = d4add(d4add(carry2, e2), m2);
+                    carry2 = d4add(carry2, tot);
+                }
             }
         }
-        if (loc && threadIdx.x == 0) part[t] = carry;
+        if (loc && threadIdx.x == 0) {
+            part[t] = carry;
+            if (nvec == 3) {
+                part[ntile + t] = carry1;
+                part[2 * ntile + t] = carry2;
+            }
+        }
     }
 }
 
 // Exclusive scan of the tile sums of one system per block.
-__global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *__restrict__ seg, int ntile)
+// (each of the nvec vectors of tile sums, part[v ntile + t])
+__global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *__restrict__ seg, int ntile, int nvec)
 {
     const int a = seg ? seg[blockIdx.x] : 0, b = seg ? seg[blockIdx.x + 1] : ntile;
-    double4 carry = make_double4(0, 0, 0, 0);
-    for (int base = a; base < b; base += kPreThr) {
-        const int i = base + threadIdx.x;
-        const double4 v = i < b ? part[i] : make_double4(0, 0, 0, 0);
-        double4 tot;
-        const double4 e = block_excl_scan(v, &tot);
-        if (i < b) part[i] = d4add(carry, e);
-        carry = d4add(carry, tot);
+    for (int vec = 0; vec < nvec; vec++) {
+        double4 *pv = part + (int64_t)vec * ntile;
+        double4 carry = make_double4(0, 0, 0, 0);
+        for (int base = a; base < b; base += kPreThr) {
+            const int i = base + threadIdx.x;
+            const double4 v = i < b ? pv[i] : make_double4(0, 0, 0, 0);
+            double4 tot;
+            const double4 e = block_excl_scan(v, &tot);
+            if (i < b) pv[i] = d4add(carry, e);
+            carry = d4add(carry, tot);
+        }
     }
 }
 
@@ -1363,12 +1400,14 @@ __device__ __forceinline__ double4 d4sub(double4 a, double4 b) { return make_dou
 // tile-local sums.  Tile of body i: seg[sys] + (i - s0) / kPreTile, s0 the
 // system's first body (one system: i / kPreTile).  In a batch, the cells
 // above the systems' roots are never walked and get no moments.
+template <bool QUAD>
 __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__ cfirst, const int *__restrict__ clast,
                                                          const uint8_t *__restrict__ clevel,
-                                                         const unsigned long long *ncellp,
+                                                         const unsigned long long *ncellp, int nb, int ntile,
                                                          const double4 *__restrict__ loc, const double4 *__restrict__ part,
                                                          const int *__restrict__ seg, const uint64_t *__restrict__ keys,
-                                                         double *mo, float4 *cA, float4 *cB, const double4 *__restrict__ box,
+                                                         double *mo, float4 *cA, float4 *cB, float4 *cQ0, float4 *cQ1,
+                                                         float4 *cF0, float4 *cF1, const double4 *__restrict__ box,
                                                          int lsys, int sysbits, const double4 *__restrict__ sysbox,
                                                          const int64_t *__restrict__ boff)
 {
@@ -1377,44 +1416,64 @@ __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__
         const int s = cfirst[c], e = clast[c];
         const int L = clevel[c];
         int Lg = L, s0 = 0, t0 = 0;
-        double rsize = box->w;
+        double4 bx = *box;
         if (sysbox) {
             const int sys = sysbits ? (int)(keys[s] >> (63 - sysbits)) : 0;
-            rsize = sysbox[sys].w;
+            bx = sysbox[sys];
             Lg = L - lsys;
             s0 = (int)boff[sys];
             t0 = seg[sys];
         }
         double *me = mo + 10 * (int64_t)c;
         if (Lg < 0) {
-            me[0] = me[1] = me[2] = me[3] = 0.0;
+#pragma unroll
+            for (int u = 0; u < (QUAD ? 10 : 4); u++) me[u] = 0.0;
             cA[c] = make_float4(0.f, 0.f, 0.f, 0.f);
             cB[c] = make_float4(0.f, 0.f, 0.f, 3.0e38f);
+            if (QUAD && cQ0) cQ0[c] = cQ1[c] = make_float4(0.f, 0.f, 0.f, 0.f);
             continue;
         }
         const int tb = t0 + (e - s0) / kPreTile;
-        double4 d = loc[e];
-        if (s > s0) {
-            const int ta = t0 + (s - 1 - s0) / kPreTile;
-            const double4 la = loc[s - 1];
-            d = ta == tb ? d4sub(d, la) : d4add(d4sub(part[tb], part[ta]), d4sub(d, la));
-        } else if (tb != t0) {
-            d = d4add(part[tb], d);
-        }
+        const int ta = s > s0 ? t0 + (s - 1 - s0) / kPreTile : t0;
+        // last body of tile ta (its tile-local sum is the tile's total)
+        const int ea = s0 + (ta - t0 + 1) * kPreTile - 1;
+        // the sums of vector v over the bodies s..e: inside one tile, and
+        // across one tile boundary, from tile-local sums only (small cells
+        // keep the precision of their own magnitudes); across more, the
+        // difference of the tiles' prefixes (cells of over kPreTile bodies)
+        auto span = [&](int v) {
+            const double4 *lv = loc + (int64_t)v * nb;
+            const double4 *pv = part + (int64_t)v * ntile;
+            double4 d = lv[e];
+            if (s > s0) {
+                const double4 la = lv[s - 1];
+                if (ta == tb) d = d4sub(d, la);
+                else if (tb == ta + 1) d = d4add(d4sub(lv[ea], la), d);
+                else d = d4add(d4sub(pv[tb], pv[ta]), d4sub(d, la));
+            } else if (tb != t0) {
+                d = d4add(pv[tb], d);
+            }
+            return d;
+        };
+        const double4 d = span(0);
         const double M = d.w;
         const double px = d.x / M, py = d.y / M, pz = d.z / M;
-        me[0] = M;
-        me[1] = px;
-        me[2] = py;
-        me[3] = pz;
-        float4 A, B;
-        split2(px, A.x, B.x);
-        split2(py, A.y, B.y);
-        split2(pz, A.z, B.z);
-        A.w = (float)M;
-        B.w = (float)(rsize * rsize * ldexp(1.0, -2 * Lg));
-        cA[c] = A;
-        cB[c] = B;
+        double q[6] = {0, 0, 0, 0, 0, 0};
+        if (QUAD) {
+            // second moments about the c.o.m. (parallel axis), then the
+            // traceless quadrupole sum m (3 d d - |d|^2 I) hackcofm builds
+            const double4 d1 = span(1), d2 = span(2);
+            const double t[6] = {d1.x - d.x * px, d1.y - d.x * py, d1.z - d.x * pz,
+                                 d1.w - d.y * py, d2.x - d.y * pz, d2.y - d.z * pz};
+            const double tr = (t[0] + t[3]) + t[5];
+            q[0] = 3.0 * t[0] - tr;
+            q[1] = 3.0 * t[1];
+            q[2] = 3.0 * t[2];
+            q[3] = 3.0 * t[3] - tr;
+            q[4] = 3.0 * t[4];
+            q[5] = 3.0 * t[5] - tr;
+        }
+        cell_records<QUAD>(c, M, px, py, pz, q, L, Lg, bx.w, bx, keys, cfirst, mo, cA, cB, cQ0, cQ1, cF0, cF1);
     }
 }
 
@@ -1788,11 +1847,15 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     double4 *bd = ensure<double4>(ctx->bd, std::max(nb, 1));
     float4 *bhi = ensure<float4>(ctx->bhi, std::max(nb, 1));
     float4 *blo = ensure<float4>(ctx->blo, std::max(nb, 1));
-    // fast monopole builds take the moments from prefix sums of the gathered
-    // bodies (on the second stream, beside the topology)
+    // fast builds take the moments from prefix sums of the gathered bodies
+    // (on the second stream, beside the topology): the first moments, and
+    // the second moments too when the walk's cells or the learned features
+    // carry the quadrupole
     const bool feat_b = p.learned != 0 || ctx->mlp_sampling;
-    const bool prefix_moments = p.mode == BH_MODE_FAST && !p.usequad && !feat_b && nb > 1;
-    double4 *mpre = prefix_moments ? ensure<double4>(ctx->mpre, (size_t)nb) : nullptr;
+    const bool prefix_moments = p.mode == BH_MODE_FAST && nb > 1;
+    const bool prefix_quad = prefix_moments && (p.usequad || feat_b);
+    const int nvec = prefix_quad ? 3 : 1;
+    double4 *mpre = prefix_moments ? ensure<double4>(ctx->mpre, (size_t)nvec * nb) : nullptr;
     // the gather (HBM-bound; with, for prefix-moment builds, the tile-local
     // sums) on the second stream beside the topology (latency-bound); only
     // the moments and the walk read its records.  A bounded grid, so that
@@ -1800,13 +1863,14 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     // prefix tiles restart at every system.
     const int2 *tiles = prefix_moments && nsys ? ptr<int2>(ctx->ptile) : nullptr;
     const int ntile = tiles ? ctx->nptile : (int)((nb + kPreTile - 1) / kPreTile);
-    double4 *part = prefix_moments ? ensure<double4>(ctx->mpart, ntile) : nullptr;
+    double4 *part = prefix_moments ? ensure<double4>(ctx->mpart, (size_t)nvec * ntile) : nullptr;
     side_fork(ctx, kForkSort);
     if (nb > 0)
         k_gather_tree<<<std::min(ntile, ctx->ncu * kGatherBlocksPerCU), kPreThr, 0, ctx->side>>>(
-            vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile, mpre, part);
+            vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile, mpre, part, nvec);
     if (prefix_moments)
-        k_pre_top<<<nsys ? nsys : 1, kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr, ntile);
+        k_pre_top<<<nsys ? nsys : 1, kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr, ntile,
+                                                             nvec);
     side_mark(ctx, kMarkGather);
     T.bd = bd;
     T.bhi = bhi;
@@ -1899,10 +1963,16 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     if (prefix_moments) {
         // the moments on the second stream (after its prefix scan), beside
         // the child slots and the groups on the main stream
-        k_moments_prefix<<<capgrid(ctx, cap), kTpb, 0, sd>>>(cfirst, clast, clevel, tt.ncellp, mpre, part,
-                                                            nsys ? ptr<int>(ctx->pseg) : nullptr, ks, mo, cA, cB,
-                                                            d_box, lsys, ctx->sysbits, sysbox,
-                                                            nsys ? ptr<int64_t>(ctx->boff) : nullptr);
+        const int *pseg = nsys ? ptr<int>(ctx->pseg) : nullptr;
+        const int64_t *boff = nsys ? ptr<int64_t>(ctx->boff) : nullptr;
+        if (prefix_quad)
+            k_moments_prefix<true><<<capgrid(ctx, cap), kTpb, 0, sd>>>(
+                cfirst, clast, clevel, tt.ncellp, nb, ntile, mpre, part, pseg, ks, mo, cA, cB,
+                p.usequad ? cQ0 : nullptr, cQ1, feat_b ? cF0 : nullptr, cF1, d_box, lsys, ctx->sysbits, sysbox, boff);
+        else
+            k_moments_prefix<false><<<capgrid(ctx, cap), kTpb, 0, sd>>>(
+                cfirst, clast, clevel, tt.ncellp, nb, ntile, mpre, part, pseg, ks, mo, cA, cB, nullptr, nullptr,
+                nullptr, nullptr, d_box, lsys, ctx->sysbits, sysbox, boff);
     } else {
         k_level_count<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount);
         k_level_place<<<capgrid(ctx, cap), kTpb, 0, sd>>>(clevel, tt.ncellp, lcount, loffd, lcursor, llist);

@@ -84,6 +84,37 @@ def test_cells_match_oracle_tree_large(eng):
     assert np.array_equal(c["quad6"], t["quad"][:, [0, 1, 2, 4, 5, 8]])
 
 
+def test_fast_quadrupole_cells_from_prefix_sums(eng):
+    """Fast builds whose cells carry the quadrupole (usequad, learned
+    features) take it from FP64 prefix sums of the second moments: mass,
+    c.o.m. and traceless quadrupole equal the oracle's tree (hackcofm's
+    child-by-child parallel-axis sums) to the rounding of the sums, measured
+    against each cell's own scale M (size^2 + |c.o.m.|^2 / 1e6)."""
+    m, x, _ = E.mkplummer(600000, seed=7, with_vel=False)
+    eng.upload(m, x)
+    st = eng.eval(E.default_params(tol=0.7, eps=0.01, usequad=1, group_size=128, nleaf=2))
+    c = eng.debug_cells()
+    t = oracle.tree_cells(m, x, np.full(3, -2.0), 4.0, usequad=True)
+    assert np.array_equal(c["level"], t["level"]) and np.array_equal(c["nbody"], t["nbody"])
+    dm = np.abs(c["mass"] - t["mass"]) / t["mass"]
+    dp = np.abs(c["pos"] - t["pos"]).max(1) / np.maximum(np.abs(t["pos"]).max(1), 1e-3)
+    size = st.rsize / 2.0 ** c["level"].astype(np.float64)
+    qt = t["quad"][:, [0, 1, 2, 4, 5, 8]]
+    scale = t["mass"] * (size ** 2 + (t["pos"] ** 2).sum(1) / 1e6)
+    dq = np.abs(c["quad6"] - qt).max(1) / scale
+    qrel = np.abs(c["quad6"] - qt).max(1) / np.maximum(np.abs(qt).max(1), 1e-300)
+    print("prefix-sum quadrupoles: mass %.2e, c.o.m. %.2e, quadrupole %.2e of M s^2 (median %.2e); "
+          "relative to |Q|: median %.2e, 99.9 %% %.2e" % (dm.max(), dp.max(), dq.max(), np.median(dq),
+                                                          np.median(qrel), np.quantile(qrel, 0.999)))
+    assert dm.max() < 1e-9 and dp.max() < 1e-9
+    assert dq.max() < 1e-5 and np.median(qrel) < 1e-8
+    # a wrong sign of the parallel-axis term fails by orders of magnitude
+    qbad = c["quad6"] + 2 * 3 * c["mass"][:, None] * np.stack(
+        [c["pos"][:, 0] ** 2, c["pos"][:, 0] * c["pos"][:, 1], c["pos"][:, 0] * c["pos"][:, 2],
+         c["pos"][:, 1] ** 2, c["pos"][:, 1] * c["pos"][:, 2], c["pos"][:, 2] ** 2], 1)
+    assert (np.abs(qbad - qt).max(1) / scale).max() > 1e-2
+
+
 def test_fast_monopole_cells_from_prefix_sums(eng):
     """Fast monopole builds take each cell's mass and c.o.m. from FP64 prefix
     sums over the sorted bodies: equal to the oracle's tree (hackcofm's
