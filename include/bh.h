@@ -220,7 +220,7 @@ bh_status bh_batch_lf_init(bh_ctx *ctx, const bh_params *p, int nsys, const int6
 bh_status bh_autotune(bh_ctx *ctx, const bh_params *p, int reps, bh_params *best, double *ms);
 
 /* Learned far-field correction (no reference counterpart).  A 12-64-64-64-9
- * SiLU MLP on BF16 MFMA predicts, for a cell accepted by the group walk whose
+ * MLP (smooth hard-SiLU activations) on BF16 MFMA predicts, for a cell accepted by the group walk whose
  * size/distance lies in the trained range, the residual between the exact
  * force of the cell's bodies and its multipole force at the group centre,
  * and that residual's Jacobian there; the group's sums R and J give each

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY (imported by tests/ and bench.py's checks).
 
-Restates the device network 12 -> 64 -> 64 -> 64 -> 9 with SiLU: weights and
+Restates the device network 12 -> 64 -> 64 -> 64 -> 9 with its smooth
+hard-SiLU activation (act below): weights and
 activations rounded to bfloat16 where the device rounds them (inputs,
 weights, hidden activations, pre-activations kept for the backward pass,
 back-propagated dZ), products accumulated in float64 here (the device uses
@@ -39,13 +40,16 @@ def unpack(p):
     return out
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def act(z):
+    """z h(z), h = 1/2 + t (3/4 - t^2/4), t = clip(z/3, -1, 1) (the device's
+    activation, bh_mlp.hip act())."""
+    t = np.clip(z / 3.0, -1.0, 1.0)
+    return z * (0.5 + t * (0.75 - 0.25 * t * t))
 
 
-def silu_grad(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def act_grad(z):
+    t = np.clip(z / 3.0, -1.0, 1.0)
+    return 0.5 + t * (0.75 - 0.25 * t * t) + 0.25 * z * (1.0 - t * t)
 
 
 def forward(p, x, keep=False):
@@ -54,7 +58,7 @@ def forward(p, x, keep=False):
     zs, hs = [], [h]
     for W, b in ((W1, b1), (W2, b2), (W3, b3)):
         z = h @ bf16(W).T + b
-        h = bf16(silu(z if EXACT else z.astype(np.float32)))
+        h = bf16(act(z if EXACT else z.astype(np.float32)))
         zs.append(bf16(z))
         hs.append(h)
     y = h @ bf16(W4).T + b4
@@ -74,7 +78,7 @@ def grad(p, x, t):
     g[7] = dz.sum(0)
     dh = dz @ bf16(W4)
     for li, (W, zi) in ((2, (W3, 2)), (1, (W2, 1)), (0, (W1, 0))):
-        dz = bf16(dh * silu_grad(zs[zi]))
+        dz = bf16(dh * act_grad(zs[zi]))
         g[2 * li] = dz.T @ hs[li]
         g[2 * li + 1] = dz.sum(0)
         dh = dz @ bf16(W)

@@ -32,3 +32,19 @@ def test_backward_matches_finite_differences():
         assert np.isclose(loss, ((M.forward(p, x) - t) ** 2).sum() / len(x))
     finally:
         M.EXACT = False
+
+
+def test_activation_closed_forms():
+    """act(z) = z h(z), h the cubic sigmoid on [-3, 3]: z beyond 3, 0 below
+    -3, odd part z/2 (h(z) + h(-z) = 1), slope 1/2 at 0, a known value, and
+    act_grad equal to the finite difference of act everywhere, including at
+    the ends of the cubic (the derivative is continuous there)."""
+    z = np.array([3.0, 4.5, 100.0])
+    assert np.array_equal(M.act(z), z) and np.array_equal(M.act(-z), np.zeros(3))
+    w = np.linspace(-5, 5, 1001)
+    assert np.allclose(M.act(w) - M.act(-w), w, rtol=0, atol=1e-15)
+    assert M.act(1.5) == 1.265625 and M.act_grad(0.0) == 0.5
+    h = 1e-6
+    for v in np.concatenate([w, [-3.0, 3.0, -3.0 + 1e-9, 3.0 - 1e-9]]):
+        fd = (M.act(v + h) - M.act(v - h)) / (2 * h)
+        assert abs(fd - M.act_grad(v)) < 1e-6, (v, fd, M.act_grad(v))
