@@ -1305,7 +1305,8 @@ __device__ double4 block_excl_scan(double4 v, double4 *tot)
 // (m x, m y, m z, m), one block scan per round, and part[t] the tile's total,
 // from the records just gathered (no second pass over bd).  nvec = 3 (cells
 // with the quadrupole): also the second moments m (xx, xy, xz, yy) and
-// m (yz, zz, 0, 0), as vectors 1 and 2 (loc[v nb + i], part[v ntile + t]).
+// m (yz, zz), as vectors 1 and 2 (loc[nb + i] and the double2 array at
+// loc + 2 nb; part[v ntile + t], the second as (yz, zz, 0, 0)).
 __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restrict__ perm, int nb,
                                                          const double *__restrict__ pos, const double *__restrict__ mass,
                                                          const double4 *__restrict__ pk, double4 *bd, float4 *bhi,
@@ -1357,7 +1358,8 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
                     if (i < r.y) loc[nb + i] = d4add(d4add(carry1, e1), m1);
                     carry1 = d4add(carry1, tot);
                     const double4 e2 = block_excl_scan(m2, &tot);
-                    if (i < r.y) loc[2 * (int64_t)nb + i] = d4add(d4add(carry2, e2), m2);
+                    const double4 s2 = d4add(d4add(carry2, e2), m2);
+                    if (i < r.y) reinterpret_cast<double2 *>(loc + 2 * (int64_t)nb)[i] = make_double2(s2.x, s2.y);
                     carry2 = d4add(carry2, tot);
                 }
             }
@@ -1373,21 +1375,19 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
 }
 
 // Exclusive scan of the tile sums of one system per block.
-// (each of the nvec vectors of tile sums, part[v ntile + t])
-__global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *__restrict__ seg, int ntile, int nvec)
+// (vector blockIdx.y of the tile sums, part[v ntile + t])
+__global__ __launch_bounds__(kPreThr) void k_pre_top(double4 *part, const int *__restrict__ seg, int ntile)
 {
     const int a = seg ? seg[blockIdx.x] : 0, b = seg ? seg[blockIdx.x + 1] : ntile;
-    for (int vec = 0; vec < nvec; vec++) {
-        double4 *pv = part + (int64_t)vec * ntile;
-        double4 carry = make_double4(0, 0, 0, 0);
-        for (int base = a; base < b; base += kPreThr) {
-            const int i = base + threadIdx.x;
-            const double4 v = i < b ? pv[i] : make_double4(0, 0, 0, 0);
-            double4 tot;
-            const double4 e = block_excl_scan(v, &tot);
-            if (i < b) pv[i] = d4add(carry, e);
-            carry = d4add(carry, tot);
-        }
+    double4 *pv = part + (int64_t)blockIdx.y * ntile;
+    double4 carry = make_double4(0, 0, 0, 0);
+    for (int base = a; base < b; base += kPreThr) {
+        const int i = base + threadIdx.x;
+        const double4 v = i < b ? pv[i] : make_double4(0, 0, 0, 0);
+        double4 tot;
+        const double4 e = block_excl_scan(v, &tot);
+        if (i < b) pv[i] = d4add(carry, e);
+        carry = d4add(carry, tot);
     }
 }
 
@@ -1441,14 +1441,18 @@ __global__ __launch_bounds__(kTpb) void k_moments_prefix(const int *__restrict__
         // across one tile boundary, from tile-local sums only (small cells
         // keep the precision of their own magnitudes); across more, the
         // difference of the tiles' prefixes (cells of over kPreTile bodies)
+        auto at = [&](int v, int i) {
+            if (v < 2) return loc[(int64_t)v * nb + i];
+            const double2 q2 = reinterpret_cast<const double2 *>(loc + 2 * (int64_t)nb)[i];
+            return make_double4(q2.x, q2.y, 0.0, 0.0);
+        };
         auto span = [&](int v) {
-            const double4 *lv = loc + (int64_t)v * nb;
             const double4 *pv = part + (int64_t)v * ntile;
-            double4 d = lv[e];
+            double4 d = at(v, e);
             if (s > s0) {
-                const double4 la = lv[s - 1];
+                const double4 la = at(v, s - 1);
                 if (ta == tb) d = d4sub(d, la);
-                else if (tb == ta + 1) d = d4add(d4sub(lv[ea], la), d);
+                else if (tb == ta + 1) d = d4add(d4sub(at(v, ea), la), d);
                 else d = d4add(d4sub(pv[tb], pv[ta]), d4sub(d, la));
             } else if (tb != t0) {
                 d = d4add(pv[tb], d);
@@ -1869,8 +1873,8 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
         k_gather_tree<<<std::min(ntile, ctx->ncu * kGatherBlocksPerCU), kPreThr, 0, ctx->side>>>(
             vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile, mpre, part, nvec);
     if (prefix_moments)
-        k_pre_top<<<nsys ? nsys : 1, kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr, ntile,
-                                                             nvec);
+        k_pre_top<<<dim3(nsys ? nsys : 1, nvec), kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr,
+                                                                         ntile);
     side_mark(ctx, kMarkGather);
     T.bd = bd;
     T.bhi = bhi;
