@@ -175,7 +175,7 @@ def cpu_baseline(m, x, idx_err, dacc, eps, target, budget_s, nthreads_list=(1,))
 
 
 def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.9, 1.1, 1.3), stride=2, epochs=10, rlo=0.45, rhi=1.5,
-                  usequad=1):
+                  usequad=1, group=256):
     """Residual samples of other snapshots, then training on the device.
     The correction (and the samples) cover accepted cells with size/r in
     [rlo, rhi]: below 0.45 the residual is small and correcting it does not
@@ -187,14 +187,17 @@ def train_learned(eng, E, n, eps, seeds=(41, 43, 47), tols=(0.9, 1.1, 1.3), stri
         m, x, _ = E.mkplummer(n, seed=s, with_vel=False)
         eng.upload(m, x)
         for tol in tols:
-            eng.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=usequad), stride=stride, rlo=rlo, rhi=rhi)
+            # a fixed group size (the learned walks' autotuned one at this N), so
+            # that the samples do not follow the last autotuner's near-ties
+            eng.mlp_collect(E.default_params(tol=tol, eps=eps, usequad=usequad, group_size=group, nleaf=1),
+                            stride=stride, rlo=rlo, rhi=rhi)
     nsamp = eng.mlp_nsamples()
     t1 = time.perf_counter()
     loss = eng.mlp_train(epochs=epochs, batch=8192, lr=1e-3)
     t2 = time.perf_counter()
     eng.mlp_clear_samples()
-    return {"samples": nsamp, "snapshots": "Plummer N=%d seeds %s, tol %s, every %d-th group" % (
-                n, list(seeds), list(tols), stride), "size_over_r_range": [rlo, rhi],
+    return {"samples": nsamp, "snapshots": "Plummer N=%d seeds %s, tol %s, %d-point groups, every %d-th group" % (
+                n, list(seeds), list(tols), group, stride), "size_over_r_range": [rlo, rhi],
             "epochs": epochs, "loss_first": float(loss[0]), "loss_last": float(loss[-1]),
             "t_collect_s": t1 - t0, "t_train_s": t2 - t1}
 
