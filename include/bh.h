@@ -106,7 +106,8 @@ typedef struct {
     double t_walk_kernel_ms; /* device time of the walk kernel launch alone */
     double t_sort_ms;    /* device time of the mass-set key sort */
     int64_t ngroups;     /* fast mode: groups of test points walked */
-    double t_mlp_ms;     /* learned mode: device time of the MLP inference kernel alone */
+    double t_mlp_ms;     /* learned mode: device time of the learned-correction kernel (MLP inference
+                            and the per-group apply) */
     int64_t n_local;     /* fast mode: the body-cell interactions of nbctot evaluated through
                             group local expansions (local_alpha > 0) */
 } bh_stats;

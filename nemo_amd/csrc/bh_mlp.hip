@@ -516,14 +516,6 @@ __device__ __forceinline__ float cell_features(float rx, float ry, float rz, flo
     return A.w / r2;
 }
 
-// Sum over lanes in a fixed xor-tree order (same result in every lane).
-__device__ __forceinline__ float wave_sum(float v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
-
 // Flatten the recorded (group, cells) of every region into one entry list:
 // entry e -> cell id and group; per group: first entry, count, centre.
 __global__ void k_flatten(const int *__restrict__ lrec, int lcap, const uint32_t *__restrict__ off, int64_t ecap,
@@ -560,62 +552,6 @@ __global__ void k_learned_count(const uint32_t *total, int64_t ecap, unsigned lo
 {
     stat[S_NMLP] += *total;
     if ((int64_t)*total > ecap) stat[S_FLAGS] |= (unsigned long long)F_ECAP_OVF;
-}
-
-// Predicted residual field of every flattened entry: res[3e..3e+2] = (R.xyz,
-// J.xx), (J.xy, J.xz, J.yy, J.yz), (J.zz, 0, 0, 0) in force units.  Each wave
-// keeps the weights in registers and strides over 64-entry tiles (the entry
-// count is read on the device); features of a tile go through LDS once.
-__global__ __launch_bounds__(64) void k_learned_reg(const int *__restrict__ ecell, const int *__restrict__ egrp,
-                                                    const uint32_t *__restrict__ total, int64_t ecap,
-                                                    const float4 *__restrict__ gcentre, const float4 *__restrict__ cA,
-                                                    const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
-                                                    const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
-                                                    const float *__restrict__ bias, float yscale, float jscale,
-                                                    float4 *__restrict__ res)
-{
-    __shared__ __attribute__((aligned(16))) float sb[208];
-    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 16];
-    const int l = lane_id();
-    for (int k = l; k < 208; k += 64) sb[k] = bias[k];
-    RegW R;
-    load_regw(R, Wb);
-    const int64_t ne = min((int64_t)*total, ecap);
-    for (int64_t t0 = (int64_t)blockIdx.x * 64; t0 < ne; t0 += (int64_t)gridDim.x * 64) {
-        const int64_t e = t0 + l;
-        float f[kMlpIn];
-        float scale = 0.f, rinv = 0.f;
-        if (e < ne) {
-            const int c = ecell[e];
-            const float4 ctr = gcentre[egrp[e]];
-            const float4 A = cA[c], B = cB[c];
-            scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c], cF1[c],
-                                  f, &rinv);
-        } else {
-#pragma unroll
-            for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
-        }
-        stage_row(sx, f);
-        wave_sync_lds();
-        const int q = l >> 4;
-#pragma unroll
-        for (int sub = 0; sub < 4; sub++) {
-            // lane l: outputs 4q..4q+3 of entry 16 sub + l % 16
-            const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
-            const float sc = __shfl(scale, 16 * sub + (l & 15), 64);
-            const float ri = __shfl(rinv, 16 * sub + (l & 15), 64);
-            const int64_t er = t0 + 16 * sub + (l & 15);
-            if (q < 3 && er < ne) {
-                const float wr = yscale * sc, wj = jscale * sc * ri;
-                float4 o;
-                if (q == 0) o = make_float4(y[0] * wr, y[1] * wr, y[2] * wr, y[3] * wj);
-                else if (q == 1) o = make_float4(y[0] * wj, y[1] * wj, y[2] * wj, y[3] * wj);
-                else o = make_float4(y[0] * wj, 0.f, 0.f, 0.f);
-                res[3 * er + q] = o;
-            }
-        }
-        wave_sync_lds();
-    }
 }
 
 // Adds the correction c (uniform per group) to the accelerations of group
@@ -659,42 +595,92 @@ __device__ __forceinline__ void apply_group1(int g, const float *R, const float 
     }
 }
 
-// Per group (one wave, striding over the groups): sums of its entries'
-// residuals and Jacobians in a fixed order, then each member's first-order
-// correction added to its acceleration.
-__global__ __launch_bounds__(64) void k_group_apply(const int *__restrict__ gstart, const int *__restrict__ gcnt,
-                                                    const unsigned long long *__restrict__ stat,
-                                                    const float4 *__restrict__ res, const float4 *__restrict__ gcentre,
-                                                    const int *__restrict__ gfirst, const int *__restrict__ gcount,
-                                                    const uint32_t *__restrict__ tperm,
-                                                    const float4 *__restrict__ thi, const float4 *__restrict__ tlo,
-                                                    const double4 *__restrict__ sacc, double *phi, double *acc)
+// Adds the per-group corrections corr[] to the group members' accelerations.
+constexpr int kLearnedWavesPerCU = 8;    // k_learned_group's grid (4 to 16 measured alike)
+
+// The learned correction of whole groups, one wave per group: the group's
+// entries (contiguous, flattened) through the MLP 64 at a time, each entry's
+// prediction written to res (per entry: R xyz, J xx | J xy xz yy yz | J zz),
+// summed per output by the lanes that hold it, then R + J (x - centre) added
+// to every member (apply_group1).  The members' scattered stores overlap the
+// other waves' MFMA work; the sums are in a fixed order (deterministic).
+__global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ecell, const int *__restrict__ gstart,
+                                                      const int *__restrict__ gcnt,
+                                                      const unsigned long long *__restrict__ stat,
+                                                      const float4 *__restrict__ gcentre, const float4 *__restrict__ cA,
+                                                      const float4 *__restrict__ cB, const float4 *__restrict__ cF0,
+                                                      const float4 *__restrict__ cF1, const bf16 *__restrict__ Wb,
+                                                      const float *__restrict__ bias, float yscale, float jscale,
+                                                      float4 *__restrict__ res, const int *__restrict__ gfirst,
+                                                      const int *__restrict__ gcount, const uint32_t *__restrict__ tperm,
+                                                      const float4 *__restrict__ thi, const float4 *__restrict__ tlo,
+                                                      const double4 *__restrict__ sacc, double *phi, double *acc)
 {
+    __shared__ __attribute__((aligned(16))) float sb[208];
+    __shared__ __attribute__((aligned(16))) bf16 sx[64 * 16];
+    const int l = lane_id(), q = l >> 4;
+    for (int k = l; k < 208; k += 64) sb[k] = bias[k];
+    RegW R;
+    load_regw(R, Wb);
     const int ng = (int)stat[S_NGROUPS];
-    const int l = lane_id();
     for (int g = blockIdx.x; g < ng; g += gridDim.x) {
         const int s = gstart[g], n = gcnt[g];
-        float v[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for (int k = l; k < n; k += 64) {
-            const float4 a = res[3 * (int64_t)(s + k)], b = res[3 * (int64_t)(s + k) + 1],
-                         c = res[3 * (int64_t)(s + k) + 2];
-            v[0] += a.x;
-            v[1] += a.y;
-            v[2] += a.z;
-            v[3] += a.w;
-            v[4] += b.x;
-            v[5] += b.y;
-            v[6] += b.z;
-            v[7] += b.w;
-            v[8] += c.x;
-        }
+        // (a group without entries adds zero; its centre is not read)
+        const float4 ctr = n > 0 ? gcentre[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float a4[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int t0 = 0; t0 < n; t0 += 64) {
+            float f[kMlpIn];
+            float scale = 0.f, rinv = 0.f;
+            if (t0 + l < n) {
+                const int c = ecell[s + t0 + l];
+                const float4 A = cA[c], B = cB[c];
+                scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c],
+                                      cF1[c], f, &rinv);
+            } else {
 #pragma unroll
-        for (int t = 0; t < 9; t++) v[t] = wave_sum(v[t]);
-        apply_group1(g, v, v + 3, gcentre[g], gfirst, gcount, tperm, thi, tlo, sacc, phi, acc);
+                for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
+            }
+            stage_row(sx, f);
+            wave_sync_lds();
+#pragma unroll
+            for (int sub = 0; sub < 4; sub++) {
+                if (t0 + 16 * sub >= n) break;
+                // lane l: outputs 4q..4q+3 of entry 16 sub + l % 16
+                const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
+                const float sc = __shfl(scale, 16 * sub + (l & 15), 64);
+                const float ri = __shfl(rinv, 16 * sub + (l & 15), 64);
+                const int er = t0 + 16 * sub + (l & 15);
+                if (q < 3 && er < n) {
+                    const float wr = yscale * sc, wj = jscale * sc * ri;
+                    float4 o;
+                    if (q == 0) o = make_float4(y[0] * wr, y[1] * wr, y[2] * wr, y[3] * wj);
+                    else if (q == 1) o = make_float4(y[0] * wj, y[1] * wj, y[2] * wj, y[3] * wj);
+                    else o = make_float4(y[0] * wj, 0.f, 0.f, 0.f);
+                    res[3 * (int64_t)(s + er) + q] = o;
+                    a4[0] += o.x;
+                    a4[1] += o.y;
+                    a4[2] += o.z;
+                    a4[3] += o.w;
+                }
+            }
+            wave_sync_lds();
+        }
+        // sums over the 16 lanes of each quarter, then the nine outputs
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1)
+#pragma unroll
+            for (int k = 0; k < 4; k++) a4[k] += __shfl_xor(a4[k], m, 64);
+        float v[9];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = __shfl(a4[k], 0, 64);
+            v[4 + k] = __shfl(a4[k], 16, 64);
+        }
+        v[8] = __shfl(a4[0], 32, 64);
+        apply_group1(g, v, v + 3, ctr, gfirst, gcount, tperm, thi, tlo, sacc, phi, acc);
     }
 }
 
-// Adds the per-group corrections corr[] to the group members' accelerations.
 __global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
                              const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
                              const float4 *__restrict__ corr, const double4 *__restrict__ sacc, double *phi,
@@ -1116,15 +1102,12 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         // the number of entries is only known on the device: launch over the
         // bound and let waves past the end exit
         k_learned_count<<<1, 1, 0, st>>>(off + nreg, ecap, stat);
-        const int64_t tiles = (ecap + 63) / 64;
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[8], st));
-        k_learned_reg<<<(int)std::min<int64_t>(tiles, (int64_t)ctx->ncu * 8), 64, 0, st>>>(
-            ecell, egrp, off + nreg, ecap, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias),
-            M.norm[0], M.norm[3], res);
+        k_learned_group<<<(int)std::min<int64_t>(ngcap, (int64_t)ctx->ncu * kLearnedWavesPerCU), 64, 0, st>>>(
+            ecell, gstart, gcnt, stat, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias),
+            M.norm[0], M.norm[3], res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), T.tperm, T.thi, T.tlo,
+            ptr<double4>(ctx->sacc), ptr<double>(ctx->phi), ptr<double>(ctx->acc));
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
-        k_group_apply<<<gridg, 64, 0, st>>>(gstart, gcnt, stat, res, gctr, ptr<int>(ctx->gfirst),
-                                            ptr<int>(ctx->gcount), T.tperm, T.thi, T.tlo, ptr<double4>(ctx->sacc),
-                                            ptr<double>(ctx->phi), ptr<double>(ctx->acc));
     }
     BH_HIP(hipGetLastError());
 }
