@@ -248,9 +248,14 @@ bh_status bh_mlp_samples(bh_ctx *ctx, int64_t cap, float *x, float *y, int64_t *
 bh_status bh_debug_samples(bh_ctx *ctx, int64_t cap, int *cell, float *centre, int64_t *n);
 /* Tests: the entries of the last learned evaluation (cell, group, predicted
  * field res[12] = residual xyz, Jacobian xx xy xz yy yz zz, 0 0 0, in force
- * units) and each group's centre ([4], w unused). */
+ * units; only when bh_debug_keep_learned was on for it, else BH_ERR_STATE
+ * for a non-null res) and each group's centre ([4], w unused). */
 bh_status bh_debug_learned(bh_ctx *ctx, int64_t cap, int *cell, int *group, float *res, int64_t *nent, int64_t gcap,
                            float *centre, int64_t *ngroups);
+/* Tests: keep (on != 0) each entry's MLP output in the following learned
+ * evaluations for bh_debug_learned; off by default, when the learned kernel
+ * stores only the corrected results. */
+void bh_debug_keep_learned(bh_ctx *ctx, int on);
 /* Tests: each cell's first body (index in the tree's sorted order, see
  * bh_debug_perm) and body count. */
 bh_status bh_debug_cell_ranges(bh_ctx *ctx, int64_t cap, int *first, int *count, int64_t *ncell);

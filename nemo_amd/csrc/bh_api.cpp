@@ -270,6 +270,7 @@ void eval(bh_ctx *ctx, const bh_params *p, bh_stats *st)
     ctx->last = s;
     ctx->last_mode = q.mode;
     ctx->mlp.nentries = (q.mode == BH_MODE_FAST && q.learned == 1) ? std::min<int64_t>(s.n_mlp, ctx->mlp.ecap) : -1;
+    ctx->mlp.res_valid = ctx->mlp.nentries >= 0 && ctx->mlp.keep_res;
     if (st) *st = s;
 }
 
@@ -444,6 +445,11 @@ void bh_destroy(bh_ctx *ctx)
 }
 
 const char *bh_last_error(const bh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void bh_debug_keep_learned(bh_ctx *ctx, int on)
+{
+    if (ctx) ctx->mlp.keep_res = on != 0;
+}
 
 void bh_debug_inject(bh_ctx *ctx, int code)
 {
@@ -1040,6 +1046,8 @@ bh_status bh_debug_learned(bh_ctx *ctx, int64_t cap, int *cell, int *group, floa
     return guarded(ctx, [&] {
         bh::Mlp &M = ctx->mlp;
         if (M.nentries < 0) throw ApiFail(BH_ERR_STATE, "the last evaluation was not a learned one");
+        if (res && !M.res_valid)
+            throw ApiFail(BH_ERR_STATE, "the entries' outputs were not kept (bh_debug_keep_learned before the evaluation)");
         const int64_t ne = M.nentries;
         if (nent) *nent = ne;
         if (ngroups) *ngroups = ctx->ngroups;

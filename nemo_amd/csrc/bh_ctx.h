@@ -16,6 +16,8 @@ struct Mlp {
     int64_t nsamples = 0;
     int64_t nlast = 0;          // samples added by the last collection
     int64_t nentries = -1;      // entries of the last evaluation if it was a learned one, else -1
+    bool keep_res = false;      // store each entry's MLP output (res) in learned evaluations (tests)
+    bool res_valid = false;     // res holds the last learned evaluation's outputs
     int64_t ecap = 0;           // capacity of the flattened correction entries
     float norm[kMlpNorm] = {1.f, 0.25f, 1.6f, 1.f};
     bool ready = false;

@@ -600,7 +600,8 @@ constexpr int kLearnedWavesPerCU = 8;    // k_learned_group's grid (4 to 16 meas
 
 // The learned correction of whole groups, one wave per group: the group's
 // entries (contiguous, flattened) through the MLP 64 at a time, each entry's
-// prediction written to res (per entry: R xyz, J xx | J xy xz yy yz | J zz),
+// prediction written to res when it is kept (per entry: R xyz, J xx | J xy xz
+// yy yz | J zz),
 // summed per output by the lanes that hold it, then R + J (x - centre) added
 // to every member (apply_group1).  The members' scattered stores overlap the
 // other waves' MFMA work; the sums are in a fixed order (deterministic).
@@ -656,7 +657,7 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
                     if (q == 0) o = make_float4(y[0] * wr, y[1] * wr, y[2] * wr, y[3] * wj);
                     else if (q == 1) o = make_float4(y[0] * wj, y[1] * wj, y[2] * wj, y[3] * wj);
                     else o = make_float4(y[0] * wj, 0.f, 0.f, 0.f);
-                    res[3 * (int64_t)(s + er) + q] = o;
+                    if (res) res[3 * (int64_t)(s + er) + q] = o;
                     a4[0] += o.x;
                     a4[1] += o.y;
                     a4[2] += o.z;
@@ -1105,7 +1106,7 @@ void mlp_correct(bh_ctx *ctx, const bh_params &p)
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[8], st));
         k_learned_group<<<(int)std::min<int64_t>(ngcap, (int64_t)ctx->ncu * kLearnedWavesPerCU), 64, 0, st>>>(
             ecell, gstart, gcnt, stat, gctr, T.cA, T.cB, T.cF0, T.cF1, ptr<bf16>(M.wb), ptr<float>(M.bias),
-            M.norm[0], M.norm[3], res, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), T.tperm, T.thi, T.tlo,
+            M.norm[0], M.norm[3], M.keep_res ? res : nullptr, ptr<int>(ctx->gfirst), ptr<int>(ctx->gcount), T.tperm, T.thi, T.tlo,
             ptr<double4>(ctx->sacc), ptr<double>(ctx->phi), ptr<double>(ctx->acc));
         if (!ctx->capturing) BH_HIP(hipEventRecord(ctx->ev[9], st));
     }

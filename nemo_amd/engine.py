@@ -81,6 +81,7 @@ SIGNATURES = {
     "bh_debug_keys": (C.c_int, [_vp, _u64]),
     "bh_debug_cells": (C.c_int, [_vp, C.c_int64, _i32, _i32, _d, _d, _d, _i64]),
     "bh_debug_inject": (None, [_vp, C.c_int]),
+    "bh_debug_keep_learned": (None, [_vp, C.c_int]),
     "bh_debug_perm": (C.c_int, [_vp, C.POINTER(C.c_uint32)]),
     "bh_debug_groups": (C.c_int, [_vp, C.c_int64, _i32, _i32, _i64]),
     "bh_mkplummer": (None, [C.c_int64, C.c_uint64, _d, _d, _d]),
@@ -356,18 +357,23 @@ class Engine:
         self._check(lib().bh_debug_samples(self._h, k, _p(cell, _i32), _p(ctr, _f), _p(n, _i64)))
         return cell, ctr
 
-    def debug_learned(self):
-        """Entries of the last learned evaluation: (cell, group, res[n][12],
-        centre[ngroups][4])."""
+    def debug_keep_learned(self, on=True):
+        lib().bh_debug_keep_learned(self._h, int(bool(on)))
+
+    def debug_learned(self, with_res=True):
+        """Entries of the last learned evaluation: (cell, group, res[n][12]
+        (None without with_res; needs debug_keep_learned before the
+        evaluation), centre[ngroups][4])."""
         n = np.zeros(1, np.int64)
         ng = np.zeros(1, np.int64)
         self._check(lib().bh_debug_learned(self._h, 0, None, None, None, _p(n, _i64), 0, None, _p(ng, _i64)))
         k, g = int(n[0]), int(ng[0])
         cell = np.zeros(k, np.int32)
         grp = np.zeros(k, np.int32)
-        res = np.zeros((k, 12), np.float32)
+        res = np.zeros((k, 12), np.float32) if with_res else None
         ctr = np.zeros((g, 4), np.float32)
-        self._check(lib().bh_debug_learned(self._h, k, _p(cell, _i32), _p(grp, _i32), _p(res, _f), _p(n, _i64), g,
+        self._check(lib().bh_debug_learned(self._h, k, _p(cell, _i32), _p(grp, _i32),
+                                           _p(res, _f) if with_res else None, _p(n, _i64), g,
                                            _p(ctr, _f), _p(ng, _i64)))
         return cell, grp, res, ctr
 

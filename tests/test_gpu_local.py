@@ -31,7 +31,7 @@ def _record(e, base):
     P, _ = e.mlp_get()
     e.mlp_set(np.zeros_like(P), np.array([1e-30, 0.0, 1e4, 1e-30], np.float32))
     e.eval(E.default_params(learned=1, **base))
-    ecell, egrp, _, gctr = e.debug_learned()
+    ecell, egrp, _, gctr = e.debug_learned(with_res=False)
     return ecell, egrp, gctr[:, :3].astype(np.float64)
 
 
@@ -234,7 +234,7 @@ def test_learned_walk_with_local_expansions(eng):
         eng.mlp_set(P, norm)
         st = eng.eval(E.default_params(local_alpha=alpha, **base))
         _, a1 = eng.download()
-        ecell, egrp, _, _ = eng.debug_learned()
+        ecell, egrp, _, _ = eng.debug_learned(with_res=False)
         eng.mlp_set(np.zeros_like(P), norm)
         st0 = eng.eval(E.default_params(local_alpha=alpha, **base))
         _, a0 = eng.download()
