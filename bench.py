@@ -452,7 +452,9 @@ def main():
     ap.add_argument("--learned-local", default="0.2",
                     help="learned walk with monopole cells: also with group local expansions at these local_alpha "
                          "values (comma-separated; 'none' skips), same model")
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--train-group", type=int, default=192, help="group size of the training-sample walks")
+    ap.add_argument("--train-stride", type=int, default=1, help="training samples from every k-th group")
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -503,7 +505,8 @@ def main():
             with np.load(model_path(args.model, lq)) as z:
                 eng.mlp_set(z["params"], z["norm"])
             return {"loaded_from": model_path(args.model, lq)}
-        tr = train_learned(eng, E, n, args.eps, epochs=args.epochs, usequad=lq)
+        tr = train_learned(eng, E, n, args.eps, epochs=args.epochs, usequad=lq, group=args.train_group,
+                           stride=args.train_stride)
         log("learned (usequad=%d): %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
             lq, tr["samples"], tr["t_collect_s"], tr["t_train_s"], tr["loss_first"], tr["loss_last"]))
         if args.save_model:
