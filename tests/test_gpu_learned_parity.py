@@ -97,6 +97,7 @@ def test_learned_entries_and_group_correction_equal_oracle():
         e.mlp_set(P, norm)
         base = dict(tol=1.0, eps=eps, usequad=0, group_size=128, nleaf=1)
         st = e.eval(E.default_params(learned=1, **base))
+        _, a1_plain = e.download()
         # the entries' outputs are stored only when asked for
         with pytest.raises(E.BHError) as ei:
             e.debug_learned()
@@ -104,6 +105,8 @@ def test_learned_entries_and_group_correction_equal_oracle():
         e.debug_keep_learned(True)
         st = e.eval(E.default_params(learned=1, **base))
         _, a1 = e.download()
+        # storing them leaves the results bit for bit
+        assert np.array_equal(a1, a1_plain)
         ecell, egrp, res, gctr = e.debug_learned()
         cells = e.debug_cells()
         first, count = e.debug_cell_ranges()
