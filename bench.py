@@ -526,7 +526,7 @@ def main():
         tune = None
         if base.get("group_size", 0) == 0:
             # group and leaf size from the autotuner, timed at a nominal tol
-            nominal = fixed_tol or (1.1 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
+            nominal = fixed_tol or (0.95 if base.get("learned") else (0.75 if base["usequad"] else 0.67))
             t0 = time.perf_counter()
             best, ms = eng.autotune(E.default_params(**dict(base, tol=nominal)), reps=2)
             # near-ties (within 2 %) go to the larger group: its conservative
