@@ -455,6 +455,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--train-group", type=int, default=192, help="group size of the training-sample walks")
     ap.add_argument("--train-stride", type=int, default=1, help="training samples from every k-th group")
+    ap.add_argument("--train-rlo", type=float, default=0.45,
+                    help="lower end of the size/r range the model is trained for and corrects")
     ap.add_argument("--save-model", default=None, help="write the trained MLP (params, norm) to this .npz")
     ap.add_argument("--model", default=None, help="load the MLP from this .npz instead of training it")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -506,7 +508,7 @@ def main():
                 eng.mlp_set(z["params"], z["norm"])
             return {"loaded_from": model_path(args.model, lq)}
         tr = train_learned(eng, E, n, args.eps, epochs=args.epochs, usequad=lq, group=args.train_group,
-                           stride=args.train_stride)
+                           stride=args.train_stride, rlo=args.train_rlo)
         log("learned (usequad=%d): %d samples, collect %.1f s, train %.1f s, loss %.3f -> %.3f" % (
             lq, tr["samples"], tr["t_collect_s"], tr["t_train_s"], tr["loss_first"], tr["loss_last"]))
         if args.save_model:
