@@ -595,7 +595,6 @@ __device__ __forceinline__ void apply_group1(int g, const float *R, const float 
     }
 }
 
-// Adds the per-group corrections corr[] to the group members' accelerations.
 constexpr int kLearnedWavesPerCU = 8;    // k_learned_group's grid (4 to 16 measured alike)
 
 // The learned correction of whole groups, one wave per group: the group's
@@ -682,6 +681,7 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
     }
 }
 
+// Adds the per-group corrections corr[] to the group members' accelerations.
 __global__ void k_apply_corr(const int *__restrict__ gfirst, const int *__restrict__ gcount,
                              const unsigned long long *__restrict__ stat, const uint32_t *__restrict__ tperm,
                              const float4 *__restrict__ corr, const double4 *__restrict__ sacc, double *phi,
