@@ -1249,8 +1249,9 @@ __device__ void cell_moments(int c, const int *__restrict__ csub, const int *__r
 // m z, m): one pass over the bodies (beside the topology, on the second
 // stream) and one O(1) step per cell, instead of 22 dependent level launches.
 // The sums round differently from hackcofm's child-by-child order, so this
-// path serves the fast walk with monopole cells only; strict mode and
-// quadrupole or learned builds keep the level-by-level pass.  The difference
+// path serves the fast walk only (monopole cells; quadrupole cells and learned
+// features also from prefix sums of the second moments); strict mode keeps
+// the level-by-level pass.  The difference
 // of two prefixes carries an absolute error of ~1e-16 of the running sums:
 // relative 1e-9 of a two-body cell's mass at N = 4M, far below the FP32 of
 // the walk.
@@ -1303,16 +1304,19 @@ __device__ double4 block_excl_scan(double4 v, double4 *tot)
 // body k * kPreThr + j of the tile (coalesced), all rounds' loads issued
 // together.  loc (prefix-moment builds): the tile-local inclusive sums of
 // (m x, m y, m z, m), one block scan per round, and part[t] the tile's total,
-// from the records just gathered (no second pass over bd).  nvec = 3 (cells
+// from the records just gathered (no second pass over bd).  NV = 3 (cells
 // with the quadrupole): also the second moments m (xx, xy, xz, yy) and
 // m (yz, zz), as vectors 1 and 2 (loc[nb + i] and the double2 array at
-// loc + 2 nb; part[v ntile + t], the second as (yz, zz, 0, 0)).
+// loc + 2 nb; part[v ntile + t], the second as (yz, zz, 0, 0)).  NV = 1
+// keeps the plain gather's registers low (4 waves per SIMD against 2).
+template <int NV>
 __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restrict__ perm, int nb,
                                                          const double *__restrict__ pos, const double *__restrict__ mass,
                                                          const double4 *__restrict__ pk, double4 *bd, float4 *bhi,
                                                          float4 *blo, const int2 *__restrict__ tiles, int ntile,
-                                                         double4 *loc, double4 *part, int nvec)
+                                                         double4 *loc, double4 *part)
 {
+    static_assert(NV == 1 || NV == 3, "prefix vectors");
     for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
         const int2 r = pre_tile(tiles, t, nb);
         double4 v[kPrePer];
@@ -1351,7 +1355,7 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
                 const double4 ex = block_excl_scan(mm, &tot);
                 if (i < r.y) loc[i] = d4add(d4add(carry, ex), mm);
                 carry = d4add(carry, tot);
-                if (nvec == 3) {
+                if (NV == 3) {
                     const double4 m1 = make_double4(mm.x * b.x, mm.x * b.y, mm.x * b.z, mm.y * b.y);
                     const double4 m2 = make_double4(mm.y * b.z, mm.z * b.z, 0.0, 0.0);
                     const double4 e1 = block_excl_scan(m1, &tot);
@@ -1366,7 +1370,7 @@ __global__ __launch_bounds__(kPreThr) void k_gather_tree(const uint32_t *__restr
         }
         if (loc && threadIdx.x == 0) {
             part[t] = carry;
-            if (nvec == 3) {
+            if (NV == 3) {
                 part[ntile + t] = carry1;
                 part[2 * ntile + t] = carry2;
             }
@@ -1869,9 +1873,15 @@ void build_tree(bh_ctx *ctx, const bh_params &p)
     const int ntile = tiles ? ctx->nptile : (int)((nb + kPreTile - 1) / kPreTile);
     double4 *part = prefix_moments ? ensure<double4>(ctx->mpart, (size_t)nvec * ntile) : nullptr;
     side_fork(ctx, kForkSort);
-    if (nb > 0)
-        k_gather_tree<<<std::min(ntile, ctx->ncu * kGatherBlocksPerCU), kPreThr, 0, ctx->side>>>(
-            vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile, mpre, part, nvec);
+    if (nb > 0) {
+        const int ggrid = std::min(ntile, ctx->ncu * kGatherBlocksPerCU);
+        if (nvec == 3)
+            k_gather_tree<3><<<ggrid, kPreThr, 0, ctx->side>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile,
+                                                               mpre, part);
+        else
+            k_gather_tree<1><<<ggrid, kPreThr, 0, ctx->side>>>(vs, nb, d_pos, d_mass, pk, bd, bhi, blo, tiles, ntile,
+                                                               mpre, part);
+    }
     if (prefix_moments)
         k_pre_top<<<dim3(nsys ? nsys : 1, nvec), kPreThr, 0, ctx->side>>>(part, nsys ? ptr<int>(ctx->pseg) : nullptr,
                                                                          ntile);
