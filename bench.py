@@ -452,7 +452,7 @@ def main():
     ap.add_argument("--learned-local", default="0.2",
                     help="learned walk with monopole cells: also with group local expansions at these local_alpha "
                          "values (comma-separated; 'none' skips), same model")
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--train-group", type=int, default=192, help="group size of the training-sample walks")
     ap.add_argument("--train-stride", type=int, default=1, help="training samples from every k-th group")
     ap.add_argument("--train-rlo", type=float, default=0.45,
