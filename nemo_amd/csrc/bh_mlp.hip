@@ -573,25 +573,44 @@ __device__ __forceinline__ void apply_group(int g, float4 c, const int *__restri
     }
 }
 
-// First-order correction of group g's members: R + J (x - centre), with the
-// member's position x from its FP32 hi/lo parts and J symmetric
-// (j = xx xy xz yy yz zz).
-__device__ __forceinline__ void apply_group1(int g, const float *R, const float *j, float4 ctr,
-                                             const int *__restrict__ gfirst, const int *__restrict__ gcount,
+// First-order correction of the members [f, f + n) of a group: R + J (x -
+// centre), with the member's position x from its FP32 hi/lo parts and J
+// symmetric (j = xx xy xz yy yz zz); the recording walk left each member's
+// (a, phi) in sorted order in sacc, and the final values go to the
+// input-order arrays in one write.  Two rows of 64 per round, both rows'
+// loads issued before either row's stores.
+__device__ __forceinline__ void apply_range1(int f, int n, const float *R, const float *j, float4 ctr,
                                              const uint32_t *__restrict__ tperm, const float4 *__restrict__ thi,
                                              const float4 *__restrict__ tlo, const double4 *__restrict__ sacc,
                                              double *phi, double *acc)
 {
-    const int f = gfirst[g], n = gcount[g];
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const float4 h = thi[f + k], lo = tlo[f + k];
-        const float dx = (h.x - ctr.x) + lo.x, dy = (h.y - ctr.y) + lo.y, dz = (h.z - ctr.z) + lo.z;
-        const int64_t o = tperm[f + k];
-        const double4 r = sacc[f + k];
-        acc[3 * o] = r.x + (double)fmaf(j[0], dx, fmaf(j[1], dy, fmaf(j[2], dz, R[0])));
-        acc[3 * o + 1] = r.y + (double)fmaf(j[1], dx, fmaf(j[3], dy, fmaf(j[4], dz, R[1])));
-        acc[3 * o + 2] = r.z + (double)fmaf(j[2], dx, fmaf(j[4], dy, fmaf(j[5], dz, R[2])));
-        phi[o] = r.w;
+    const int l = threadIdx.x & 63;
+    for (int k0 = 0; k0 < n; k0 += 128) {
+        float4 h[2], lo[2];
+        double4 r[2];
+        int64_t o[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int k = k0 + 64 * u + l;
+            if (k < n) {
+                h[u] = thi[f + k];
+                lo[u] = tlo[f + k];
+                o[u] = tperm[f + k];
+                r[u] = sacc[f + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int k = k0 + 64 * u + l;
+            if (k < n) {
+                const float dx = (h[u].x - ctr.x) + lo[u].x, dy = (h[u].y - ctr.y) + lo[u].y,
+                            dz = (h[u].z - ctr.z) + lo[u].z;
+                acc[3 * o[u]] = r[u].x + (double)fmaf(j[0], dx, fmaf(j[1], dy, fmaf(j[2], dz, R[0])));
+                acc[3 * o[u] + 1] = r[u].y + (double)fmaf(j[1], dx, fmaf(j[3], dy, fmaf(j[4], dz, R[1])));
+                acc[3 * o[u] + 2] = r[u].z + (double)fmaf(j[2], dx, fmaf(j[4], dy, fmaf(j[5], dz, R[2])));
+                phi[o[u]] = r[u].w;
+            }
+        }
     }
 }
 
@@ -602,7 +621,7 @@ constexpr int kLearnedWavesPerCU = 8;    // k_learned_group's grid (4 to 16 meas
 // prediction written to res when it is kept (per entry: R xyz, J xx | J xy xz
 // yy yz | J zz),
 // summed per output by the lanes that hold it, then R + J (x - centre) added
-// to every member (apply_group1).  The members' scattered stores overlap the
+// to every member (apply_range1).  The members' scattered stores overlap the
 // other waves' MFMA work; the sums are in a fixed order (deterministic).
 __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ecell, const int *__restrict__ gstart,
                                                       const int *__restrict__ gcnt,
@@ -622,31 +641,75 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
     for (int k = l; k < 208; k += 64) sb[k] = bias[k];
     RegW R;
     load_regw(R, Wb);
+    // the weights in place before the loop, so the loop's own waits (the
+    // weights' first uses inside it) do not also wait for its prefetches
+    __builtin_amdgcn_s_waitcnt(0);
     const int ng = (int)stat[S_NGROUPS];
-    for (int g = blockIdx.x; g < ng; g += gridDim.x) {
-        const int s = gstart[g], n = gcnt[g];
+    // Loads run one step ahead of their use: the next group's entry range
+    // while this group runs, the next chunk's cell ids before this chunk's
+    // MLP and their cell records after its first sub-tile (a next chunk
+    // exists only when this one is full), so the dependent loads overlap
+    // the MFMA work instead of stalling the wave.  The records of the chunk
+    // being processed are only read for its features, before the sub-tiles.
+    int g = blockIdx.x;
+    int s = 0, n = 0;
+    if (g < ng) {
+        s = gstart[g];
+        n = gcnt[g];
+    }
+    for (; g < ng; g += gridDim.x) {
+        const int gn = g + gridDim.x;
+        int sN = 0, nN = 0;
+        if (gn < ng) {
+            sN = gstart[gn];
+            nN = gcnt[gn];
+        }
         // (a group without entries adds zero; its centre is not read)
         const float4 ctr = n > 0 ? gcentre[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int mf = gfirst[g], mn = gcount[g];
+        float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A, F0 = A, F1 = A;
+        if (n > 0) {
+            const unsigned c = (unsigned)ecell[s + min(l, n - 1)];
+            A = cA[c];
+            B = cB[c];
+            F0 = cF0[c];
+            F1 = cF1[c];
+        }
         float a4[4] = {0.f, 0.f, 0.f, 0.f};
         for (int t0 = 0; t0 < n; t0 += 64) {
             float f[kMlpIn];
             float scale = 0.f, rinv = 0.f;
             if (t0 + l < n) {
-                const int c = ecell[s + t0 + l];
-                const float4 A = cA[c], B = cB[c];
-                scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, cF0[c],
-                                      cF1[c], f, &rinv);
+                scale = cell_features((A.x - ctr.x) + B.x, (A.y - ctr.y) + B.y, (A.z - ctr.z) + B.z, A, B, F0, F1, f,
+                                      &rinv);
             } else {
 #pragma unroll
                 for (int k = 0; k < kMlpIn; k++) f[k] = 0.f;
             }
             stage_row(sx, f);
             wave_sync_lds();
+            // (unconditional: a lane past the end, or every lane of the last
+            // chunk, reads the last entry's cell, so no test and no merge of
+            // registers waits on these loads; unsigned, so forming the
+            // address does not wait on the id either)
+            const unsigned cN = (unsigned)ecell[s + min(t0 + 64 + l, n - 1)];
 #pragma unroll
             for (int sub = 0; sub < 4; sub++) {
                 if (t0 + 16 * sub >= n) break;
                 // lane l: outputs 4q..4q+3 of entry 16 sub + l % 16
                 const f32x4 y = mlp_fwd_reg(R, sb, xb_of(sx, sub));
+                if (sub == 0) {
+                    // (kept after this sub-tile's MFMAs: the compiler would
+                    // otherwise hoist the address arithmetic, and its wait for
+                    // the cell id, above them; the empty asm pins the id here)
+                    __builtin_amdgcn_sched_barrier(0);
+                    unsigned cq = cN;
+                    asm volatile("" : "+v"(cq));
+                    A = cA[cq];
+                    B = cB[cq];
+                    F0 = cF0[cq];
+                    F1 = cF1[cq];
+                }
                 const float sc = __shfl(scale, 16 * sub + (l & 15), 64);
                 const float ri = __shfl(rinv, 16 * sub + (l & 15), 64);
                 const int er = t0 + 16 * sub + (l & 15);
@@ -677,7 +740,9 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
             v[4 + k] = __shfl(a4[k], 16, 64);
         }
         v[8] = __shfl(a4[0], 32, 64);
-        apply_group1(g, v, v + 3, ctr, gfirst, gcount, tperm, thi, tlo, sacc, phi, acc);
+        apply_range1(mf, mn, v, v + 3, ctr, tperm, thi, tlo, sacc, phi, acc);
+        s = sN;
+        n = nN;
     }
 }
 
