@@ -647,28 +647,35 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
     const int ng = (int)stat[S_NGROUPS];
     // Loads run one step ahead of their use: the next group's entry range
     // while this group runs, the next chunk's cell ids before this chunk's
-    // MLP and their cell records after its first sub-tile (a next chunk
-    // exists only when this one is full), so the dependent loads overlap
-    // the MFMA work instead of stalling the wave.  The records of the chunk
-    // being processed are only read for its features, before the sub-tiles.
+    // MLP and their cell records after its first sub-tile (a group's last
+    // chunk fetches the next group's first one), so the dependent loads
+    // overlap the MFMA work instead of stalling the wave.  The records of the
+    // chunk being processed are only read for its features, before the
+    // sub-tiles.
     int g = blockIdx.x;
     int s = 0, n = 0;
     if (g < ng) {
         s = gstart[g];
         n = gcnt[g];
     }
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 A = zero4, B = A, F0 = A, F1 = A;
+    bool pre = false;   // A..F1 hold this group's first chunk
+    float4 ctrL = g < ng ? gcentre[g] : zero4;
     for (; g < ng; g += gridDim.x) {
         const int gn = g + gridDim.x;
         int sN = 0, nN = 0;
+        float4 ctrN = zero4;
         if (gn < ng) {
             sN = gstart[gn];
             nN = gcnt[gn];
+            ctrN = gcentre[gn];
         }
-        // (a group without entries adds zero; its centre is not read)
-        const float4 ctr = n > 0 ? gcentre[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // (a group without entries adds zero: R = J = 0, and its centre,
+        // which it may not have, is replaced by 0 so that J dx stays 0)
+        const float4 ctr = n > 0 ? ctrL : zero4;
         const int mf = gfirst[g], mn = gcount[g];
-        float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A, F0 = A, F1 = A;
-        if (n > 0) {
+        if (!pre && n > 0) {
             const unsigned c = (unsigned)ecell[s + min(l, n - 1)];
             A = cA[c];
             B = cB[c];
@@ -688,11 +695,13 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
             }
             stage_row(sx, f);
             wave_sync_lds();
-            // (unconditional: a lane past the end, or every lane of the last
-            // chunk, reads the last entry's cell, so no test and no merge of
-            // registers waits on these loads; unsigned, so forming the
-            // address does not wait on the id either)
-            const unsigned cN = (unsigned)ecell[s + min(t0 + 64 + l, n - 1)];
+            // (unconditional, with wave-uniform choices: a lane past the end
+            // reads the last entry's cell, and without a next chunk or group
+            // every lane does, so no per-lane test and no merge of registers
+            // waits on these loads; unsigned, so forming the address does not
+            // wait on the id either)
+            const int ie = t0 + 64 < n ? s + min(t0 + 64 + l, n - 1) : (nN > 0 ? sN + min(l, nN - 1) : s + n - 1);
+            const unsigned cN = (unsigned)ecell[ie];
 #pragma unroll
             for (int sub = 0; sub < 4; sub++) {
                 if (t0 + 16 * sub >= n) break;
@@ -741,8 +750,10 @@ __global__ __launch_bounds__(64) void k_learned_group(const int *__restrict__ ec
         }
         v[8] = __shfl(a4[0], 32, 64);
         apply_range1(mf, mn, v, v + 3, ctr, tperm, thi, tlo, sacc, phi, acc);
+        pre = n > 0 && nN > 0;
         s = sN;
         n = nN;
+        ctrL = ctrN;
     }
 }
 
