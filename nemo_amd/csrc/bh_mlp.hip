@@ -37,24 +37,27 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 // Activation: act(z) = z h(z) with the cubic sigmoid h = 1/2 + t (3/4 - t^2/4),
 // t = clamp(z/3, -1, 1): h rises from 0 at z = -3 to 1 at z = 3 with SiLU's
-// slope 1/4 at 0 and a continuous derivative (h' = (1 - t^2)/4).  Six
-// full-rate vector instructions (multiply, med3, three FMA/multiplies, the
-// product) where SiLU's exp and reciprocal run at a quarter of the rate.
-__device__ __forceinline__ float act_h(float z, float &t)
+// slope 1/4 at 0 and a continuous derivative (h' = (1 - t^2)/4).  Computed as
+// the smoothstep h = u^2 (3 - 2u) of u = (t + 1)/2 = clamp(z/6 + 1/2, 0, 1)
+// (the same polynomial; h' = u (1 - u)), whose clamp folds into the FMA's
+// output clamp: five full-rate vector instructions (FMA with clamp, FMA,
+// two multiplies, the product) where SiLU's exp and reciprocal run at a
+// quarter of the rate.  u is returned through the second argument.
+__device__ __forceinline__ float act_h(float z, float &u)
 {
-    t = __builtin_amdgcn_fmed3f(z * (1.f / 3.f), -1.f, 1.f);
-    return fmaf(t, fmaf(-0.25f, t * t, 0.75f), 0.5f);
+    u = __builtin_amdgcn_fmed3f(fmaf(z, 1.f / 6.f, 0.5f), 0.f, 1.f);
+    return (u * u) * fmaf(-2.f, u, 3.f);
 }
 __device__ __forceinline__ float act(float z)
 {
-    float t;
-    return z * act_h(z, t);
+    float u;
+    return z * act_h(z, u);
 }
 __device__ __forceinline__ float act_grad(float z)
 {
-    float t;
-    const float h = act_h(z, t);
-    return fmaf(z * 0.25f, fmaf(-t, t, 1.f), h);
+    float u;
+    const float h = act_h(z, u);
+    return fmaf(z, u * (1.f - u), h);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
