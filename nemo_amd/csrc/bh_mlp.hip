@@ -37,27 +37,34 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
 // Activation: act(z) = z h(z) with the cubic sigmoid h = 1/2 + t (3/4 - t^2/4),
 // t = clamp(z/3, -1, 1): h rises from 0 at z = -3 to 1 at z = 3 with SiLU's
-// slope 1/4 at 0 and a continuous derivative (h' = (1 - t^2)/4).  Computed as
-// the smoothstep h = u^2 (3 - 2u) of u = (t + 1)/2 = clamp(z/6 + 1/2, 0, 1)
-// (the same polynomial; h' = u (1 - u)), whose clamp folds into the FMA's
-// output clamp: five full-rate vector instructions (FMA with clamp, FMA,
-// two multiplies, the product) where SiLU's exp and reciprocal run at a
-// quarter of the rate.  u is returned through the second argument.
-__device__ __forceinline__ float act_h(float z, float &u)
+// slope 1/4 at 0 and a continuous derivative (h' = (1 - t^2)/4).  Six
+// full-rate vector instructions (multiply, med3, three FMA/multiplies, the
+// product) where SiLU's exp and reciprocal run at a quarter of the rate.
+// Training (forward and backward) and k_mlp_forward use this form.
+__device__ __forceinline__ float act_h(float z, float &t)
 {
-    u = __builtin_amdgcn_fmed3f(fmaf(z, 1.f / 6.f, 0.5f), 0.f, 1.f);
-    return (u * u) * fmaf(-2.f, u, 3.f);
+    t = __builtin_amdgcn_fmed3f(z * (1.f / 3.f), -1.f, 1.f);
+    return fmaf(t, fmaf(-0.25f, t * t, 0.75f), 0.5f);
 }
 __device__ __forceinline__ float act(float z)
 {
-    float u;
-    return z * act_h(z, u);
+    float t;
+    return z * act_h(z, t);
 }
 __device__ __forceinline__ float act_grad(float z)
 {
-    float u;
-    const float h = act_h(z, u);
-    return fmaf(z, u * (1.f - u), h);
+    float t;
+    const float h = act_h(z, t);
+    return fmaf(z * 0.25f, fmaf(-t, t, 1.f), h);
+}
+// The same polynomial as the smoothstep h = u^2 (3 - 2u) of u = (t + 1)/2 =
+// clamp(z/6 + 1/2, 0, 1), whose clamp folds into the FMA's output clamp:
+// five instructions.  The learned kernel's inference uses it; it rounds
+// differently from act() at the last bit of h.
+__device__ __forceinline__ float act_fast(float z)
+{
+    const float u = __builtin_amdgcn_fmed3f(fmaf(z, 1.f / 6.f, 0.5f), 0.f, 1.f);
+    return z * ((u * u) * fmaf(-2.f, u, 3.f));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
@@ -218,8 +225,8 @@ __device__ __forceinline__ void act_pack(const f32x4 h[4], bf16x8 hb[2])
     for (int s = 0; s < 2; s++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            hb[s][e] = (bf16)act(h[2 * s][e]);
-            hb[s][4 + e] = (bf16)act(h[2 * s + 1][e]);
+            hb[s][e] = (bf16)act_fast(h[2 * s][e]);
+            hb[s][4 + e] = (bf16)act_fast(h[2 * s + 1][e]);
         }
 }
 

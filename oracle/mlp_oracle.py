@@ -42,8 +42,8 @@ def unpack(p):
 
 def act(z):
     """z h(z), h = 1/2 + t (3/4 - t^2/4), t = clip(z/3, -1, 1) (the device's
-    activation, bh_mlp.hip act(), which evaluates the same polynomial as the
-    smoothstep u^2 (3 - 2u) of u = (t + 1)/2)."""
+    activation, bh_mlp.hip act(); its learned kernel evaluates the same
+    polynomial as the smoothstep u^2 (3 - 2u) of u = (t + 1)/2, act_fast())."""
     t = np.clip(z / 3.0, -1.0, 1.0)
     return z * (0.5 + t * (0.75 - 0.25 * t * t))
 

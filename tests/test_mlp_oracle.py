@@ -48,8 +48,9 @@ def test_activation_closed_forms():
     for v in np.concatenate([w, [-3.0, 3.0, -3.0 + 1e-9, 3.0 - 1e-9]]):
         fd = (M.act(v + h) - M.act(v - h)) / (2 * h)
         assert abs(fd - M.act_grad(v)) < 1e-6, (v, fd, M.act_grad(v))
-    # the device evaluates h as the smoothstep u^2 (3 - 2u), u = clip(z/6 + 1/2,
-    # 0, 1), and act_grad as h + z u (1 - u): the same functions
+    # the learned kernel evaluates h as the smoothstep u^2 (3 - 2u), u =
+    # clip(z/6 + 1/2, 0, 1), whose derivative gives h + z u (1 - u): the same
+    # functions
     u = np.clip(w / 6.0 + 0.5, 0.0, 1.0)
     assert np.allclose(w * u * u * (3.0 - 2.0 * u), M.act(w), rtol=0, atol=1e-13)
     assert np.allclose(u * u * (3.0 - 2.0 * u) + w * u * (1.0 - u), M.act_grad(w), rtol=0, atol=1e-13)
