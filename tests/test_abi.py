@@ -52,3 +52,19 @@ def test_create_without_device_reports_error():
     h = C.c_void_p()
     st = E.lib().bh_create(C.byref(h), 0)
     assert st == 2
+
+
+def test_every_hip_compile_disables_packed_fp32():
+    """Every hipcc compile in nemo_amd/Makefile takes $(COMMON), which turns
+    off the packed FP32 instructions; the MFMA register-form option is
+    limited to the MLP kernels' file."""
+    mk = open(os.path.join(ROOT, "nemo_amd", "Makefile")).read()
+    common = re.search(r"^COMMON\s*=\s*(.*?)(?<!\\)\n", mk, flags=re.S | re.M).group(1)
+    assert "-Xclang -target-feature -Xclang -packed-fp32-ops" in common
+    compiles = [l.strip() for l in mk.splitlines() if l.startswith("\t$(HIPCC)") and " -c " in l]
+    assert len(compiles) >= 6
+    for line in compiles:
+        assert "$(COMMON)" in line, line
+    rules = re.findall(r"^(\S+\.o):.*\n\t(.*)$", mk, flags=re.M)
+    with_vgpr = [obj for obj, cmd in rules if "$(MFMAVGPR)" in cmd]
+    assert with_vgpr == ["$(OBJDIR)/bh_mlp.o"], with_vgpr
